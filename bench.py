@@ -1,0 +1,153 @@
+"""Benchmark: GRHD cell-updates/s per RK4 step, plain FP64, one MI355X.
+
+Workload (the 128^3 plain-FP64-at-scale configuration): one 128^3 box with 3
+ghost cells on [-32, 32]^3 M (dx = 0.5 M), Kerr a = 0.9 M in Kerr-Schild
+coordinates, seeded Gaussian torus at R = 12 M with Keplerian rotation,
+Gamma = 4/3, minmod limiter, RK4 with dt = 0.25 dx.  A step is one full RK4
+step of the box (4 right-hand sides with ghost fill, reconstruction, HLLE,
+sources, and 4 recoveries with atmosphere resets); the state is resident in
+HBM before timing starts.
+
+Output: one JSON line with the throughput, a roofline record for the
+dominant kernel (its average launch time measured with HIP events on the
+launch stream during the timed steps) and a CPU baseline (the oracle -- a C
+restatement of the reference's CPU path with its call structure -- timed on
+this host's cores on one RK4 step of the same box).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("GRHD cell-updates/s per RK4 step, plain FP64 vs mixed/batched "
+          "at fixed error")
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per unit of each kernel class: every value the
+# kernel must read or write once, in FP64.
+#   faces_d: per face, the primitives of one cell (6 x 8 B) read, one flux
+#            (5 x 8 B) written.
+#   cells:   per interior cell, three face fluxes (each face is shared by
+#            two cells: 3 x 5 x 8 B), W (6 x 8), U_s, U0 and the RK
+#            accumulator read (3 x 5 x 8), U_next, the accumulator and W of
+#            the next stage written (5 x 8 + 5 x 8 + 6 x 8).
+#   prims:   per exterior cell: one state read, state + W written.
+ALG_BYTES = {"faces_x": 88, "faces_y": 88, "faces_z": 88,
+             "cells": 120 + 48 + 120 + 40 + 40 + 48, "prims": 40 + 40 + 48}
+
+
+def units_per_launch(cls, n, g):
+    if cls.startswith("faces"):
+        return (n + 1) * n * n
+    if cls == "cells":
+        return n ** 3
+    if cls == "prims":
+        return (n + 2 * g) ** 3 - n ** 3
+    return 0
+
+
+def cpu_baseline(w):
+    """Oracle RK4 step of the same box on this host's cores."""
+    import numpy as np
+
+    import oracle
+    O = oracle.Oracle(w.background, w.box, w.options)
+    st = np.ascontiguousarray(w.state).copy()
+    t0 = time.perf_counter()
+    O.advance_rk4(st, w.dt, 1)
+    t1 = time.perf_counter()
+    cells = w.box.num_interior
+    return {"value": cells / (t1 - t0), "unit": "cell-updates/s",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": "1 RK4 step of the same %d^3 box (%d interior cells), "
+                      "%.1f s wall, OpenMP over cells, reference call "
+                      "structure" % (w.box.interior_shape[0], cells, t1 - t0)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not "
+              "supported (the box would shard by z-slabs across GPUs, which "
+              "is not built)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from grdzhadzha_amd import workloads
+    from grdzhadzha_amd.operator import FixedBGLevelDataFiniteVolumeOperator
+
+    w = workloads.torus_box(n=args.n, nsteps=args.steps)
+    op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, w.options)
+    init = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
+    print("bench: %s, %d interior cells, dt %.4g; warmup %d, timed %d steps"
+          % (w.name, w.box.num_interior, w.dt, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+
+    st = init.clone()
+    if args.warmup:
+        op.advance_rk4(st, w.dt, args.warmup)
+    st = init.clone()
+    torch.cuda.synchronize()
+    op.ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    diag = op.advance_rk4(st, w.dt, args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing = op.ctx.timing()
+    op.ctx.set_timing(False)
+    elapsed = t1 - t0
+    cells = w.box.num_interior
+    value = cells * args.steps / elapsed
+    print("bench: %.3f s for %d steps; recoveries %s" % (elapsed, args.steps,
+                                                        diag),
+          file=sys.stderr, flush=True)
+    for k, (ms, n) in timing.items():
+        if n:
+            print("bench:   %-8s %9.3f ms total, %5d launches, %.4f ms avg"
+                  % (k, ms, n, ms / n), file=sys.stderr, flush=True)
+
+    dom = max((k for k in timing if k in ALG_BYTES),
+              key=lambda k: timing[k][0])
+    ms, nl = timing[dom]
+    avg_s = ms / nl / 1e3
+    units = units_per_launch(dom, args.n, w.box.ghosts)
+    achieved = ALG_BYTES[dom] * units / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_ms": ms / nl,
+                "alg_bytes_per_unit": ALG_BYTES[dom], "units_per_launch": units}
+
+    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": elapsed / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": w.name + " plain FP64 RK4 (one box, "
+                      "3 ghosts, minmod, dt = 0.25 dx)",
+                      "interior_cells": cells, "ghosts": w.box.ghosts,
+                      "dt": w.dt, "parallelism": "single GPU"},
+           "roofline": roofline}
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU oracle on one RK4 step ...",
+              file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,920 @@
+/*
+ * grhd_physics.hpp -- device-inline GRHD numerics for CDNA4 (gfx950).
+ *
+ * Per-cell and per-face arithmetic of GRDzhadzha's fixed-background GRHD
+ * path, in binary64, with the reference's operation order so that results
+ * (and the threshold decisions behind the atmosphere and failure masks)
+ * agree bit for bit with a CPU evaluation of the same formulas.  The whole
+ * translation unit is compiled with -ffp-contract=off: no multiply-add is
+ * fused, as on the CPU.  pow(x, -0.5), pow(x, 2), pow(x, 3) of the
+ * Kerr-Schild lapse are written 1/sqrt(x), x*x, x*x*x, and the natural log
+ * is grhd_log below, so only IEEE-exact operations (+ - * / sqrt) and
+ * frexp appear.
+ *
+ * Citations are to the reference repository (Source/...).
+ */
+#ifndef GRHD_PHYSICS_HPP_
+#define GRHD_PHYSICS_HPP_
+
+#include <hip/hip_runtime.h>
+
+namespace grhd
+{
+
+struct Prim
+{
+    double rho, eps, pressure, v[3];
+};
+
+struct Cons
+{
+    double D, S[3], tau;
+};
+
+/* CellGeometryData without K (StaticMetric.hpp:19-40); the flux and
+ * recovery paths never read K_ij. */
+struct Geo
+{
+    double g[3][3];
+    double gu[3][3];
+    double lapse;
+    double shift[3];
+};
+
+/* Geometry plus everything the static-metric sources read
+ * (StaticMetric.hpp:79-102, FixedBGMetric.hpp:17-55). */
+struct FullGeo
+{
+    Geo geo;
+    double K[3][3];
+    double dlapse[3];
+    double dlogsqrtdet[3];
+    double dg[3][3][3];    /* [dir][i][j] */
+    double dshift[3][3];   /* [i][dir]    */
+};
+
+/* Operator options passed by value to kernels. */
+struct Options
+{
+    double gamma;
+    double rho_floor, pressure_floor, max_v2, source_atmosphere_factor;
+    int max_iterations;
+    double tolerance, rec_pressure_floor, rec_max_v2;
+    double theta;
+    int use_reconstruction, use_static_sources, use_volume_sources;
+};
+
+/* A background evaluated at one time (background_factory(time)); the
+ * binary's positions and velocities come from the host (cos/sin there). */
+struct Background
+{
+    int kind; /* 0 = Kerr-Schild, 1 = circular binary PN */
+    double M, a;
+    double m1, m2, soft;
+    double p1[3], p2[3], u1[3], u2[3];
+};
+
+__device__ __forceinline__ double std_max(double a, double b)
+{
+    return (a < b) ? b : a;
+}
+__device__ __forceinline__ double std_min(double a, double b)
+{
+    return (b < a) ? b : a;
+}
+__device__ __forceinline__ double kdelta(int i, int j)
+{
+    return (i == j) ? 1.0 : 0.0;
+}
+
+/* ---------------- GRChombo TensorAlgebra (restated) ---------------- */
+__device__ __forceinline__ double det_sym(const double m[3][3])
+{
+    return m[0][0] * m[1][1] * m[2][2] + 2 * m[0][1] * m[0][2] * m[1][2] -
+           m[0][0] * m[1][2] * m[1][2] - m[1][1] * m[0][2] * m[0][2] -
+           m[2][2] * m[0][1] * m[0][1];
+}
+
+__device__ __forceinline__ void inverse_sym(const double m[3][3],
+                                            double out[3][3])
+{
+    const double deth = det_sym(m);
+    const double inv = 1. / deth;
+    out[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[1][2]) * inv;
+    out[0][1] = (m[0][2] * m[1][2] - m[0][1] * m[2][2]) * inv;
+    out[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv;
+    out[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[0][2]) * inv;
+    out[1][2] = (m[0][1] * m[0][2] - m[0][0] * m[1][2]) * inv;
+    out[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[0][1]) * inv;
+    out[1][0] = out[0][1];
+    out[2][0] = out[0][2];
+    out[2][1] = out[1][2];
+}
+
+/* IEEE-only natural log (k ln2 + 2 atanh((m-1)/(m+1))), ~1 ulp. */
+__device__ __forceinline__ double grhd_log(double x)
+{
+    if (!(x > 0.0) || x == __builtin_inf())
+    {
+        if (x == 0.0)
+            return -__builtin_inf();
+        if (x == __builtin_inf())
+            return __builtin_inf();
+        return __builtin_nan("");
+    }
+    int k;
+    double m = frexp(x, &k);
+    if (m < 0.70710678118654752440)
+    {
+        m = m * 2.0;
+        k = k - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 25.0;
+    p = 1.0 / 23.0 + s2 * p;
+    p = 1.0 / 21.0 + s2 * p;
+    p = 1.0 / 19.0 + s2 * p;
+    p = 1.0 / 17.0 + s2 * p;
+    p = 1.0 / 15.0 + s2 * p;
+    p = 1.0 / 13.0 + s2 * p;
+    p = 1.0 / 11.0 + s2 * p;
+    p = 1.0 / 9.0 + s2 * p;
+    p = 1.0 / 7.0 + s2 * p;
+    p = 1.0 / 5.0 + s2 * p;
+    p = 1.0 / 3.0 + s2 * p;
+    const double two_s = 2.0 * s;
+    const double log_m = two_s + two_s * (s2 * p);
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double dk = (double)k;
+    return (dk * ln2_hi + (log_m + dk * ln2_lo));
+}
+
+/* ---------------- Coordinates ---------------- */
+__device__ __forceinline__ void coords(int i, int j, int k, double dx,
+                                       const double c[3], double x[3])
+{
+    x[0] = (i + 0.5) * dx - c[0];
+    x[1] = (j + 0.5) * dx - c[1];
+    x[2] = (k + 0.5) * dx - c[2];
+}
+
+__device__ __forceinline__ double get_radius(const double x[3])
+{
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    return std_max(r, 1e-6);
+}
+
+/* ---------------- Kerr-Schild (KerrSchild.hpp:65-166) ---------------- */
+
+/* Radius, H and l_i shared by the geometry-only and full evaluations. */
+struct KSCore
+{
+    double rho, rho2, r2, r, cos_theta, H, el[3];
+};
+
+__device__ __forceinline__ void ks_core(double M, double a, const double x[3],
+                                        KSCore &c)
+{
+    const double a2 = a * a;
+    const double z = x[2];
+    c.rho = get_radius(x);
+    c.rho2 = c.rho * c.rho;
+    c.r2 = 0.5 * (c.rho2 - a2) +
+           sqrt(0.25 * (c.rho2 - a2) * (c.rho2 - a2) + a2 * z * z);
+    c.r = sqrt(c.r2);
+    c.cos_theta = z / c.r;
+    c.H = M * c.r / (c.r2 + a2 * c.cos_theta * c.cos_theta);
+    c.el[0] = (c.r * x[0] + a * x[1]) / (c.r2 + a2);
+    c.el[1] = (c.r * x[1] - a * x[0]) / (c.r2 + a2);
+    c.el[2] = z / c.r;
+}
+
+/* gamma, gamma^-1, lapse, shift only (derivatives are not needed by the
+ * flux/recovery paths and do not feed these values). */
+__device__ __forceinline__ void ks_geometry(double M, double a,
+                                            const double x[3], Geo &g)
+{
+    KSCore c;
+    ks_core(M, a, x, c);
+    const double el_t = 1.0;
+    g.lapse = 1.0 / sqrt(1.0 + 2.0 * c.H * el_t * el_t);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            g.g[i][j] = kdelta(i, j) + 2.0 * c.H * c.el[i] * c.el[j];
+    /* KerrSchild.hpp:108 inverse; FixedBGMetric.hpp:23-24 inverse again:
+     * both are the same function of the same gamma. */
+    inverse_sym(g.g, g.gu);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        g.shift[i] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            g.shift[i] += g.gu[i][j] * 2.0 * c.H * c.el[j] * el_t;
+    }
+}
+
+/* Full ADM evaluation converted to geometry + source derivatives
+ * (KerrSchild.hpp:65-242 + FixedBGMetric.hpp:17-55). */
+__device__ __forceinline__ void ks_full(double M, double a, const double xc[3],
+                                        FullGeo &f)
+{
+    const double a2 = a * a;
+    KSCore c;
+    ks_core(M, a, xc, c);
+    const double z = xc[2];
+    const double el_t = 1.0;
+    const double H = c.H;
+    const double r = c.r, r2 = c.r2, rho = c.rho, rho2 = c.rho2;
+
+    /* get_KS_derivs (KerrSchild.hpp:171-242) */
+    const double cos_theta2 = c.cos_theta * c.cos_theta;
+    double drdx[3], dHdx[3], dldx[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const double drhodx = xc[i] / rho;
+        drdx[i] = 0.5 / r *
+                  (rho * drhodx +
+                   0.5 / sqrt(0.25 * (rho2 - a2) * (rho2 - a2) + a2 * z * z) *
+                       (drhodx * rho * (rho2 - a2) +
+                        kdelta(i, 2) * 2.0 * a2 * z));
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const double dcosthetadx = -z / r2 * drdx[i] + kdelta(i, 2) / r;
+        dHdx[i] = H * (drdx[i] / r -
+                       2.0 / (r2 + a2 * cos_theta2) *
+                           (r * drdx[i] + a2 * c.cos_theta * dcosthetadx));
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        dldx[0][i] =
+            (xc[0] * drdx[i] + r * kdelta(i, 0) + a * kdelta(i, 1) -
+             2.0 * r * drdx[i] * (r * xc[0] + a * xc[1]) / (r2 + a2)) /
+            (r2 + a2);
+        dldx[1][i] =
+            (xc[1] * drdx[i] + r * kdelta(i, 1) - a * kdelta(i, 0) -
+             2.0 * r * drdx[i] * (r * xc[1] - a * xc[0]) / (r2 + a2)) /
+            (r2 + a2);
+        dldx[2][i] = -xc[2] * drdx[i] / r2 + kdelta(i, 2) / r;
+    }
+    const double dltdx = 0.0;
+
+    Geo &g = f.geo;
+    g.lapse = 1.0 / sqrt(1.0 + 2.0 * H * el_t * el_t);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            g.g[i][j] = kdelta(i, j) + 2.0 * H * c.el[i] * c.el[j];
+    inverse_sym(g.g, g.gu);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        g.shift[i] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            g.shift[i] += g.gu[i][j] * 2.0 * H * c.el[j] * el_t;
+    }
+
+    /* d1_gamma[i][j][k] stored as dg[k][i][j] */
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                f.dg[k][i][j] =
+                    2.0 * (c.el[i] * c.el[j] * dHdx[k] +
+                           H * c.el[i] * dldx[j][k] + H * c.el[j] * dldx[i][k]);
+
+    const double lapse2 = g.lapse * g.lapse;
+    const double lapse3 = g.lapse * g.lapse * g.lapse;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        f.dlapse[i] = -lapse3 * el_t * (el_t * dHdx[i] + 2.0 * H * dltdx);
+
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            f.dshift[i][j] = 2.0 * el_t * dHdx[j] * lapse2 * c.el[i] +
+                             4.0 * el_t * H * g.lapse * f.dlapse[j] * c.el[i] +
+                             2.0 * el_t * H * lapse2 * dldx[i][j] +
+                             2.0 * dltdx * H * lapse2 * c.el[i];
+
+    /* K_ij with the Christoffel symbols formed per (i, j)
+     * (GRChombo compute_christoffel, KerrSchild.hpp:146-164) */
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            double LLL[3];
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+                LLL[l] = 0.5 * (f.dg[j][i][l] + f.dg[i][j][l] - f.dg[l][i][j]);
+            double Kij = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                double ULL = 0;
+#pragma unroll
+                for (int l = 0; l < 3; ++l)
+                    ULL += g.gu[k][l] * LLL[l];
+                Kij += g.g[k][j] * f.dshift[k][i] + g.g[k][i] * f.dshift[k][j] +
+                       (f.dg[j][k][i] + f.dg[i][k][j]) * g.shift[k];
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    Kij += -2.0 * ULL * g.g[k][m] * g.shift[m];
+            }
+            Kij *= 0.5 / g.lapse;
+            f.K[i][j] = Kij;
+        }
+
+    /* make_static_metric_derivatives_from_adm (FixedBGMetric.hpp:32-55) */
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                s += 0.5 * g.gu[i][j] * f.dg[d][i][j];
+        f.dlogsqrtdet[d] = s;
+    }
+}
+
+/* ---------------- Circular binary PN (CircularBinaryPN.hpp) ---------------- */
+struct Potentials
+{
+    double scalar, scalar_dt, dscalar[3], vec[3], dvec[3][3];
+};
+
+__device__ __forceinline__ void pn_add_point(Potentials &p, double mass,
+                                             double soft, const double src[3],
+                                             const double vel[3],
+                                             const double pos[3])
+{
+    double disp[3];
+    double r2 = soft * soft;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        disp[i] = pos[i] - src[i];
+        r2 += disp[i] * disp[i];
+    }
+    const double ir = 1.0 / sqrt(r2);
+    const double ir3 = ir * ir * ir;
+    p.scalar += mass * ir;
+    double ddv = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        ddv += disp[i] * vel[i];
+    p.scalar_dt += mass * ddv * ir3;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        p.dscalar[d] += -mass * disp[d] * ir3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            p.dvec[c][d] += -mass * vel[c] * disp[d] * ir3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        p.vec[c] += mass * vel[c] * ir;
+}
+
+__device__ __forceinline__ void pn_potentials(const Background &bg,
+                                              const double x[3],
+                                              Potentials &p)
+{
+    p.scalar = 0.0;
+    p.scalar_dt = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        p.dscalar[i] = 0.0;
+        p.vec[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            p.dvec[i][j] = 0.0;
+    }
+    pn_add_point(p, bg.m1, bg.soft, bg.p1, bg.u1, x);
+    pn_add_point(p, bg.m2, bg.soft, bg.p2, bg.u2, x);
+}
+
+__device__ __forceinline__ void pn_geometry(const Background &bg,
+                                            const double x[3], Geo &g)
+{
+    Potentials p;
+    pn_potentials(bg, x, p);
+    const double U = p.scalar;
+    const double cf = 1.0 + 2.0 * U;
+    double vsq = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        vsq += p.vec[i] * p.vec[i];
+    const double lapse_squared = 1.0 - 2.0 * U + 2.0 * U * U + 16.0 * vsq / cf;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            g.g[i][j] = 0.0;
+    g.lapse = sqrt(lapse_squared);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        g.g[i][i] = cf;
+        g.shift[i] = -4.0 * p.vec[i] / cf;
+    }
+    inverse_sym(g.g, g.gu);
+}
+
+__device__ __forceinline__ void pn_full(const Background &bg,
+                                        const double x[3], FullGeo &f)
+{
+    Potentials p;
+    pn_potentials(bg, x, p);
+    Geo &g = f.geo;
+    const double U = p.scalar;
+    const double dcf_dt = 2.0 * p.scalar_dt;
+    const double cf = 1.0 + 2.0 * U;
+    double vsq = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        vsq += p.vec[i] * p.vec[i];
+    const double lapse_squared = 1.0 - 2.0 * U + 2.0 * U * U + 16.0 * vsq / cf;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        f.dlapse[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            g.g[i][j] = 0.0;
+            f.dshift[i][j] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                f.dg[k][i][j] = 0.0;
+        }
+    }
+    g.lapse = sqrt(lapse_squared);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        g.g[i][i] = cf;
+        g.shift[i] = -4.0 * p.vec[i] / cf;
+    }
+    inverse_sym(g.g, g.gu);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const double dU = p.dscalar[d];
+        const double dcf = 2.0 * dU;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            f.dg[d][m][m] = dcf;
+        double dvsq = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            dvsq += 2.0 * p.vec[c] * p.dvec[c][d];
+            f.dshift[c][d] =
+                -4.0 * (p.dvec[c][d] / cf - p.vec[c] * dcf / (cf * cf));
+        }
+        const double dls = (-2.0 + 4.0 * U) * dU +
+                           16.0 * (dvsq / cf - vsq * dcf / (cf * cf));
+        f.dlapse[d] = 0.5 * dls / g.lapse;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            double LLL[3];
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+                LLL[l] = 0.5 * (f.dg[j][i][l] + f.dg[i][j][l] - f.dg[l][i][j]);
+            double Kij = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                double ULL = 0;
+#pragma unroll
+                for (int l = 0; l < 3; ++l)
+                    ULL += g.gu[k][l] * LLL[l];
+                Kij += g.g[k][j] * f.dshift[k][i] + g.g[k][i] * f.dshift[k][j] +
+                       (f.dg[j][k][i] + f.dg[i][k][j]) * g.shift[k];
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    Kij += -2.0 * ULL * g.g[k][m] * g.shift[m];
+            }
+            if (i == j)
+                Kij -= dcf_dt;
+            Kij *= 0.5 / g.lapse;
+            f.K[i][j] = Kij;
+        }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                s += 0.5 * g.gu[i][j] * f.dg[d][i][j];
+        f.dlogsqrtdet[d] = s;
+    }
+}
+
+__device__ __forceinline__ void cell_geometry(const Background &bg,
+                                              const double x[3], Geo &g)
+{
+    if (bg.kind == 1)
+        pn_geometry(bg, x, g);
+    else
+        ks_geometry(bg.M, bg.a, x, g);
+}
+
+__device__ __forceinline__ void cell_full(const Background &bg,
+                                          const double x[3], FullGeo &f)
+{
+    if (bg.kind == 1)
+        pn_full(bg, x, f);
+    else
+        ks_full(bg.M, bg.a, x, f);
+}
+
+/* log sqrt det gamma (StaticMetric.hpp:293-302) */
+__device__ __forceinline__ double log_sqrt_det(const Geo &g)
+{
+    const double det = det_sym(g.g);
+    if (det <= 0.0 || !isfinite(det))
+        return 0.0;
+    return 0.5 * grhd_log(det);
+}
+
+/* ---------------- Valencia (Valencia.hpp) ---------------- */
+__device__ __forceinline__ double vsq(const double v[3], const double g[3][3])
+{
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            s += g[i][j] * v[i] * v[j];
+    return s;
+}
+
+/* compute_conserved (Valencia.hpp:55-73) */
+__device__ __forceinline__ void prim2con(const Prim &p, double gamma,
+                                         const double g[3][3], Cons &c)
+{
+    double vL[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        vL[i] = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            vL[i] += g[i][j] * p.v[j];
+    }
+    const double W = 1.0 / sqrt(1.0 - vsq(p.v, g));
+    const double h = 1.0 + p.eps + p.pressure / p.rho;
+    const double rhW2 = p.rho * h * W * W;
+    c.D = p.rho * W;
+    c.tau = rhW2 - p.pressure - c.D;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        c.S[i] = rhW2 * vL[i];
+}
+
+__device__ __forceinline__ double presid(double p, double D, double tau,
+                                         double S2, double gamma,
+                                         double max_v2)
+{
+    const double epp = tau + D + p;
+    double v2 = S2 / (epp * epp);
+    if (v2 > max_v2)
+        v2 = max_v2;
+    const double W = 1.0 / sqrt(1.0 - v2);
+    return D * W + gamma * p * W * W / (gamma - 1.0) - epp;
+}
+
+__device__ __forceinline__ double presid_deriv(double p, double D, double tau,
+                                               double S2, double gamma,
+                                               double max_v2)
+{
+    const double epp = tau + D + p;
+    double v2 = S2 / (epp * epp);
+    if (v2 > max_v2)
+        v2 = max_v2;
+    const double W = 1.0 / sqrt(1.0 - v2);
+    const double dW_dp = -W * W * W * v2 / epp;
+    const double dW2_dp = 2.0 * W * dW_dp;
+    return D * dW_dp + gamma * (W * W + p * dW2_dp) / (gamma - 1.0) - 1.0;
+}
+
+/* recover_primitive (Valencia.hpp:116-228).  Returns success; `residual`
+ * receives the final residual (0 for the early D <= 0 / bracket exits). */
+__device__ __forceinline__ bool con2prim(const Cons &c, const Options &o,
+                                         const double gu[3][3], Prim &out,
+                                         double &residual_out)
+{
+    residual_out = 0.0;
+    if (c.D <= 0.0)
+        return false;
+    const double gamma = o.gamma;
+    double S2 = 0.;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+            S2 += gu[m][n] * c.S[m] * c.S[n];
+    const double mv2 = o.rec_max_v2;
+    double lower = o.rec_pressure_floor;
+    const double f_lower = presid(lower, c.D, c.tau, S2, gamma, mv2);
+    if (f_lower >= 0.0 && fabs(f_lower) > o.tolerance)
+        return false;
+    double upper = lower;
+    if (f_lower < 0.0)
+    {
+        upper = std_max(1.0, fabs(c.tau) + c.D);
+        double f_upper = presid(upper, c.D, c.tau, S2, gamma, mv2);
+        int it = 0;
+        while (f_upper <= 0.0 && it < o.max_iterations)
+        {
+            upper *= 2.0;
+            f_upper = presid(upper, c.D, c.tau, S2, gamma, mv2);
+            ++it;
+        }
+        if (f_upper <= 0.0)
+            return false;
+    }
+    double p = (fabs(f_lower) <= o.tolerance)
+                   ? lower
+                   : std_min(std_max(c.tau * (gamma - 1.0), lower), upper);
+    double res = presid(p, c.D, c.tau, S2, gamma, mv2);
+    for (int iter = 0; iter < o.max_iterations; ++iter)
+    {
+        if (fabs(res) < o.tolerance)
+            break;
+        if (f_lower < 0.0)
+        {
+            if (res > 0.0)
+                upper = p;
+            else
+                lower = p;
+        }
+        const double der = presid_deriv(p, c.D, c.tau, S2, gamma, mv2);
+        double np = p - res / der;
+        if (!isfinite(np) || np <= lower || np >= upper)
+            np = 0.5 * (lower + upper);
+        p = np;
+        res = presid(p, c.D, c.tau, S2, gamma, mv2);
+    }
+    residual_out = res;
+    if (fabs(res) >= o.tolerance)
+        return false;
+    const double epp = c.tau + c.D + p;
+    const double v2 = S2 / (epp * epp);
+    if (v2 >= mv2)
+        return false;
+    const double W = 1.0 / sqrt(1.0 - v2);
+    out.rho = c.D / W;
+    out.pressure = p;
+    out.eps = p / ((gamma - 1.0) * out.rho);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        double vi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            vi += gu[i][j] * c.S[j] / epp;
+        out.v[i] = vi;
+    }
+    return true;
+}
+
+/* compute_flux (Valencia.hpp:230-249) */
+__device__ __forceinline__ void flux_of(const Prim &p, const Cons &c, int d,
+                                        double lapse, const double shift[3],
+                                        Cons &f)
+{
+    const double tv = lapse * p.v[d] - shift[d];
+    f.D = c.D * tv;
+    f.tau = c.tau * tv + lapse * p.pressure * p.v[d];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+        f.S[j] = c.S[j] * tv;
+        if (j == d)
+            f.S[j] += lapse * p.pressure;
+    }
+}
+
+/* ---------------- Atmosphere (Atmosphere.hpp:24-60) ---------------- */
+__device__ __forceinline__ void floors(Prim &p, const Options &o,
+                                       const double g[3][3])
+{
+    p.rho = std_max(p.rho, o.rho_floor);
+    p.pressure = std_max(p.pressure, o.pressure_floor);
+    const double v2 = vsq(p.v, g);
+    if (v2 >= o.max_v2)
+    {
+        const double scale = sqrt(o.max_v2 / v2);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            p.v[i] *= scale;
+    }
+    p.eps = p.pressure / ((o.gamma - 1.0) * p.rho);
+}
+
+__device__ __forceinline__ void atmosphere_prim(const Options &o, Prim &p)
+{
+    p.rho = o.rho_floor;
+    p.pressure = o.pressure_floor;
+    p.eps = p.pressure / ((o.gamma - 1.0) * p.rho);
+    p.v[0] = p.v[1] = p.v[2] = 0.0;
+}
+
+/* FArrayBoxFiniteVolume.hpp:118-126 */
+__device__ __forceinline__ bool is_atmosphere(const Prim &p, const Options &o,
+                                              double threshold_factor)
+{
+    const double factor = std_max(1.0, threshold_factor);
+    return p.rho <= factor * o.rho_floor &&
+           p.pressure <= factor * o.pressure_floor;
+}
+
+/* FArrayBoxFiniteVolume.hpp:704-711 */
+__device__ __forceinline__ bool prim_differs(const Prim &a, const Prim &b)
+{
+    return a.rho != b.rho || a.eps != b.eps || a.pressure != b.pressure ||
+           a.v[0] != b.v[0] || a.v[1] != b.v[1] || a.v[2] != b.v[2];
+}
+
+/* ---------------- Reconstruction (Reconstruction.hpp:16-69) ---------------- */
+__device__ __forceinline__ double minmod2(double l, double r)
+{
+    if (l * r <= 0.0)
+        return 0.0;
+    return l > 0.0 ? std_min(l, r) : std_max(l, r);
+}
+
+__device__ __forceinline__ double limited_slope(double l, double c, double r,
+                                                double theta)
+{
+    return minmod2(theta * (c - l), minmod2(0.5 * (r - l), theta * (r - c)));
+}
+
+/* ---------------- Riemann solver (RiemannSolvers.hpp:23-118) ---------------- */
+__device__ __forceinline__ void char_speeds(const Prim &p, double gamma,
+                                            const Geo &g, int d,
+                                            double &lower, double &upper)
+{
+    const double v2 = vsq(p.v, g.g);
+    const double h = 1.0 + p.eps + p.pressure / p.rho;
+    const double cs2 = gamma * p.pressure / (p.rho * h);
+    const double cs = sqrt(cs2);
+    const double vel = p.v[d];
+    const double den = 1.0 - v2 * cs2;
+    const double rad =
+        (1.0 - v2) * (g.gu[d][d] * (1.0 - v2 * cs2) - vel * vel * (1.0 - cs2));
+    const double root = cs * sqrt(std_max(0.0, rad));
+    const double lo = (vel * (1.0 - cs2) - root) / den;
+    const double hi = (vel * (1.0 - cs2) + root) / den;
+    lower = g.lapse * lo - g.shift[d];
+    upper = g.lapse * hi - g.shift[d];
+}
+
+/* compute_muscl_hlle_flux (FiniteVolume.hpp:63-98) after the MUSCL step:
+ * floors, prim2con and HLLE on the face geometry. */
+__device__ __forceinline__ void hlle_face(Prim lp, Prim rp, const Options &o,
+                                          const Geo &face, int d, Cons &f)
+{
+    floors(lp, o, face.g);
+    floors(rp, o, face.g);
+    Cons lc, rc;
+    prim2con(lp, o.gamma, face.g, lc);
+    prim2con(rp, o.gamma, face.g, rc);
+    Cons fl, fr;
+    flux_of(lp, lc, d, face.lapse, face.shift, fl);
+    flux_of(rp, rc, d, face.lapse, face.shift, fr);
+    double ll, lu, rl, ru;
+    char_speeds(lp, o.gamma, face, d, ll, lu);
+    char_speeds(rp, o.gamma, face, d, rl, ru);
+    const double sl = std_min(0.0, std_min(ll, rl));
+    const double su = std_max(0.0, std_max(lu, ru));
+    if (sl >= 0.0)
+    {
+        f = fl;
+        return;
+    }
+    if (su <= 0.0)
+    {
+        f = fr;
+        return;
+    }
+    const double inv = 1.0 / (su - sl);
+    f.D = (su * fl.D - sl * fr.D + sl * su * (rc.D - lc.D)) * inv;
+    f.tau = (su * fl.tau - sl * fr.tau + sl * su * (rc.tau - lc.tau)) * inv;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        f.S[i] = (su * fl.S[i] - sl * fr.S[i] + sl * su * (rc.S[i] - lc.S[i])) *
+                 inv;
+}
+
+/* average_geometry (FArrayBoxFiniteVolume.hpp:194-216), K omitted */
+__device__ __forceinline__ void average_geo(const Geo &l, const Geo &r,
+                                            Geo &out)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            out.g[i][j] = 0.5 * (l.g[i][j] + r.g[i][j]);
+            out.gu[i][j] = 0.5 * (l.gu[i][j] + r.gu[i][j]);
+        }
+    out.lapse = 0.5 * (l.lapse + r.lapse);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        out.shift[i] = 0.5 * (l.shift[i] + r.shift[i]);
+}
+
+/* ---------------- Static-metric sources (StaticMetric.hpp:193-275) ------- */
+__device__ __forceinline__ void static_sources(const Prim &p, const Cons &c,
+                                               double gamma, const FullGeo &f,
+                                               Cons &src)
+{
+    const Geo &g = f.geo;
+    const double W = 1.0 / sqrt(1.0 - vsq(p.v, g.g));
+    const double h = 1.0 + p.eps + p.pressure / p.rho;
+    const double rhW2 = p.rho * h * W * W;
+    double T[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            T[i][j] = rhW2 * p.v[i] * p.v[j] + p.pressure * g.gu[i][j];
+    const double E = c.D + c.tau;
+    double mU[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        mU[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            mU[i] += g.gu[i][j] * c.S[j];
+    Cons cf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        flux_of(p, c, d, g.lapse, g.shift, cf[d]);
+    src.D = 0.0;
+    src.tau = 0.0;
+    src.S[0] = src.S[1] = src.S[2] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        src.D -= cf[d].D * f.dlogsqrtdet[d];
+        src.tau -= cf[d].tau * f.dlogsqrtdet[d];
+        src.tau -= mU[d] * f.dlapse[d];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            src.tau += g.lapse * T[i][j] * f.K[i][j];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+    {
+        double ms = -E * f.dlapse[m];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                ms += 0.5 * g.lapse * T[i][j] * f.dg[m][i][j];
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            ms += c.S[s] * f.dshift[s][m];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            ms -= cf[d].S[m] * f.dlogsqrtdet[d];
+        src.S[m] += ms;
+    }
+}
+
+} // namespace grhd
+
+#endif /* GRHD_PHYSICS_HPP_ */

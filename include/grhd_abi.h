@@ -1,0 +1,304 @@
+/*
+ * grhd_abi.h -- C-ABI of the MI355X fixed-background GRHD operator.
+ *
+ * One call = one FArrayBox (one box of a level).  Every array argument is a
+ * DEVICE pointer into HBM laid out exactly like a Chombo FArrayBox:
+ * component-major, Fortran order, i fastest:
+ *
+ *     fab[c * npts + ((k - glo[2]) * ny + (j - glo[1])) * nx + (i - glo[0])]
+ *
+ * with nx = ghi[0] - glo[0] + 1 (etc.) and npts = nx * ny * nz.  The eleven
+ * GRHD components sit at comp_base + {0..10} in the order of the reference's
+ * UserVariables (Tests/GRHDFixedBGTest/UserVariables.hpp:24-40):
+ *     D, S1, S2, S3, tau, rho, eps, pressure, v1, v2, v3.
+ * The rhs FAB covers the interior box only (the reference allocates it with
+ * IntVect::Zero ghosts) and uses the same component numbering.
+ *
+ * Calls are stream-ordered on `stream` (a hipStream_t, NULL = default
+ * stream).  Every entry point that returns a status waits for its own work
+ * to finish (the reference is synchronous and reports failures by throwing);
+ * grhd_advance_rk4_box runs `nsteps` steps and waits once at the end.
+ * The caller owns every buffer; scratch space lives in a grhd_ctx and is
+ * reused across calls.  A grhd_ctx must not be used from two host threads at
+ * once.  All arithmetic is IEEE binary64 without contraction, following the
+ * reference's operation order.
+ *
+ * Return value: GRHD_OK (0) or one of the GRHD_ERR_* codes below; on a
+ * recovery error `err` (if non-NULL) names the first failing cell in the
+ * reference's BoxIterator order and the con2prim residual there.
+ */
+#ifndef GRHD_ABI_H_
+#define GRHD_ABI_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of GRHD components starting at comp_base. */
+#define GRHD_NUM_COMPS 11
+enum
+{
+    GRHD_D = 0,
+    GRHD_S1 = 1,
+    GRHD_S2 = 2,
+    GRHD_S3 = 3,
+    GRHD_TAU = 4,
+    GRHD_RHO = 5,
+    GRHD_EPS = 6,
+    GRHD_PRESSURE = 7,
+    GRHD_V1 = 8,
+    GRHD_V2 = 9,
+    GRHD_V3 = 10
+};
+
+/* Status codes.  The recovery codes map onto the std::runtime_error the
+ * reference throws at FixedBGFArrayBoxFiniteVolume.hpp:157-164 (ghost fill),
+ * :48-55 (RHS primitive recovery), :272-279 (metric sources) and :383-390
+ * (CFL speed); GRHD_ERR_INVALID_ARGUMENT maps onto std::invalid_argument
+ * (:549-550) and MayDay::Error for bad background parameters
+ * (KerrSchild.hpp:41-46, CircularBinaryPN.hpp:52-56). */
+enum
+{
+    GRHD_OK = 0,
+    GRHD_ERR_RECOVERY_GHOST_FILL = 1,
+    GRHD_ERR_RECOVERY_RHS = 2,
+    GRHD_ERR_RECOVERY_SOURCES = 3,
+    GRHD_ERR_RECOVERY_CFL = 4,
+    GRHD_ERR_INVALID_ARGUMENT = 5,
+    GRHD_ERR_DEVICE = 6
+};
+
+/* Background kinds (Source/Background/KerrSchild.hpp,
+ * Source/Background/CircularBinaryPN.hpp). */
+enum
+{
+    GRHD_BG_KERR_SCHILD = 0,
+    GRHD_BG_CIRCULAR_BINARY_PN = 1
+};
+
+/* Per-cell recovery mask bits (uint8 per interior cell). */
+enum
+{
+    GRHD_MASK_FAILED = 1,          /* con2prim failed -> atmosphere      */
+    GRHD_MASK_FLOORED = 2,         /* floors changed the primitive       */
+    GRHD_MASK_RESET = 4,           /* conserved variables were rewritten */
+    GRHD_MASK_SOURCE_ATMOSPHERE = 8 /* metric sources skipped (RHS)      */
+};
+
+/* Geometry of one box of a level: Chombo's grids[dit] (interior),
+ * state[dit].box() (FAB storage), ProblemDomain, and the Coordinates
+ * convention x = (i + 1/2) dx - center (GRChombo Coordinates). */
+typedef struct grhd_box
+{
+    int lo[3], hi[3];   /* interior (valid) box, inclusive      */
+    int glo[3], ghi[3]; /* FAB storage box, inclusive (ghosted) */
+    int dlo[3], dhi[3]; /* ProblemDomain box, inclusive         */
+    int periodic[3];    /* ProblemDomain::isPeriodic(dir)       */
+    double dx;
+    double center[3];
+} grhd_box;
+
+/* IdealGasEOS + AtmosphereOptions + RecoveryOptions + operator flags
+ * (IdealGasEOS.hpp:10-53, Atmosphere.hpp:16-22, Valencia.hpp:17-23,
+ * FixedBGFArrayBoxFiniteVolume.hpp:450-464 and :681-699). */
+typedef struct grhd_params
+{
+    double adiabatic_index;
+    /* AtmosphereOptions */
+    double rho_floor;
+    double pressure_floor;
+    double max_velocity_squared;
+    double metric_source_atmosphere_factor;
+    /* RecoveryOptions */
+    int max_iterations;
+    double tolerance;
+    double recovery_pressure_floor;
+    double recovery_max_velocity_squared;
+    /* operator */
+    double limiter_theta;
+    int use_reconstruction;
+    int use_static_metric_sources;
+    int use_metric_volume_time_sources;
+    double metric_time_derivative_step;
+} grhd_params;
+
+/* Analytic background.  For the binary, the orbital time at a call is
+ * `time` (the field) plus the call's `time` argument, which is what the
+ * reference's time factory does (GRHDFixedBGTest.cpp:435-446). */
+typedef struct grhd_background
+{
+    int kind;
+    /* KerrSchild::params_t */
+    double mass;
+    double spin;
+    /* CircularBinaryPN::params_t */
+    double mass_1;
+    double mass_2;
+    double separation;
+    double angular_velocity;
+    double phase;
+    double time;
+    double softening_radius;
+} grhd_background;
+
+/* AtmosphereResetDiagnostics (FArrayBoxFiniteVolume.hpp:78-92). */
+typedef struct grhd_diagnostics
+{
+    long long num_cells;
+    long long num_failed_recoveries;
+    long long num_floored_primitives;
+    long long num_conserved_resets;
+} grhd_diagnostics;
+
+/* Where and why a call failed. */
+typedef struct grhd_error
+{
+    int code;
+    int iv[3];
+    double residual;
+} grhd_error;
+
+/* Opaque scratch/stream context (device scratch buffers, status words). */
+typedef struct grhd_ctx grhd_ctx;
+
+/* Create / destroy a context on the current HIP device.  No reference
+ * counterpart: the reference allocates its temporaries per call
+ * (FixedBGFArrayBoxFiniteVolume.hpp:214-217, :569-573). */
+int grhd_ctx_create(grhd_ctx **out);
+void grhd_ctx_destroy(grhd_ctx *ctx);
+
+/* RHS of one box: flux divergence over the three directional sweeps plus
+ * the static-metric sources (and, if enabled, the metric volume-time
+ * sources).  Mutates the physical ghost cells of `state` exactly as the
+ * reference does; overwrites the interior of `rhs`.
+ * Replaces compute_leveldata_flux_rhs_from_fixed_background
+ * (FixedBGFArrayBoxFiniteVolume.hpp:322-363) for one box, as called by
+ * FixedBGLevelDataFiniteVolumeOperator::compute_rhs (:468-477) and
+ * TimeDependentFixedBGLevelDataFiniteVolumeOperator::compute_rhs (:702-720),
+ * including add_metric_volume_time_sources_... (:633-657).  `source_mask`
+ * (nullable, uint8 per interior cell) receives GRHD_MASK_SOURCE_ATMOSPHERE
+ * where the sources were skipped (:284-285). */
+int grhd_compute_rhs_box(grhd_ctx *ctx, const grhd_box *box,
+                         const grhd_params *params,
+                         const grhd_background *background, double time,
+                         double *state, int ncomp, int comp_base, double *rhs,
+                         unsigned char *source_mask, grhd_error *err,
+                         void *stream);
+
+/* Post-update primitive recovery with atmosphere resets over the interior.
+ * Replaces recover_primitives_and_apply_atmosphere_from_fixed_background
+ * (FixedBGFArrayBoxFiniteVolume.hpp:60-110), as called by
+ * recover_primitives (:506-520, :728-743).  `mask` (nullable) receives the
+ * GRHD_MASK_FAILED/FLOORED/RESET bits per interior cell. */
+int grhd_recover_primitives_box(grhd_ctx *ctx, const grhd_box *box,
+                                const grhd_params *params,
+                                const grhd_background *background,
+                                double time, double *state, int ncomp,
+                                int comp_base, unsigned char *mask,
+                                grhd_diagnostics *diagnostics, void *stream);
+
+/* state += dt * rhs over the interior conserved components, then (if
+ * recover != 0) recovery at recovery_time.  Replaces update_conserved
+ * (FixedBGFArrayBoxFiniteVolume.hpp:522-533, :745-757) for one box, built on
+ * add_scaled_conserved_rhs (FArrayBoxFiniteVolume.hpp:927-940). */
+int grhd_update_conserved_box(grhd_ctx *ctx, const grhd_box *box,
+                              const grhd_params *params,
+                              const grhd_background *background,
+                              double *state, int ncomp, int comp_base,
+                              const double *rhs, double dt, int recover,
+                              double recovery_time, unsigned char *mask,
+                              grhd_diagnostics *diagnostics, void *stream);
+
+/* Maximum over non-atmosphere interior cells of sum_d max|lambda|/dx.
+ * Replaces compute_max_inverse_dt_from_fixed_background
+ * (FixedBGFArrayBoxFiniteVolume.hpp:364-412). */
+int grhd_compute_max_inverse_dt_box(grhd_ctx *ctx, const grhd_box *box,
+                                    const grhd_params *params,
+                                    const grhd_background *background,
+                                    double time, const double *state,
+                                    int ncomp, int comp_base,
+                                    double *max_inverse_dt, grhd_error *err,
+                                    void *stream);
+
+/* cfl / max(max_inverse_dt, 1e-14); GRHD_ERR_INVALID_ARGUMENT if cfl <= 0.
+ * Replaces compute_stable_dt (FixedBGFArrayBoxFiniteVolume.hpp:544-555,
+ * :769-780). */
+int grhd_compute_stable_dt_box(grhd_ctx *ctx, const grhd_box *box,
+                               const grhd_params *params,
+                               const grhd_background *background, double time,
+                               const double *state, int ncomp, int comp_base,
+                               double cfl, double *stable_dt, grhd_error *err,
+                               void *stream);
+
+/* One SSPRK2 step, the reference's integrator:
+ * U1 = U + dt L(U); recover(U1, t+dt); U = (U + U1 + dt L(U1)) / 2;
+ * recover(U, t+dt).  Replaces advance_ssprk2
+ * (FixedBGFArrayBoxFiniteVolume.hpp:563-589, :782-809).  `diagnostics`
+ * (nullable) receives the final recovery's counts, `mask` its bits. */
+int grhd_advance_ssprk2_box(grhd_ctx *ctx, const grhd_box *box,
+                            const grhd_params *params,
+                            const grhd_background *background, double time,
+                            double dt, double *state, int ncomp,
+                            int comp_base, unsigned char *mask,
+                            grhd_diagnostics *diagnostics, grhd_error *err,
+                            void *stream);
+
+/* `nsteps` classical RK4 steps of size dt starting at `time`.  Stage states
+ * U0 + (dt/2) k1, U0 + (dt/2) k2, U0 + dt k3 and the result
+ * ((((U0 + dt/6 k1) + dt/3 k2) + dt/3 k3) + dt/6 k4) are each recovered with
+ * atmosphere resets.  No counterpart in the reference, which integrates GRHD
+ * only with SSPRK2; it drives the per-stage seam of
+ * FixedBGAMRLevelHooks::specific_eval_rhs / specific_update_ode
+ * (FixedBGAMRLevelHooks.hpp:40-86) the way GRChombo's RK4 drives
+ * GRAMRLevel.  `diagnostics` receives the sum over all recoveries, `mask`
+ * the last recovery's bits. */
+int grhd_advance_rk4_box(grhd_ctx *ctx, const grhd_box *box,
+                         const grhd_params *params,
+                         const grhd_background *background, double time,
+                         double dt, int nsteps, double *state, int ncomp,
+                         int comp_base, unsigned char *mask,
+                         grhd_diagnostics *diagnostics, grhd_error *err,
+                         void *stream);
+
+/* Analytic cell geometry over the FAB storage box, for checking the device
+ * metric: out holds 16 components [g11 g12 g13 g22 g23 g33, gu11 gu12 gu13
+ * gu22 gu23 gu33, lapse, beta1 beta2 beta3] in the FAB layout of the
+ * storage box.  Replaces load_fixed_background_geometry
+ * (FixedBGFArrayBoxFiniteVolume.hpp:15-22) evaluated per cell. */
+int grhd_cell_geometry_box(grhd_ctx *ctx, const grhd_box *box,
+                           const grhd_background *background, double time,
+                           double *out, void *stream);
+
+
+/* ------------------------------------------------------------------ */
+/* Instrumentation (no counterpart in the reference).                 */
+/* Per-kernel-class timing of every call made on `ctx` while enabled: */
+/* a HIP event is recorded on the call's stream after each launch and */
+/* the gaps are summed per class when the call waits.                 */
+/* ------------------------------------------------------------------ */
+enum
+{
+    GRHD_KERNEL_PRIMS = 0,   /* ghost fill + con2prim over the storage box */
+    GRHD_KERNEL_FACES_X = 1, /* MUSCL-HLLE faces, direction 0             */
+    GRHD_KERNEL_FACES_Y = 2,
+    GRHD_KERNEL_FACES_Z = 3,
+    GRHD_KERNEL_CELLS = 4,   /* divergence + sources + stage update       */
+    GRHD_KERNEL_OTHER = 5,
+    GRHD_NUM_KERNEL_CLASSES = 6
+};
+
+/* Enable (1) or disable (0) timing; either resets the accumulators. */
+int grhd_ctx_set_timing(grhd_ctx *ctx, int enable);
+
+/* Accumulated milliseconds and launch counts per kernel class
+ * (arrays of GRHD_NUM_KERNEL_CLASSES). */
+int grhd_ctx_get_timing(grhd_ctx *ctx, double *total_ms,
+                        long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRHD_ABI_H_ */

@@ -1,0 +1,2203 @@
+/*
+ * grhd_oracle.c -- CPU ORACLE for the fixed-background GRHD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * implementation in grdzhadzha_amd/csrc and the timed CPU baseline of
+ * bench.py.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it; the product never links or calls it.
+ *
+ * It restates, in plain C99 + OpenMP, the algorithm of GRDzhadzha's
+ * header-only C++ GRHD path (paths relative to the reference repository):
+ *   Source/Matter/GRHD/{GRHDVars,IdealGasEOS,Atmosphere,Valencia,
+ *     Reconstruction,RiemannSolvers,FiniteVolume,StaticMetric,FixedBGMetric,
+ *     FArrayBoxFiniteVolume,FixedBGFArrayBoxFiniteVolume}.hpp
+ *   Source/Background/{KerrSchild,CircularBinaryPN,ADMFixedBGVars}.hpp
+ * Each function cites the file:line it follows.  It keeps the reference's
+ * CALL STRUCTURE (per-sweep ghost fill, per-sweep primitive recovery over
+ * the ghosted box, every face evaluated twice with two metric evaluations,
+ * sources with their own metric evaluations) so that timing it times the
+ * reference's work; OpenMP splits the per-box cell loops.
+ *
+ * Deliberate, documented deviations from a literal restatement:
+ *  - pow(1+2H, -0.5), pow(lapse, 3.0), pow(lapse, 2.0) (KerrSchild.hpp:101,
+ *    129-140) are written 1.0/sqrt(1+2H), lapse*lapse*lapse, lapse*lapse:
+ *    IEEE-exact operations that the GPU reproduces bit for bit; each differs
+ *    from libm pow by at most a couple of ulp (tested).
+ *  - log in spatial_metric_log_sqrt_det (StaticMetric.hpp:293-302) is
+ *    grhd_oracle_log below, an IEEE-only log accurate to ~1 ulp (tested
+ *    against libm), for the same reason.
+ *  - The GRChombo TensorAlgebra / Coordinates helpers are not vendored in
+ *    the reference; they are restated from GRChombo's public definitions
+ *    (operation order there is unpinned; the invariants in tests/ pin the
+ *    mathematics).
+ * Compile with -ffp-contract=off (see oracle/Makefile).
+ */
+#include "grhd_oracle.h"
+
+#include <omp.h>
+
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* Types (GRHDVars.hpp:15-37, StaticMetric.hpp:19-40,79-102,          */
+/*        ADMFixedBGVars.hpp:28-39)                                   */
+/* ------------------------------------------------------------------ */
+typedef struct
+{
+    double rho, eps, pressure, v[3];
+} prim_t;
+
+typedef struct
+{
+    double D, S[3], tau;
+} cons_t; /* also the Flux layout */
+
+typedef struct
+{
+    double g[3][3];  /* spatial_metric_LL */
+    double gu[3][3]; /* spatial_metric_UU */
+    double lapse;
+    double shift[3];
+    double K[3][3]; /* extrinsic_curvature_LL */
+} geom_t;
+
+typedef struct
+{
+    double lapse[3];               /* d_dir lapse */
+    double log_sqrt_det[3];        /* d_dir log sqrt gamma */
+    double g[3][3][3];             /* [dir][i][j] = d_dir gamma_ij */
+    double shift[3][3];            /* [i][dir] = d_dir beta^i */
+} derivs_t;
+
+typedef struct
+{
+    double gamma[3][3];
+    double K_tensor[3][3];
+    double K;
+    double lapse;
+    double shift[3];
+    double d1_gamma[3][3][3]; /* [i][j][k] = d_k gamma_ij */
+    double d1_lapse[3];
+    double d1_shift[3][3]; /* [i][j] = d_j beta^i */
+} adm_t;
+
+/* Binary positions/velocities (CircularBinaryPN.hpp:192-198). */
+typedef struct
+{
+    double position_1[3], position_2[3], velocity_1[3], velocity_2[3];
+} pn_state_t;
+
+/* A background evaluated at one time: what background_factory(time)
+ * returns in the reference. */
+typedef struct
+{
+    int kind;
+    double mass, spin;
+    double mass_1, mass_2, softening_radius;
+    pn_state_t pn;
+} bg_eval_t;
+
+/* ------------------------------------------------------------------ */
+/* Small helpers with std::min / std::max semantics                   */
+/* ------------------------------------------------------------------ */
+static inline double std_max(double a, double b) { return (a < b) ? b : a; }
+static inline double std_min(double a, double b) { return (b < a) ? b : a; }
+static inline double delta(int i, int j) { return (i == j) ? 1.0 : 0.0; }
+
+/* ------------------------------------------------------------------ */
+/* GRChombo TensorAlgebra (not vendored; restated)                    */
+/* ------------------------------------------------------------------ */
+static double compute_determinant_sym(const double m[3][3])
+{
+    return m[0][0] * m[1][1] * m[2][2] + 2 * m[0][1] * m[0][2] * m[1][2] -
+           m[0][0] * m[1][2] * m[1][2] - m[1][1] * m[0][2] * m[0][2] -
+           m[2][2] * m[0][1] * m[0][1];
+}
+
+static void compute_inverse_sym(const double m[3][3], double out[3][3])
+{
+    const double deth = compute_determinant_sym(m);
+    const double deth_inverse = 1. / deth;
+    out[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[1][2]) * deth_inverse;
+    out[0][1] = (m[0][2] * m[1][2] - m[0][1] * m[2][2]) * deth_inverse;
+    out[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * deth_inverse;
+    out[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[0][2]) * deth_inverse;
+    out[1][2] = (m[0][1] * m[0][2] - m[0][0] * m[1][2]) * deth_inverse;
+    out[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[0][1]) * deth_inverse;
+    out[1][0] = out[0][1];
+    out[2][0] = out[0][2];
+    out[2][1] = out[1][2];
+}
+
+/* chris.ULL[i][j][k] = h^il 1/2 (d_k g_lj + d_j g_lk - d_l g_jk), with
+ * d1_metric[a][b][c] = d_c g_ab. */
+static void compute_christoffel_ULL(const double d1[3][3][3],
+                                    const double hUU[3][3],
+                                    double ULL[3][3][3])
+{
+    double LLL[3][3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+                LLL[i][j][k] =
+                    0.5 * (d1[j][i][k] + d1[k][i][j] - d1[j][k][i]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+            {
+                ULL[i][j][k] = 0;
+                for (int l = 0; l < 3; ++l)
+                    ULL[i][j][k] += hUU[i][l] * LLL[l][j][k];
+            }
+}
+
+static double compute_trace(const double hUU[3][3], const double T[3][3])
+{
+    double t = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            t += hUU[i][j] * T[i][j];
+    return t;
+}
+
+/* ------------------------------------------------------------------ */
+/* IEEE-only natural log used for log sqrt det (see header comment).  */
+/* log x = k ln2 + 2 atanh(s), s = (m-1)/(m+1), m in [sqrt(1/2),      */
+/* sqrt(2)); ln2 split in two parts so k*ln2_hi is exact.             */
+/* ------------------------------------------------------------------ */
+double grhd_oracle_log(double x)
+{
+    if (!(x > 0.0) || x == INFINITY)
+    {
+        if (x == 0.0)
+            return -INFINITY;
+        if (x == INFINITY)
+            return INFINITY;
+        return NAN;
+    }
+    int k;
+    double m = frexp(x, &k); /* x = m 2^k, m in [0.5, 1) */
+    if (m < 0.70710678118654752440)
+    {
+        m = m * 2.0;
+        k = k - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    /* 2 atanh(s) = 2 s (1 + s2/3 + s2^2/5 + ...), |s| < 0.1716 */
+    double p = 1.0 / 25.0;
+    p = 1.0 / 23.0 + s2 * p;
+    p = 1.0 / 21.0 + s2 * p;
+    p = 1.0 / 19.0 + s2 * p;
+    p = 1.0 / 17.0 + s2 * p;
+    p = 1.0 / 15.0 + s2 * p;
+    p = 1.0 / 13.0 + s2 * p;
+    p = 1.0 / 11.0 + s2 * p;
+    p = 1.0 / 9.0 + s2 * p;
+    p = 1.0 / 7.0 + s2 * p;
+    p = 1.0 / 5.0 + s2 * p;
+    p = 1.0 / 3.0 + s2 * p;
+    const double two_s = 2.0 * s;
+    const double log_m = two_s + two_s * (s2 * p);
+    const double ln2_hi = 6.93147180369123816490e-01; /* 0x3fe62e42fee00000 */
+    const double ln2_lo = 1.90821492927058770002e-10; /* 0x3dea39ef35793c76 */
+    const double dk = (double)k;
+    return (dk * ln2_hi + (log_m + dk * ln2_lo));
+}
+
+/* ------------------------------------------------------------------ */
+/* Coordinates: x = (i + 0.5) dx - center (GRChombo Coordinates)      */
+/* ------------------------------------------------------------------ */
+static inline void cell_coords(int i, int j, int k, double dx,
+                               const double center[3], double x[3])
+{
+    x[0] = (i + 0.5) * dx - center[0];
+    x[1] = (j + 0.5) * dx - center[1];
+    x[2] = (k + 0.5) * dx - center[2];
+}
+
+/* Coordinates::get_radius: sqrt(x^2+y^2+z^2) clamped below at 1e-6. */
+static inline double get_radius(const double x[3])
+{
+    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    return std_max(r, 1e-6);
+}
+
+/* ------------------------------------------------------------------ */
+/* KerrSchild::get_KS_derivs (KerrSchild.hpp:171-242)                 */
+/* ------------------------------------------------------------------ */
+static void ks_derivs(double M, double a, const double xc[3], double H,
+                      double dHdx[3], double dldx[3][3], double dltdx[3])
+{
+    (void)M;
+    const double a2 = a * a;
+    double x[3] = {xc[0], xc[1], xc[2]};
+    const double z = xc[2];
+    const double rho = get_radius(xc);
+    const double rho2 = rho * rho;
+    const double r2 =
+        0.5 * (rho2 - a2) + sqrt(0.25 * (rho2 - a2) * (rho2 - a2) + a2 * z * z);
+    const double r = sqrt(r2);
+    const double cos_theta = z / r;
+    const double cos_theta2 = cos_theta * cos_theta;
+
+    double drhodx[3];
+    for (int i = 0; i < 3; ++i)
+        drhodx[i] = x[i] / rho;
+
+    double drdx[3];
+    for (int i = 0; i < 3; ++i)
+        drdx[i] = 0.5 / r *
+                  (rho * drhodx[i] +
+                   0.5 / sqrt(0.25 * (rho2 - a2) * (rho2 - a2) + a2 * z * z) *
+                       (drhodx[i] * rho * (rho2 - a2) +
+                        delta(i, 2) * 2.0 * a2 * z));
+
+    double dcosthetadx[3];
+    for (int i = 0; i < 3; ++i)
+        dcosthetadx[i] = -z / r2 * drdx[i] + delta(i, 2) / r;
+
+    for (int i = 0; i < 3; ++i)
+        dHdx[i] = H * (drdx[i] / r -
+                       2.0 / (r2 + a2 * cos_theta2) *
+                           (r * drdx[i] + a2 * cos_theta * dcosthetadx[i]));
+
+    for (int i = 0; i < 3; ++i)
+    {
+        dldx[0][i] = (x[0] * drdx[i] + r * delta(i, 0) + a * delta(i, 1) -
+                      2.0 * r * drdx[i] * (r * x[0] + a * x[1]) / (r2 + a2)) /
+                     (r2 + a2);
+        dldx[1][i] = (x[1] * drdx[i] + r * delta(i, 1) - a * delta(i, 0) -
+                      2.0 * r * drdx[i] * (r * x[1] - a * x[0]) / (r2 + a2)) /
+                     (r2 + a2);
+        dldx[2][i] = -x[2] * drdx[i] / r2 + delta(i, 2) / r;
+    }
+    for (int i = 0; i < 3; ++i)
+        dltdx[i] = 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* KerrSchild::compute_metric_background (KerrSchild.hpp:65-166)      */
+/* ------------------------------------------------------------------ */
+static void ks_metric(double M, double a, const double xc[3], adm_t *v)
+{
+    const double a2 = a * a;
+    const double x = xc[0];
+    const double y = xc[1];
+    const double z = xc[2];
+    const double rho = get_radius(xc);
+    const double rho2 = rho * rho;
+    const double r2 =
+        0.5 * (rho2 - a2) + sqrt(0.25 * (rho2 - a2) * (rho2 - a2) + a2 * z * z);
+    const double r = sqrt(r2);
+    const double cos_theta = z / r;
+
+    const double H = M * r / (r2 + a2 * cos_theta * cos_theta);
+    const double el[3] = {(r * x + a * y) / (r2 + a2),
+                          (r * y - a * x) / (r2 + a2), z / r};
+    const double el_t = 1.0;
+
+    double dHdx[3], dltdx[3], dldx[3][3];
+    ks_derivs(M, a, xc, H, dHdx, dldx, dltdx);
+
+    /* pow(1 + 2 H el_t el_t, -0.5) */
+    v->lapse = 1.0 / sqrt(1.0 + 2.0 * H * el_t * el_t);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            v->gamma[i][j] = delta(i, j) + 2.0 * H * el[i] * el[j];
+    double gUU[3][3];
+    compute_inverse_sym(v->gamma, gUU);
+    for (int i = 0; i < 3; ++i)
+    {
+        v->shift[i] = 0;
+        for (int j = 0; j < 3; ++j)
+            v->shift[i] += gUU[i][j] * 2.0 * H * el[j] * el_t;
+    }
+
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+                v->d1_gamma[i][j][k] =
+                    2.0 * (el[i] * el[j] * dHdx[k] + H * el[i] * dldx[j][k] +
+                           H * el[j] * dldx[i][k]);
+
+    const double lapse2 = v->lapse * v->lapse;              /* pow(lapse,2) */
+    const double lapse3 = v->lapse * v->lapse * v->lapse;   /* pow(lapse,3) */
+    for (int i = 0; i < 3; ++i)
+        v->d1_lapse[i] =
+            -lapse3 * el_t * (el_t * dHdx[i] + 2.0 * H * dltdx[i]);
+
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            v->d1_shift[i][j] =
+                2.0 * el_t * dHdx[j] * lapse2 * el[i] +
+                4.0 * el_t * H * v->lapse * v->d1_lapse[j] * el[i] +
+                2.0 * el_t * H * lapse2 * dldx[i][j] +
+                2.0 * dltdx[j] * H * lapse2 * el[i];
+
+    double ULL[3][3][3];
+    compute_christoffel_ULL(v->d1_gamma, gUU, ULL);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            v->K_tensor[i][j] = 0.0;
+            for (int k = 0; k < 3; ++k)
+            {
+                v->K_tensor[i][j] +=
+                    v->gamma[k][j] * v->d1_shift[k][i] +
+                    v->gamma[k][i] * v->d1_shift[k][j] +
+                    (v->d1_gamma[k][i][j] + v->d1_gamma[k][j][i]) *
+                        v->shift[k];
+                for (int m = 0; m < 3; ++m)
+                    v->K_tensor[i][j] +=
+                        -2.0 * ULL[k][i][j] * v->gamma[k][m] * v->shift[m];
+            }
+            v->K_tensor[i][j] *= 0.5 / v->lapse;
+        }
+    v->K = compute_trace(gUU, v->K_tensor);
+}
+
+/* ------------------------------------------------------------------ */
+/* CircularBinaryPN (CircularBinaryPN.hpp:209-297, 70-189)            */
+/* ------------------------------------------------------------------ */
+static double pn_angular_velocity(const grhd_background *p)
+{
+    if (p->angular_velocity != 0.0)
+        return p->angular_velocity;
+    const double total_mass = p->mass_1 + p->mass_2;
+    return sqrt(total_mass / (p->separation * p->separation * p->separation));
+}
+
+/* compute_state at params.time = p->time + time (the time factory). */
+static void pn_compute_state(const grhd_background *p, double time,
+                             pn_state_t *s)
+{
+    const double params_time = p->time + time;
+    const double total_mass = p->mass_1 + p->mass_2;
+    const double omega = pn_angular_velocity(p);
+    const double phase = p->phase + omega * params_time;
+    const double cos_phase = cos(phase);
+    const double sin_phase = sin(phase);
+    const double radius_1 = p->mass_2 / total_mass * p->separation;
+    const double radius_2 = p->mass_1 / total_mass * p->separation;
+    for (int i = 0; i < 3; ++i)
+    {
+        s->position_1[i] = 0.0;
+        s->position_2[i] = 0.0;
+        s->velocity_1[i] = 0.0;
+        s->velocity_2[i] = 0.0;
+    }
+    s->position_1[0] = radius_1 * cos_phase;
+    s->position_1[1] = radius_1 * sin_phase;
+    s->position_2[0] = -radius_2 * cos_phase;
+    s->position_2[1] = -radius_2 * sin_phase;
+    s->velocity_1[0] = -omega * radius_1 * sin_phase;
+    s->velocity_1[1] = omega * radius_1 * cos_phase;
+    s->velocity_2[0] = omega * radius_2 * sin_phase;
+    s->velocity_2[1] = -omega * radius_2 * cos_phase;
+}
+
+typedef struct
+{
+    double scalar, scalar_time_derivative, scalar_derivative[3], vector[3],
+        vector_derivative[3][3];
+} potentials_t;
+
+static void pn_add_point_potential(potentials_t *pot, double mass,
+                                   double softening_radius,
+                                   const double src[3], const double vel[3],
+                                   const double pos[3])
+{
+    double displacement[3];
+    double radius_squared = softening_radius * softening_radius;
+    for (int i = 0; i < 3; ++i)
+    {
+        displacement[i] = pos[i] - src[i];
+        radius_squared += displacement[i] * displacement[i];
+    }
+    const double inverse_radius = 1.0 / sqrt(radius_squared);
+    const double inverse_radius_cubed =
+        inverse_radius * inverse_radius * inverse_radius;
+    pot->scalar += mass * inverse_radius;
+    double displacement_dot_velocity = 0.0;
+    for (int i = 0; i < 3; ++i)
+        displacement_dot_velocity += displacement[i] * vel[i];
+    pot->scalar_time_derivative +=
+        mass * displacement_dot_velocity * inverse_radius_cubed;
+    for (int d = 0; d < 3; ++d)
+        pot->scalar_derivative[d] +=
+            -mass * displacement[d] * inverse_radius_cubed;
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d)
+            pot->vector_derivative[c][d] +=
+                -mass * vel[c] * displacement[d] * inverse_radius_cubed;
+    for (int c = 0; c < 3; ++c)
+        pot->vector[c] += mass * vel[c] * inverse_radius;
+}
+
+static void pn_metric(const bg_eval_t *bg, const double pos[3], adm_t *v)
+{
+    potentials_t pot;
+    memset(&pot, 0, sizeof(pot));
+    pn_add_point_potential(&pot, bg->mass_1, bg->softening_radius,
+                           bg->pn.position_1, bg->pn.velocity_1, pos);
+    pn_add_point_potential(&pot, bg->mass_2, bg->softening_radius,
+                           bg->pn.position_2, bg->pn.velocity_2, pos);
+
+    const double U = pot.scalar;
+    const double d_conformal_factor_dt = 2.0 * pot.scalar_time_derivative;
+    const double conformal_factor = 1.0 + 2.0 * U;
+    double vector_squared = 0.0;
+    for (int i = 0; i < 3; ++i)
+        vector_squared += pot.vector[i] * pot.vector[i];
+    const double lapse_squared = 1.0 - 2.0 * U + 2.0 * U * U +
+                                 16.0 * vector_squared / conformal_factor;
+    for (int i = 0; i < 3; ++i)
+    {
+        v->shift[i] = 0.0;
+        v->d1_lapse[i] = 0.0;
+        for (int j = 0; j < 3; ++j)
+        {
+            v->gamma[i][j] = 0.0;
+            v->d1_shift[i][j] = 0.0;
+            v->K_tensor[i][j] = 0.0;
+            for (int k = 0; k < 3; ++k)
+                v->d1_gamma[i][j][k] = 0.0;
+        }
+    }
+    v->lapse = sqrt(lapse_squared);
+    for (int i = 0; i < 3; ++i)
+    {
+        v->gamma[i][i] = conformal_factor;
+        v->shift[i] = -4.0 * pot.vector[i] / conformal_factor;
+    }
+    double gUU[3][3];
+    compute_inverse_sym(v->gamma, gUU);
+
+    for (int d = 0; d < 3; ++d)
+    {
+        const double dU = pot.scalar_derivative[d];
+        const double d_conformal_factor = 2.0 * dU;
+        for (int m = 0; m < 3; ++m)
+            v->d1_gamma[m][m][d] = d_conformal_factor;
+        double d_vector_squared = 0.0;
+        for (int c = 0; c < 3; ++c)
+        {
+            d_vector_squared +=
+                2.0 * pot.vector[c] * pot.vector_derivative[c][d];
+            v->d1_shift[c][d] =
+                -4.0 * (pot.vector_derivative[c][d] / conformal_factor -
+                        pot.vector[c] * d_conformal_factor /
+                            (conformal_factor * conformal_factor));
+        }
+        const double d_lapse_squared =
+            (-2.0 + 4.0 * U) * dU +
+            16.0 * (d_vector_squared / conformal_factor -
+                    vector_squared * d_conformal_factor /
+                        (conformal_factor * conformal_factor));
+        v->d1_lapse[d] = 0.5 * d_lapse_squared / v->lapse;
+    }
+
+    double ULL[3][3][3];
+    compute_christoffel_ULL(v->d1_gamma, gUU, ULL);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            for (int k = 0; k < 3; ++k)
+            {
+                v->K_tensor[i][j] +=
+                    v->gamma[k][j] * v->d1_shift[k][i] +
+                    v->gamma[k][i] * v->d1_shift[k][j] +
+                    (v->d1_gamma[k][i][j] + v->d1_gamma[k][j][i]) *
+                        v->shift[k];
+                for (int m = 0; m < 3; ++m)
+                    v->K_tensor[i][j] +=
+                        -2.0 * ULL[k][i][j] * v->gamma[k][m] * v->shift[m];
+            }
+            if (i == j)
+                v->K_tensor[i][j] -= d_conformal_factor_dt;
+            v->K_tensor[i][j] *= 0.5 / v->lapse;
+        }
+    v->K = compute_trace(gUU, v->K_tensor);
+}
+
+/* background_factory(time) */
+static void background_at(const grhd_background *p, double time,
+                          bg_eval_t *out)
+{
+    memset(out, 0, sizeof(*out));
+    out->kind = p->kind;
+    out->mass = p->mass;
+    out->spin = p->spin;
+    out->mass_1 = p->mass_1;
+    out->mass_2 = p->mass_2;
+    out->softening_radius = p->softening_radius;
+    if (p->kind == GRHD_BG_CIRCULAR_BINARY_PN)
+        pn_compute_state(p, time, &out->pn);
+}
+
+static void background_metric(const bg_eval_t *bg, const double x[3],
+                              adm_t *v)
+{
+    if (bg->kind == GRHD_BG_CIRCULAR_BINARY_PN)
+        pn_metric(bg, x, v);
+    else
+        ks_metric(bg->mass, bg->spin, x, v);
+}
+
+/* ------------------------------------------------------------------ */
+/* FixedBGMetric.hpp:17-55                                            */
+/* ------------------------------------------------------------------ */
+static void geometry_from_adm(const adm_t *v, geom_t *g)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            g->g[i][j] = v->gamma[i][j];
+            g->K[i][j] = v->K_tensor[i][j];
+        }
+    compute_inverse_sym(v->gamma, g->gu);
+    g->lapse = v->lapse;
+    for (int i = 0; i < 3; ++i)
+        g->shift[i] = v->shift[i];
+}
+
+static void derivs_from_adm(const adm_t *v, derivs_t *d)
+{
+    for (int i = 0; i < 3; ++i)
+    {
+        d->lapse[i] = v->d1_lapse[i];
+        for (int j = 0; j < 3; ++j)
+            d->shift[i][j] = v->d1_shift[i][j];
+    }
+    double gUU[3][3];
+    compute_inverse_sym(v->gamma, gUU);
+    for (int dir = 0; dir < 3; ++dir)
+    {
+        d->log_sqrt_det[dir] = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+            {
+                d->g[dir][i][j] = v->d1_gamma[i][j][dir];
+                d->log_sqrt_det[dir] +=
+                    0.5 * gUU[i][j] * v->d1_gamma[i][j][dir];
+            }
+    }
+}
+
+/* load_fixed_background_geometry (FixedBGFArrayBoxFiniteVolume.hpp:15-22) */
+static void load_geometry(const bg_eval_t *bg, int i, int j, int k,
+                          double dx, const double center[3], geom_t *g)
+{
+    double x[3];
+    adm_t v;
+    cell_coords(i, j, k, dx, center, x);
+    background_metric(bg, x, &v);
+    geometry_from_adm(&v, g);
+}
+
+/* load_fixed_background_metric_derivatives (:24-31) */
+static void load_derivs(const bg_eval_t *bg, int i, int j, int k, double dx,
+                        const double center[3], derivs_t *d)
+{
+    double x[3];
+    adm_t v;
+    cell_coords(i, j, k, dx, center, x);
+    background_metric(bg, x, &v);
+    derivs_from_adm(&v, d);
+}
+
+/* ------------------------------------------------------------------ */
+/* IdealGasEOS.hpp:10-53                                              */
+/* ------------------------------------------------------------------ */
+static inline double eos_h(double gamma, double rho, double eps, double p)
+{
+    (void)gamma;
+    return 1.0 + eps + p / rho;
+}
+static inline double eos_cs2(double gamma, double rho, double eps, double p)
+{
+    const double h = eos_h(gamma, rho, eps, p);
+    return gamma * p / (rho * h);
+}
+
+/* ------------------------------------------------------------------ */
+/* Valencia.hpp                                                       */
+/* ------------------------------------------------------------------ */
+/* compute_velocity_squared (:33-44) */
+static inline double velocity_squared(const double v[3], const double g[3][3])
+{
+    double v2 = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            v2 += g[i][j] * v[i] * v[j];
+    return v2;
+}
+
+/* compute_conserved (:55-73) */
+static void compute_conserved(const prim_t *p, double gamma,
+                              const double g[3][3], cons_t *c)
+{
+    double vL[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        vL[i] = 0;
+        for (int j = 0; j < 3; ++j)
+            vL[i] += g[i][j] * p->v[j];
+    }
+    const double v2 = velocity_squared(p->v, g);
+    const double W = 1.0 / sqrt(1.0 - v2);
+    const double h = eos_h(gamma, p->rho, p->eps, p->pressure);
+    const double rho_h_W2 = p->rho * h * W * W;
+    c->D = p->rho * W;
+    c->tau = rho_h_W2 - p->pressure - c->D;
+    for (int i = 0; i < 3; ++i)
+        c->S[i] = rho_h_W2 * vL[i];
+}
+
+static inline double square(double x) { return x * x; }
+
+/* detail::pressure_residual (:79-91) */
+static inline double pressure_residual(double pressure, double D, double tau,
+                                       double S_squared, double gamma,
+                                       double max_v2)
+{
+    const double epp = tau + D + pressure;
+    double v_squared = S_squared / square(epp);
+    if (v_squared > max_v2)
+        v_squared = max_v2;
+    const double W = 1.0 / sqrt(1.0 - v_squared);
+    return D * W + gamma * pressure * W * W / (gamma - 1.0) - epp;
+}
+
+/* detail::pressure_residual_derivative (:93-108) */
+static inline double pressure_residual_derivative(double pressure, double D,
+                                                  double tau,
+                                                  double S_squared,
+                                                  double gamma, double max_v2)
+{
+    const double epp = tau + D + pressure;
+    double v_squared = S_squared / square(epp);
+    if (v_squared > max_v2)
+        v_squared = max_v2;
+    const double W = 1.0 / sqrt(1.0 - v_squared);
+    const double dW_dp = -W * W * W * v_squared / epp;
+    const double dW2_dp = 2.0 * W * dW_dp;
+    return D * dW_dp + gamma * (W * W + pressure * dW2_dp) / (gamma - 1.0) -
+           1.0;
+}
+
+typedef struct
+{
+    prim_t prim;
+    int success;
+    int iterations;
+    double residual;
+} recovery_t;
+
+/* recover_primitive (:116-228) */
+static void recover_primitive(const cons_t *c, double gamma,
+                              const double gu[3][3], const grhd_params *o,
+                              recovery_t *res)
+{
+    memset(res, 0, sizeof(*res));
+    res->success = 0;
+    res->iterations = 0;
+    res->residual = 0.0;
+    if (c->D <= 0.0)
+        return;
+
+    double S_squared = 0.;
+    for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n)
+            S_squared += gu[m][n] * c->S[m] * c->S[n];
+
+    const double max_v2 = o->recovery_max_velocity_squared;
+    double lower = o->recovery_pressure_floor;
+    double f_lower =
+        pressure_residual(lower, c->D, c->tau, S_squared, gamma, max_v2);
+    if (f_lower >= 0.0 && fabs(f_lower) > o->tolerance)
+        return;
+
+    double upper = lower;
+    double f_upper = f_lower;
+    if (f_lower < 0.0)
+    {
+        upper = std_max(1.0, fabs(c->tau) + c->D);
+        f_upper =
+            pressure_residual(upper, c->D, c->tau, S_squared, gamma, max_v2);
+        int bracket_iterations = 0;
+        while (f_upper <= 0.0 && bracket_iterations < o->max_iterations)
+        {
+            upper *= 2.0;
+            f_upper = pressure_residual(upper, c->D, c->tau, S_squared, gamma,
+                                        max_v2);
+            ++bracket_iterations;
+        }
+        if (f_upper <= 0.0)
+            return;
+    }
+
+    double pressure =
+        (fabs(f_lower) <= o->tolerance)
+            ? lower
+            : std_min(std_max(c->tau * (gamma - 1.0), lower), upper);
+    double residual =
+        pressure_residual(pressure, c->D, c->tau, S_squared, gamma, max_v2);
+
+    for (int iter = 0; iter < o->max_iterations; ++iter)
+    {
+        res->iterations = iter + 1;
+        if (fabs(residual) < o->tolerance)
+            break;
+        if (f_lower < 0.0)
+        {
+            if (residual > 0.0)
+                upper = pressure;
+            else
+                lower = pressure;
+        }
+        const double derivative = pressure_residual_derivative(
+            pressure, c->D, c->tau, S_squared, gamma, max_v2);
+        double next_pressure = pressure - residual / derivative;
+        if (!isfinite(next_pressure) || next_pressure <= lower ||
+            next_pressure >= upper)
+            next_pressure = 0.5 * (lower + upper);
+        pressure = next_pressure;
+        residual = pressure_residual(pressure, c->D, c->tau, S_squared, gamma,
+                                     max_v2);
+    }
+
+    res->residual = residual;
+    if (fabs(residual) >= o->tolerance)
+        return;
+
+    const double epp = c->tau + c->D + pressure;
+    const double v_squared = S_squared / square(epp);
+    if (v_squared >= max_v2)
+        return;
+
+    const double W = 1.0 / sqrt(1.0 - v_squared);
+    res->prim.rho = c->D / W;
+    res->prim.pressure = pressure;
+    res->prim.eps = pressure / ((gamma - 1.0) * res->prim.rho);
+    for (int i = 0; i < 3; ++i)
+    {
+        res->prim.v[i] = 0.0;
+        for (int j = 0; j < 3; ++j)
+            res->prim.v[i] += gu[i][j] * c->S[j] / epp;
+    }
+    res->success = 1;
+}
+
+/* compute_flux (:230-249) */
+static void compute_flux(const prim_t *p, const cons_t *c, int d,
+                         double lapse, const double shift[3], cons_t *f)
+{
+    const double tv = lapse * p->v[d] - shift[d];
+    f->D = c->D * tv;
+    f->tau = c->tau * tv + lapse * p->pressure * p->v[d];
+    for (int j = 0; j < 3; ++j)
+    {
+        f->S[j] = c->S[j] * tv;
+        if (j == d)
+            f->S[j] += lapse * p->pressure;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Atmosphere.hpp:24-60, FArrayBoxFiniteVolume.hpp:118-134, 704-711   */
+/* ------------------------------------------------------------------ */
+static void enforce_primitive_floors(prim_t *p, double gamma,
+                                     const double g[3][3],
+                                     const grhd_params *o)
+{
+    p->rho = std_max(p->rho, o->rho_floor);
+    p->pressure = std_max(p->pressure, o->pressure_floor);
+    const double v2 = velocity_squared(p->v, g);
+    if (v2 >= o->max_velocity_squared)
+    {
+        const double scale = sqrt(o->max_velocity_squared / v2);
+        for (int i = 0; i < 3; ++i)
+            p->v[i] *= scale;
+    }
+    p->eps = p->pressure / ((gamma - 1.0) * p->rho);
+}
+
+static void make_atmosphere_primitive(double gamma, const grhd_params *o,
+                                      prim_t *p)
+{
+    p->rho = o->rho_floor;
+    p->pressure = o->pressure_floor;
+    p->eps = p->pressure / ((gamma - 1.0) * p->rho);
+    for (int i = 0; i < 3; ++i)
+        p->v[i] = 0.0;
+}
+
+static int is_atmosphere_primitive(const prim_t *p, const grhd_params *o,
+                                   double threshold_factor)
+{
+    const double factor = std_max(1.0, threshold_factor);
+    return p->rho <= factor * o->rho_floor &&
+           p->pressure <= factor * o->pressure_floor;
+}
+
+static int primitive_differs(const prim_t *a, const prim_t *b)
+{
+    int differs =
+        a->rho != b->rho || a->eps != b->eps || a->pressure != b->pressure;
+    for (int i = 0; i < 3; ++i)
+        differs = differs || a->v[i] != b->v[i];
+    return differs;
+}
+
+/* ------------------------------------------------------------------ */
+/* Reconstruction.hpp:16-69                                           */
+/* ------------------------------------------------------------------ */
+static inline double minmod2(double left, double right)
+{
+    if (left * right <= 0.0)
+        return 0.0;
+    return left > 0.0 ? std_min(left, right) : std_max(left, right);
+}
+static inline double minmod3(double a, double b, double c)
+{
+    return minmod2(a, minmod2(b, c));
+}
+static inline double limited_slope(double l, double c, double r,
+                                   double theta)
+{
+    return minmod3(theta * (c - l), 0.5 * (r - l), theta * (r - c));
+}
+static void limited_slope_prim(const prim_t *l, const prim_t *c,
+                               const prim_t *r, double theta, prim_t *s)
+{
+    s->rho = limited_slope(l->rho, c->rho, r->rho, theta);
+    s->eps = limited_slope(l->eps, c->eps, r->eps, theta);
+    s->pressure = limited_slope(l->pressure, c->pressure, r->pressure, theta);
+    for (int i = 0; i < 3; ++i)
+        s->v[i] = limited_slope(l->v[i], c->v[i], r->v[i], theta);
+}
+static void add_scaled_prim(const prim_t *p, const prim_t *s, double scale,
+                            prim_t *out)
+{
+    out->rho = p->rho + scale * s->rho;
+    out->eps = p->eps + scale * s->eps;
+    out->pressure = p->pressure + scale * s->pressure;
+    for (int i = 0; i < 3; ++i)
+        out->v[i] = p->v[i] + scale * s->v[i];
+}
+
+/* ------------------------------------------------------------------ */
+/* RiemannSolvers.hpp:23-118                                          */
+/* ------------------------------------------------------------------ */
+static void characteristic_speeds(const prim_t *p, double gamma,
+                                  const double g[3][3], const double gu[3][3],
+                                  int d, double lapse, const double shift[3],
+                                  double *lower, double *upper)
+{
+    const double v2 = velocity_squared(p->v, g);
+    const double cs2 = eos_cs2(gamma, p->rho, p->eps, p->pressure);
+    const double cs = sqrt(cs2);
+    const double vel = p->v[d];
+    const double denominator = 1.0 - v2 * cs2;
+    const double radicand =
+        (1.0 - v2) *
+        (gu[d][d] * (1.0 - v2 * cs2) - vel * vel * (1.0 - cs2));
+    const double root = cs * sqrt(std_max(0.0, radicand));
+    double lo = (vel * (1.0 - cs2) - root) / denominator;
+    double hi = (vel * (1.0 - cs2) + root) / denominator;
+    *lower = lapse * lo - shift[d];
+    *upper = lapse * hi - shift[d];
+}
+
+static void hlle_flux(const prim_t *pl, const prim_t *pr, const cons_t *cl,
+                      const cons_t *cr, double gamma, const double g[3][3],
+                      const double gu[3][3], int d, double lapse,
+                      const double shift[3], cons_t *f)
+{
+    cons_t fl, fr;
+    compute_flux(pl, cl, d, lapse, shift, &fl);
+    compute_flux(pr, cr, d, lapse, shift, &fr);
+    double ll, lu, rl, ru;
+    characteristic_speeds(pl, gamma, g, gu, d, lapse, shift, &ll, &lu);
+    characteristic_speeds(pr, gamma, g, gu, d, lapse, shift, &rl, &ru);
+    const double s_lower = std_min(0.0, std_min(ll, rl));
+    const double s_upper = std_max(0.0, std_max(lu, ru));
+    if (s_lower >= 0.0)
+    {
+        *f = fl;
+        return;
+    }
+    if (s_upper <= 0.0)
+    {
+        *f = fr;
+        return;
+    }
+    const double inv = 1.0 / (s_upper - s_lower);
+    f->D = (s_upper * fl.D - s_lower * fr.D +
+            s_lower * s_upper * (cr->D - cl->D)) *
+           inv;
+    f->tau = (s_upper * fl.tau - s_lower * fr.tau +
+              s_lower * s_upper * (cr->tau - cl->tau)) *
+             inv;
+    for (int i = 0; i < 3; ++i)
+        f->S[i] = (s_upper * fl.S[i] - s_lower * fr.S[i] +
+                   s_lower * s_upper * (cr->S[i] - cl->S[i])) *
+                  inv;
+}
+
+/* FiniteVolume.hpp:63-98 (compute_muscl_hlle_flux) */
+static void muscl_hlle_flux(const prim_t *lc, const prim_t *rc,
+                            const prim_t *ls, const prim_t *rs, double gamma,
+                            const geom_t *face, int d, const grhd_params *o,
+                            cons_t *f)
+{
+    prim_t lp = *lc, rp = *rc;
+    if (o->use_reconstruction)
+    {
+        add_scaled_prim(lc, ls, 0.5, &lp);
+        add_scaled_prim(rc, rs, -0.5, &rp);
+    }
+    enforce_primitive_floors(&lp, gamma, face->g, o);
+    enforce_primitive_floors(&rp, gamma, face->g, o);
+    cons_t lcons, rcons;
+    compute_conserved(&lp, gamma, face->g, &lcons);
+    compute_conserved(&rp, gamma, face->g, &rcons);
+    hlle_flux(&lp, &rp, &lcons, &rcons, gamma, face->g, face->gu, d,
+              face->lapse, face->shift, f);
+}
+
+/* average_geometry (FArrayBoxFiniteVolume.hpp:194-216) */
+static void average_geometry(const geom_t *l, const geom_t *r, geom_t *out)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            out->g[i][j] = 0.5 * (l->g[i][j] + r->g[i][j]);
+            out->gu[i][j] = 0.5 * (l->gu[i][j] + r->gu[i][j]);
+            out->K[i][j] = 0.5 * (l->K[i][j] + r->K[i][j]);
+        }
+    out->lapse = 0.5 * (l->lapse + r->lapse);
+    for (int i = 0; i < 3; ++i)
+        out->shift[i] = 0.5 * (l->shift[i] + r->shift[i]);
+}
+
+/* ------------------------------------------------------------------ */
+/* StaticMetric.hpp:193-302                                           */
+/* ------------------------------------------------------------------ */
+static void spatial_stress_UU(const prim_t *p, double gamma,
+                              const geom_t *geo, double T[3][3])
+{
+    const double W = 1.0 / sqrt(1.0 - velocity_squared(p->v, geo->g));
+    const double h = eos_h(gamma, p->rho, p->eps, p->pressure);
+    const double rho_h_W2 = p->rho * h * W * W;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            T[i][j] = rho_h_W2 * p->v[i] * p->v[j] + p->pressure * geo->gu[i][j];
+}
+
+static void static_metric_source_terms(const prim_t *p, const cons_t *c,
+                                       double gamma, const geom_t *geo,
+                                       const derivs_t *dv, cons_t *src)
+{
+    src->D = 0.0;
+    src->tau = 0.0;
+    for (int i = 0; i < 3; ++i)
+        src->S[i] = 0.0;
+    double T[3][3];
+    spatial_stress_UU(p, gamma, geo, T);
+    const double energy_density = c->D + c->tau;
+    double mU[3];
+    for (int i = 0; i < 3; ++i)
+        mU[i] = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            mU[i] += geo->gu[i][j] * c->S[j];
+    cons_t cf[3];
+    for (int d = 0; d < 3; ++d)
+        compute_flux(p, c, d, geo->lapse, geo->shift, &cf[d]);
+    for (int d = 0; d < 3; ++d)
+    {
+        src->D -= cf[d].D * dv->log_sqrt_det[d];
+        src->tau -= cf[d].tau * dv->log_sqrt_det[d];
+        src->tau -= mU[d] * dv->lapse[d];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            src->tau += geo->lapse * T[i][j] * geo->K[i][j];
+    for (int m = 0; m < 3; ++m)
+    {
+        double ms = -energy_density * dv->lapse[m];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                ms += 0.5 * geo->lapse * T[i][j] * dv->g[m][i][j];
+        for (int s = 0; s < 3; ++s)
+            ms += c->S[s] * dv->shift[s][m];
+        for (int d = 0; d < 3; ++d)
+            ms -= cf[d].S[m] * dv->log_sqrt_det[d];
+        src->S[m] += ms;
+    }
+}
+
+static double spatial_metric_log_sqrt_det(const geom_t *geo)
+{
+    const double det = compute_determinant_sym(geo->g);
+    if (det <= 0.0 || !isfinite(det))
+        return 0.0;
+    return 0.5 * grhd_oracle_log(det);
+}
+
+/* ------------------------------------------------------------------ */
+/* FArrayBox access                                                   */
+/* ------------------------------------------------------------------ */
+typedef struct
+{
+    int lo[3], n[3];
+    long long npts;
+} fab_shape_t;
+
+static inline fab_shape_t shape_of(const int lo[3], const int hi[3])
+{
+    fab_shape_t s;
+    for (int d = 0; d < 3; ++d)
+    {
+        s.lo[d] = lo[d];
+        s.n[d] = hi[d] - lo[d] + 1;
+    }
+    s.npts = (long long)s.n[0] * s.n[1] * s.n[2];
+    return s;
+}
+
+static inline long long fab_off(const fab_shape_t *s, int i, int j, int k)
+{
+    return ((long long)(k - s->lo[2]) * s->n[1] + (j - s->lo[1])) * s->n[0] +
+           (i - s->lo[0]);
+}
+
+#define FAB(f, s, c, off) ((f)[(long long)(c) * (s)->npts + (off)])
+
+static inline void load_cons(const double *f, const fab_shape_t *s, int cb,
+                             long long off, cons_t *c)
+{
+    c->D = FAB(f, s, cb + GRHD_D, off);
+    c->S[0] = FAB(f, s, cb + GRHD_S1, off);
+    c->S[1] = FAB(f, s, cb + GRHD_S2, off);
+    c->S[2] = FAB(f, s, cb + GRHD_S3, off);
+    c->tau = FAB(f, s, cb + GRHD_TAU, off);
+}
+static inline void store_cons(double *f, const fab_shape_t *s, int cb,
+                              long long off, const cons_t *c)
+{
+    FAB(f, s, cb + GRHD_D, off) = c->D;
+    FAB(f, s, cb + GRHD_S1, off) = c->S[0];
+    FAB(f, s, cb + GRHD_S2, off) = c->S[1];
+    FAB(f, s, cb + GRHD_S3, off) = c->S[2];
+    FAB(f, s, cb + GRHD_TAU, off) = c->tau;
+}
+static inline void load_prim(const double *f, const fab_shape_t *s, int cb,
+                             long long off, prim_t *p)
+{
+    p->rho = FAB(f, s, cb + GRHD_RHO, off);
+    p->eps = FAB(f, s, cb + GRHD_EPS, off);
+    p->pressure = FAB(f, s, cb + GRHD_PRESSURE, off);
+    p->v[0] = FAB(f, s, cb + GRHD_V1, off);
+    p->v[1] = FAB(f, s, cb + GRHD_V2, off);
+    p->v[2] = FAB(f, s, cb + GRHD_V3, off);
+}
+static inline void store_prim(double *f, const fab_shape_t *s, int cb,
+                              long long off, const prim_t *p)
+{
+    FAB(f, s, cb + GRHD_RHO, off) = p->rho;
+    FAB(f, s, cb + GRHD_EPS, off) = p->eps;
+    FAB(f, s, cb + GRHD_PRESSURE, off) = p->pressure;
+    FAB(f, s, cb + GRHD_V1, off) = p->v[0];
+    FAB(f, s, cb + GRHD_V2, off) = p->v[1];
+    FAB(f, s, cb + GRHD_V3, off) = p->v[2];
+}
+
+/* First failure in BoxIterator order, shared by OpenMP threads. */
+typedef struct
+{
+    long long first; /* linear index in the iterated box, LLONG_MAX = none */
+    int iv[3];
+    double residual;
+} fail_t;
+
+static void fail_init(fail_t *f)
+{
+    f->first = LLONG_MAX;
+    f->iv[0] = f->iv[1] = f->iv[2] = 0;
+    f->residual = 0.0;
+}
+
+static void fail_record(fail_t *f, long long lin, int i, int j, int k,
+                        double residual)
+{
+#pragma omp critical(grhd_oracle_fail)
+    {
+        if (lin < f->first)
+        {
+            f->first = lin;
+            f->iv[0] = i;
+            f->iv[1] = j;
+            f->iv[2] = k;
+            f->residual = residual;
+        }
+    }
+}
+
+static int fail_to_error(const fail_t *f, int code, grhd_error *err)
+{
+    if (f->first == LLONG_MAX)
+        return GRHD_OK;
+    if (err)
+    {
+        err->code = code;
+        err->iv[0] = f->iv[0];
+        err->iv[1] = f->iv[1];
+        err->iv[2] = f->iv[2];
+        err->residual = f->residual;
+    }
+    return code;
+}
+
+/* ------------------------------------------------------------------ */
+/* Per-FArrayBox operations (FixedBGFArrayBoxFiniteVolume.hpp)         */
+/* ------------------------------------------------------------------ */
+
+/* recover_primitives_from_fixed_background (:33-58): con2prim over `box`
+ * (no floors) into `prims` (FAB on the storage box, 6 comps at 0..5). */
+static int recover_primitives_box(double *prims, const fab_shape_t *ps,
+                                  const double *state, const fab_shape_t *ss,
+                                  int cb, const int lo[3], const int hi[3],
+                                  const bg_eval_t *bg, const grhd_box *b,
+                                  const grhd_params *o, grhd_error *err)
+{
+    fail_t fail;
+    fail_init(&fail);
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i)
+            {
+                geom_t geo;
+                load_geometry(bg, i, j, k, b->dx, b->center, &geo);
+                cons_t c;
+                load_cons(state, ss, cb, fab_off(ss, i, j, k), &c);
+                recovery_t rec;
+                recover_primitive(&c, o->adiabatic_index, geo.gu, o, &rec);
+                if (!rec.success)
+                {
+                    const long long lin =
+                        ((long long)(k - lo[2]) * ny + (j - lo[1])) * nx +
+                        (i - lo[0]);
+                    fail_record(&fail, lin, i, j, k, rec.residual);
+                    continue;
+                }
+                store_prim(prims, ps, -GRHD_RHO, fab_off(ps, i, j, k),
+                           &rec.prim);
+            }
+    return fail_to_error(&fail, GRHD_ERR_RECOVERY_RHS, err);
+}
+
+/* recover_primitives_and_apply_atmosphere_from_fixed_background (:60-110) */
+static void recover_and_apply_atmosphere_box(double *state,
+                                             const fab_shape_t *ss, int cb,
+                                             const grhd_box *b,
+                                             const bg_eval_t *bg,
+                                             const grhd_params *o,
+                                             unsigned char *mask,
+                                             grhd_diagnostics *diag)
+{
+    long long cells = 0, failed = 0, floored = 0, resets = 0;
+    const fab_shape_t is = shape_of(b->lo, b->hi);
+    const double gamma = o->adiabatic_index;
+#pragma omp parallel for collapse(2) schedule(static) \
+    reduction(+ : cells, failed, floored, resets)
+    for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+        for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+            for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+            {
+                ++cells;
+                const long long off = fab_off(ss, i, j, k);
+                geom_t geo;
+                load_geometry(bg, i, j, k, b->dx, b->center, &geo);
+                cons_t c;
+                load_cons(state, ss, cb, off, &c);
+                recovery_t rec;
+                recover_primitive(&c, gamma, geo.gu, o, &rec);
+                prim_t prim;
+                int reset = !rec.success;
+                unsigned char m = 0;
+                if (rec.success)
+                {
+                    prim = rec.prim;
+                    prim_t fl = prim;
+                    enforce_primitive_floors(&fl, gamma, geo.g, o);
+                    reset = primitive_differs(&prim, &fl);
+                    if (reset)
+                    {
+                        ++floored;
+                        m |= GRHD_MASK_FLOORED;
+                    }
+                    prim = fl;
+                }
+                else
+                {
+                    ++failed;
+                    m |= GRHD_MASK_FAILED;
+                    make_atmosphere_primitive(gamma, o, &prim);
+                }
+                store_prim(state, ss, cb, off, &prim);
+                if (reset)
+                {
+                    ++resets;
+                    m |= GRHD_MASK_RESET;
+                    cons_t nc;
+                    compute_conserved(&prim, gamma, geo.g, &nc);
+                    store_cons(state, ss, cb, off, &nc);
+                }
+                if (mask)
+                    mask[fab_off(&is, i, j, k)] = m;
+            }
+    if (diag)
+    {
+        diag->num_cells += cells;
+        diag->num_failed_recoveries += failed;
+        diag->num_floored_primitives += floored;
+        diag->num_conserved_resets += resets;
+    }
+}
+
+/* fill_conserved_outflow_boundaries_from_fixed_background (:113-177) */
+static int fill_outflow_boundaries(double *state, const fab_shape_t *ss,
+                                   int cb, const grhd_box *b,
+                                   const bg_eval_t *bg, const grhd_params *o,
+                                   grhd_error *err)
+{
+    fail_t fail;
+    fail_init(&fail);
+    const double gamma = o->adiabatic_index;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b->glo[2]; k <= b->ghi[2]; ++k)
+        for (int j = b->glo[1]; j <= b->ghi[1]; ++j)
+            for (int i = b->glo[0]; i <= b->ghi[0]; ++i)
+            {
+                const int iv[3] = {i, j, k};
+                int src[3] = {i, j, k};
+                int physical = 0;
+                for (int d = 0; d < 3; ++d)
+                {
+                    if (iv[d] < b->dlo[d])
+                    {
+                        if (!b->periodic[d])
+                        {
+                            physical = 1;
+                            src[d] = b->lo[d];
+                        }
+                    }
+                    else if (iv[d] > b->dhi[d])
+                    {
+                        if (!b->periodic[d])
+                        {
+                            physical = 1;
+                            src[d] = b->hi[d];
+                        }
+                    }
+                }
+                if (!physical)
+                    continue;
+                geom_t sgeo;
+                load_geometry(bg, src[0], src[1], src[2], b->dx, b->center,
+                              &sgeo);
+                cons_t c;
+                load_cons(state, ss, cb, fab_off(ss, src[0], src[1], src[2]),
+                          &c);
+                recovery_t rec;
+                recover_primitive(&c, gamma, sgeo.gu, o, &rec);
+                if (!rec.success)
+                {
+                    fail_record(&fail, fab_off(ss, i, j, k), src[0], src[1],
+                                src[2], rec.residual);
+                    continue;
+                }
+                prim_t prim = rec.prim;
+                geom_t tgeo;
+                load_geometry(bg, i, j, k, b->dx, b->center, &tgeo);
+                enforce_primitive_floors(&prim, gamma, tgeo.g, o);
+                const long long off = fab_off(ss, i, j, k);
+                store_prim(state, ss, cb, off, &prim);
+                cons_t nc;
+                compute_conserved(&prim, gamma, tgeo.g, &nc);
+                store_cons(state, ss, cb, off, &nc);
+            }
+    return fail_to_error(&fail, GRHD_ERR_RECOVERY_GHOST_FILL, err);
+}
+
+/* compute_farraybox_muscl_hlle_flux_from_fixed_background (:179-197) */
+static void face_flux(const double *prims, const double *slopes,
+                      const fab_shape_t *ps, const bg_eval_t *bg,
+                      const grhd_box *b, const grhd_params *o, int i, int j,
+                      int k, int d, cons_t *f)
+{
+    const int ri = i + (d == 0), rj = j + (d == 1), rk = k + (d == 2);
+    geom_t gl, gr, face;
+    load_geometry(bg, i, j, k, b->dx, b->center, &gl);
+    load_geometry(bg, ri, rj, rk, b->dx, b->center, &gr);
+    average_geometry(&gl, &gr, &face);
+    prim_t pl, pr, sl, sr;
+    load_prim(prims, ps, -GRHD_RHO, fab_off(ps, i, j, k), &pl);
+    load_prim(prims, ps, -GRHD_RHO, fab_off(ps, ri, rj, rk), &pr);
+    load_prim(slopes, ps, -GRHD_RHO, fab_off(ps, i, j, k), &sl);
+    load_prim(slopes, ps, -GRHD_RHO, fab_off(ps, ri, rj, rk), &sr);
+    muscl_hlle_flux(&pl, &pr, &sl, &sr, o->adiabatic_index, &face, d, o, f);
+}
+
+/* compute_directional_flux_rhs_from_fixed_background (:199-253), writing
+ * the directional rhs into `rhs_dir` (interior box, 5 comps at 0..4). */
+static int directional_flux_rhs(double *rhs_dir, double *state,
+                                const fab_shape_t *ss, int cb,
+                                const grhd_box *b, const bg_eval_t *bg,
+                                const grhd_params *o, int d,
+                                double inverse_dx, double *prims,
+                                double *slopes, grhd_error *err)
+{
+    int rc = fill_outflow_boundaries(state, ss, cb, b, bg, o, err);
+    if (rc)
+        return rc;
+    memset(prims, 0, sizeof(double) * 6 * (size_t)ss->npts);
+    memset(slopes, 0, sizeof(double) * 6 * (size_t)ss->npts);
+    rc = recover_primitives_box(prims, ss, state, ss, cb, b->glo, b->ghi, bg,
+                                b, o, err);
+    if (rc)
+        return rc;
+
+    /* compute_primitive_slopes (FArrayBoxFiniteVolume.hpp:573-589) on the
+     * ghosted box shrunk by one along d */
+    int slo[3] = {b->glo[0], b->glo[1], b->glo[2]};
+    int shi[3] = {b->ghi[0], b->ghi[1], b->ghi[2]};
+    slo[d] += 1;
+    shi[d] -= 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = slo[2]; k <= shi[2]; ++k)
+        for (int j = slo[1]; j <= shi[1]; ++j)
+            for (int i = slo[0]; i <= shi[0]; ++i)
+            {
+                prim_t l, c, r, s;
+                load_prim(prims, ss, -GRHD_RHO,
+                          fab_off(ss, i - (d == 0), j - (d == 1), k - (d == 2)),
+                          &l);
+                load_prim(prims, ss, -GRHD_RHO, fab_off(ss, i, j, k), &c);
+                load_prim(prims, ss, -GRHD_RHO,
+                          fab_off(ss, i + (d == 0), j + (d == 1), k + (d == 2)),
+                          &r);
+                limited_slope_prim(&l, &c, &r, o->limiter_theta, &s);
+                store_prim(slopes, ss, -GRHD_RHO, fab_off(ss, i, j, k), &s);
+            }
+
+    const fab_shape_t is = shape_of(b->lo, b->hi);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+        for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+            for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+            {
+                cons_t fhi, flo, r;
+                face_flux(prims, slopes, ss, bg, b, o, i, j, k, d, &fhi);
+                face_flux(prims, slopes, ss, bg, b, o, i - (d == 0),
+                          j - (d == 1), k - (d == 2), d, &flo);
+                r.D = -inverse_dx * (fhi.D - flo.D);
+                r.tau = -inverse_dx * (fhi.tau - flo.tau);
+                for (int m = 0; m < 3; ++m)
+                    r.S[m] = -inverse_dx * (fhi.S[m] - flo.S[m]);
+                store_cons(rhs_dir, &is, 0, fab_off(&is, i, j, k), &r);
+            }
+    return GRHD_OK;
+}
+
+/* add_static_metric_sources_from_fixed_background_to_conserved_rhs
+ * (:255-298) */
+static int add_static_sources(double *rhs, const fab_shape_t *rs, int cb,
+                              const double *state, const fab_shape_t *ss,
+                              const grhd_box *b, const bg_eval_t *bg,
+                              const grhd_params *o,
+                              unsigned char *source_mask, grhd_error *err)
+{
+    fail_t fail;
+    fail_init(&fail);
+    const double gamma = o->adiabatic_index;
+    const int nx = b->hi[0] - b->lo[0] + 1, ny = b->hi[1] - b->lo[1] + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+        for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+            for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+            {
+                const long long lin =
+                    ((long long)(k - b->lo[2]) * ny + (j - b->lo[1])) * nx +
+                    (i - b->lo[0]);
+                if (source_mask)
+                    source_mask[lin] = 0;
+                geom_t geo;
+                load_geometry(bg, i, j, k, b->dx, b->center, &geo);
+                cons_t c;
+                load_cons(state, ss, cb, fab_off(ss, i, j, k), &c);
+                recovery_t rec;
+                recover_primitive(&c, gamma, geo.gu, o, &rec);
+                if (!rec.success)
+                {
+                    fail_record(&fail, lin, i, j, k, rec.residual);
+                    continue;
+                }
+                prim_t p = rec.prim;
+                enforce_primitive_floors(&p, gamma, geo.g, o);
+                if (is_atmosphere_primitive(&p, o,
+                                            o->metric_source_atmosphere_factor))
+                {
+                    if (source_mask)
+                        source_mask[lin] = GRHD_MASK_SOURCE_ATMOSPHERE;
+                    continue;
+                }
+                derivs_t dv;
+                load_derivs(bg, i, j, k, b->dx, b->center, &dv);
+                cons_t src;
+                static_metric_source_terms(&p, &c, gamma, &geo, &dv, &src);
+                const long long roff = fab_off(rs, i, j, k);
+                FAB(rhs, rs, cb + GRHD_D, roff) += src.D;
+                FAB(rhs, rs, cb + GRHD_TAU, roff) += src.tau;
+                for (int m = 0; m < 3; ++m)
+                    FAB(rhs, rs, cb + GRHD_S1 + m, roff) += src.S[m];
+            }
+    return fail_to_error(&fail, GRHD_ERR_RECOVERY_SOURCES, err);
+}
+
+/* add_metric_volume_time_sources_from_fixed_background_to_conserved_rhs
+ * (:616-657) */
+static void add_volume_time_sources(double *rhs, const fab_shape_t *rs,
+                                    int cb, const double *state,
+                                    const fab_shape_t *ss, const grhd_box *b,
+                                    const grhd_background *bgp, double time,
+                                    double step)
+{
+    const double dt = std_max(step, 1.0e-12);
+    bg_eval_t plus, minus;
+    background_at(bgp, time + dt, &plus);
+    background_at(bgp, time - dt, &minus);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+        for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+            for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+            {
+                geom_t gp, gm;
+                load_geometry(&plus, i, j, k, b->dx, b->center, &gp);
+                load_geometry(&minus, i, j, k, b->dx, b->center, &gm);
+                const double lp = spatial_metric_log_sqrt_det(&gp);
+                const double lm = spatial_metric_log_sqrt_det(&gm);
+                const double dlog = (lp - lm) / (2.0 * dt);
+                cons_t c;
+                load_cons(state, ss, cb, fab_off(ss, i, j, k), &c);
+                /* compute_metric_volume_time_source_terms
+                 * (StaticMetric.hpp:277-291) */
+                cons_t src;
+                src.D = 0.0 - c.D * dlog;
+                src.tau = 0.0 - c.tau * dlog;
+                for (int m = 0; m < 3; ++m)
+                    src.S[m] = 0.0 - c.S[m] * dlog;
+                const long long roff = fab_off(rs, i, j, k);
+                FAB(rhs, rs, cb + GRHD_D, roff) += src.D;
+                FAB(rhs, rs, cb + GRHD_TAU, roff) += src.tau;
+                for (int m = 0; m < 3; ++m)
+                    FAB(rhs, rs, cb + GRHD_S1 + m, roff) += src.S[m];
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* Argument checks                                                    */
+/* ------------------------------------------------------------------ */
+static int check_args(const grhd_box *b, const grhd_params *o,
+                      const grhd_background *bg, int ncomp, int cb)
+{
+    if (!b || !o || !bg)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (cb < 0 || cb + GRHD_NUM_COMPS > ncomp)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (!(b->dx > 0.0))
+        return GRHD_ERR_INVALID_ARGUMENT;
+    for (int d = 0; d < 3; ++d)
+    {
+        if (b->lo[d] > b->hi[d])
+            return GRHD_ERR_INVALID_ARGUMENT;
+        /* the sweeps need two cells of storage beyond the interior */
+        if (b->glo[d] > b->lo[d] - 2 || b->ghi[d] < b->hi[d] + 2)
+            return GRHD_ERR_INVALID_ARGUMENT;
+    }
+    if (bg->kind == GRHD_BG_KERR_SCHILD)
+    {
+        if (bg->spin > bg->mass || bg->spin < -bg->mass)
+            return GRHD_ERR_INVALID_ARGUMENT;
+    }
+    else if (bg->kind == GRHD_BG_CIRCULAR_BINARY_PN)
+    {
+        if (bg->mass_1 <= 0.0 || bg->mass_2 <= 0.0 || bg->separation <= 0.0 ||
+            bg->softening_radius < 0.0)
+            return GRHD_ERR_INVALID_ARGUMENT;
+    }
+    else
+        return GRHD_ERR_INVALID_ARGUMENT;
+    return GRHD_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Public oracle API                                                  */
+/* ------------------------------------------------------------------ */
+
+/* compute_leveldata_flux_rhs_from_fixed_background (:322-363) for one box,
+ * plus the volume-time sources of the time-dependent operator (:702-720). */
+int grhd_oracle_compute_rhs(const grhd_box *b, const grhd_params *o,
+                            const grhd_background *bgp, double time,
+                            double *state, int ncomp, int cb, double *rhs,
+                            unsigned char *source_mask, grhd_error *err)
+{
+    int rc = check_args(b, o, bgp, ncomp, cb);
+    if (rc)
+        return rc;
+    bg_eval_t bg;
+    background_at(bgp, time, &bg);
+    const fab_shape_t ss = shape_of(b->glo, b->ghi);
+    const fab_shape_t rs = shape_of(b->lo, b->hi);
+    const fab_shape_t is = rs;
+
+    /* rhs.setVal(0.0) over every component (:335-336) */
+    memset(rhs, 0, sizeof(double) * (size_t)ncomp * (size_t)rs.npts);
+    double *rhs_dir = (double *)malloc(sizeof(double) * 5 * (size_t)is.npts);
+    double *prims = (double *)malloc(sizeof(double) * 6 * (size_t)ss.npts);
+    double *slopes = (double *)malloc(sizeof(double) * 6 * (size_t)ss.npts);
+    if (!rhs_dir || !prims || !slopes)
+    {
+        free(rhs_dir);
+        free(prims);
+        free(slopes);
+        return GRHD_ERR_INVALID_ARGUMENT;
+    }
+    for (int d = 0; d < 3 && rc == GRHD_OK; ++d)
+    {
+        memset(rhs_dir, 0, sizeof(double) * 5 * (size_t)is.npts);
+        rc = directional_flux_rhs(rhs_dir, state, &ss, cb, b, &bg, o, d,
+                                  1.0 / b->dx, prims, slopes, err);
+        if (rc)
+            break;
+        /* add_conserved_rhs (FArrayBoxFiniteVolume.hpp:912-925) */
+        for (int c = 0; c < 5; ++c)
+        {
+            double *dst = rhs + (size_t)(cb + c) * rs.npts;
+            const double *src = rhs_dir + (size_t)c * is.npts;
+            for (long long n = 0; n < rs.npts; ++n)
+                dst[n] += src[n];
+        }
+    }
+    free(rhs_dir);
+    free(prims);
+    free(slopes);
+    if (rc)
+        return rc;
+    if (source_mask)
+        memset(source_mask, 0, (size_t)is.npts);
+    if (o->use_static_metric_sources)
+    {
+        rc = add_static_sources(rhs, &rs, cb, state, &ss, b, &bg, o,
+                                source_mask, err);
+        if (rc)
+            return rc;
+    }
+    if (o->use_metric_volume_time_sources)
+        add_volume_time_sources(rhs, &rs, cb, state, &ss, b, bgp, time,
+                                o->metric_time_derivative_step);
+    return GRHD_OK;
+}
+
+int grhd_oracle_recover_primitives(const grhd_box *b, const grhd_params *o,
+                                   const grhd_background *bgp, double time,
+                                   double *state, int ncomp, int cb,
+                                   unsigned char *mask,
+                                   grhd_diagnostics *diag)
+{
+    int rc = check_args(b, o, bgp, ncomp, cb);
+    if (rc)
+        return rc;
+    bg_eval_t bg;
+    background_at(bgp, time, &bg);
+    const fab_shape_t ss = shape_of(b->glo, b->ghi);
+    grhd_diagnostics local;
+    memset(&local, 0, sizeof(local));
+    recover_and_apply_atmosphere_box(state, &ss, cb, b, &bg, o, mask, &local);
+    if (diag)
+        *diag = local;
+    return GRHD_OK;
+}
+
+/* add_scaled_conserved_rhs (FArrayBoxFiniteVolume.hpp:927-940) */
+static void add_scaled_conserved(double *state, const fab_shape_t *ss, int cb,
+                                 const double *rhs, const fab_shape_t *rs,
+                                 const grhd_box *b, double scale)
+{
+    static const int order[5] = {GRHD_D, GRHD_TAU, GRHD_S1, GRHD_S2,
+                                 GRHD_S3};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+        for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+            for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+                for (int n = 0; n < 5; ++n)
+                {
+                    const int c = cb + order[n];
+                    FAB(state, ss, c, fab_off(ss, i, j, k)) +=
+                        scale * FAB(rhs, rs, c, fab_off(rs, i, j, k));
+                }
+}
+
+int grhd_oracle_update_conserved(const grhd_box *b, const grhd_params *o,
+                                 const grhd_background *bgp, double *state,
+                                 int ncomp, int cb, const double *rhs,
+                                 double dt, int recover, double recovery_time,
+                                 unsigned char *mask, grhd_diagnostics *diag)
+{
+    int rc = check_args(b, o, bgp, ncomp, cb);
+    if (rc)
+        return rc;
+    const fab_shape_t ss = shape_of(b->glo, b->ghi);
+    const fab_shape_t rs = shape_of(b->lo, b->hi);
+    add_scaled_conserved(state, &ss, cb, rhs, &rs, b, dt);
+    if (diag)
+        memset(diag, 0, sizeof(*diag));
+    if (recover)
+        return grhd_oracle_recover_primitives(b, o, bgp, recovery_time, state,
+                                              ncomp, cb, mask, diag);
+    return GRHD_OK;
+}
+
+/* compute_max_inverse_dt_from_fixed_background (:364-412) */
+int grhd_oracle_compute_max_inverse_dt(const grhd_box *b,
+                                       const grhd_params *o,
+                                       const grhd_background *bgp,
+                                       double time, const double *state,
+                                       int ncomp, int cb, double *out,
+                                       grhd_error *err)
+{
+    int rc = check_args(b, o, bgp, ncomp, cb);
+    if (rc)
+        return rc;
+    bg_eval_t bg;
+    background_at(bgp, time, &bg);
+    const fab_shape_t ss = shape_of(b->glo, b->ghi);
+    const double gamma = o->adiabatic_index;
+    const double inverse_dx = 1.0 / b->dx;
+    const int nx = b->hi[0] - b->lo[0] + 1, ny = b->hi[1] - b->lo[1] + 1;
+    fail_t fail;
+    fail_init(&fail);
+    double max_inverse_dt = 0.0;
+#pragma omp parallel for collapse(2) schedule(static) \
+    reduction(max : max_inverse_dt)
+    for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+        for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+            for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+            {
+                geom_t geo;
+                load_geometry(&bg, i, j, k, b->dx, b->center, &geo);
+                cons_t c;
+                load_cons(state, &ss, cb, fab_off(&ss, i, j, k), &c);
+                recovery_t rec;
+                recover_primitive(&c, gamma, geo.gu, o, &rec);
+                if (!rec.success)
+                {
+                    const long long lin =
+                        ((long long)(k - b->lo[2]) * ny + (j - b->lo[1])) *
+                            nx +
+                        (i - b->lo[0]);
+                    fail_record(&fail, lin, i, j, k, rec.residual);
+                    continue;
+                }
+                prim_t p = rec.prim;
+                enforce_primitive_floors(&p, gamma, geo.g, o);
+                if (is_atmosphere_primitive(&p, o, 1.0 + 1.0e-12))
+                    continue;
+                double cell = 0.0;
+                for (int d = 0; d < 3; ++d)
+                {
+                    double lo, hi;
+                    characteristic_speeds(&p, gamma, geo.g, geo.gu, d,
+                                          geo.lapse, geo.shift, &lo, &hi);
+                    const double ms = std_max(fabs(lo), fabs(hi));
+                    cell += ms * inverse_dx;
+                }
+                max_inverse_dt = std_max(max_inverse_dt, cell);
+            }
+    rc = fail_to_error(&fail, GRHD_ERR_RECOVERY_CFL, err);
+    if (rc)
+        return rc;
+    *out = max_inverse_dt;
+    return GRHD_OK;
+}
+
+int grhd_oracle_compute_stable_dt(const grhd_box *b, const grhd_params *o,
+                                  const grhd_background *bgp, double time,
+                                  const double *state, int ncomp, int cb,
+                                  double cfl, double *out, grhd_error *err)
+{
+    if (cfl <= 0.0)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    double m = 0.0;
+    int rc = grhd_oracle_compute_max_inverse_dt(b, o, bgp, time, state, ncomp,
+                                                cb, &m, err);
+    if (rc)
+        return rc;
+    *out = cfl / std_max(m, 1.0e-14);
+    return GRHD_OK;
+}
+
+static double *fab_clone(const double *state, size_t n)
+{
+    double *c = (double *)malloc(sizeof(double) * n);
+    if (c)
+        memcpy(c, state, sizeof(double) * n);
+    return c;
+}
+
+/* copy the interior conserved components of src into dst */
+static void copy_interior_conserved(double *dst, const double *src,
+                                    const fab_shape_t *ss, int cb,
+                                    const grhd_box *b)
+{
+    for (int c = 0; c < 5; ++c)
+        for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+            for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+                for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+                {
+                    const long long off = fab_off(ss, i, j, k);
+                    FAB(dst, ss, cb + c, off) = FAB(src, ss, cb + c, off);
+                }
+}
+
+/* advance_ssprk2 (:563-589 and :782-809) */
+int grhd_oracle_advance_ssprk2(const grhd_box *b, const grhd_params *o,
+                               const grhd_background *bgp, double time,
+                               double dt, double *state, int ncomp, int cb,
+                               unsigned char *mask, grhd_diagnostics *diag,
+                               grhd_error *err)
+{
+    int rc = check_args(b, o, bgp, ncomp, cb);
+    if (rc)
+        return rc;
+    const fab_shape_t ss = shape_of(b->glo, b->ghi);
+    const fab_shape_t rs = shape_of(b->lo, b->hi);
+    const size_t nstate = (size_t)ncomp * ss.npts;
+    const size_t nrhs = (size_t)ncomp * rs.npts;
+    double *old_state = fab_clone(state, nstate);
+    double *stage = fab_clone(state, nstate);
+    double *rhs = (double *)calloc(nrhs, sizeof(double));
+    if (!old_state || !stage || !rhs)
+    {
+        free(old_state);
+        free(stage);
+        free(rhs);
+        return GRHD_ERR_INVALID_ARGUMENT;
+    }
+    rc = grhd_oracle_compute_rhs(b, o, bgp, time, state, ncomp, cb, rhs, NULL,
+                                 err);
+    if (!rc)
+        rc = grhd_oracle_update_conserved(b, o, bgp, stage, ncomp, cb, rhs, dt,
+                                          1, time + dt, NULL, NULL);
+    if (!rc)
+        rc = grhd_oracle_compute_rhs(b, o, bgp, time + dt, stage, ncomp, cb,
+                                     rhs, NULL, err);
+    if (!rc)
+    {
+        /* combine_ssprk2_conserved_state (FArrayBoxFiniteVolume.hpp:942-969) */
+        for (int c = 0; c < 5; ++c)
+            for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+                for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+                    for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+                    {
+                        const long long off = fab_off(&ss, i, j, k);
+                        FAB(state, &ss, cb + c, off) =
+                            0.5 * (FAB(old_state, &ss, cb + c, off) +
+                                   FAB(stage, &ss, cb + c, off) +
+                                   dt * FAB(rhs, &rs, cb + c,
+                                            fab_off(&rs, i, j, k)));
+                    }
+        rc = grhd_oracle_recover_primitives(b, o, bgp, time + dt, state, ncomp,
+                                            cb, mask, diag);
+    }
+    free(old_state);
+    free(stage);
+    free(rhs);
+    return rc;
+}
+
+/* Classical RK4 driven through the per-stage seam (RHS, then recovery of
+ * each stage state), accumulating (((U0 + dt/6 k1) + dt/3 k2) + dt/3 k3)
+ * + dt/6 k4. */
+int grhd_oracle_advance_rk4(const grhd_box *b, const grhd_params *o,
+                            const grhd_background *bgp, double time,
+                            double dt, int nsteps, double *state, int ncomp,
+                            int cb, unsigned char *mask,
+                            grhd_diagnostics *diag, grhd_error *err)
+{
+    int rc = check_args(b, o, bgp, ncomp, cb);
+    if (rc)
+        return rc;
+    const fab_shape_t ss = shape_of(b->glo, b->ghi);
+    const fab_shape_t rs = shape_of(b->lo, b->hi);
+    const size_t nstate = (size_t)ncomp * ss.npts;
+    const size_t nrhs = (size_t)ncomp * rs.npts;
+    double *u0 = (double *)malloc(sizeof(double) * nstate);
+    double *acc = (double *)malloc(sizeof(double) * nstate);
+    double *stage = (double *)malloc(sizeof(double) * nstate);
+    double *k = (double *)calloc(nrhs, sizeof(double));
+    if (!u0 || !acc || !stage || !k)
+    {
+        free(u0);
+        free(acc);
+        free(stage);
+        free(k);
+        return GRHD_ERR_INVALID_ARGUMENT;
+    }
+    grhd_diagnostics total;
+    memset(&total, 0, sizeof(total));
+    const double c2 = dt / 2.0, c3 = dt / 3.0, c6 = dt / 6.0;
+    for (int step = 0; step < nsteps && !rc; ++step)
+    {
+        const double t = time + step * dt;
+        grhd_diagnostics dg;
+        memcpy(u0, state, sizeof(double) * nstate);
+        memcpy(acc, state, sizeof(double) * nstate);
+        memcpy(stage, state, sizeof(double) * nstate);
+        /* stage 1 */
+        rc = grhd_oracle_compute_rhs(b, o, bgp, t, state, ncomp, cb, k, NULL,
+                                     err);
+        if (rc)
+            break;
+        add_scaled_conserved(acc, &ss, cb, k, &rs, b, c6);
+        copy_interior_conserved(stage, u0, &ss, cb, b);
+        add_scaled_conserved(stage, &ss, cb, k, &rs, b, c2);
+        grhd_oracle_recover_primitives(b, o, bgp, t + c2, stage, ncomp, cb,
+                                       NULL, &dg);
+        total.num_cells += dg.num_cells;
+        total.num_failed_recoveries += dg.num_failed_recoveries;
+        total.num_floored_primitives += dg.num_floored_primitives;
+        total.num_conserved_resets += dg.num_conserved_resets;
+        /* stage 2 */
+        rc = grhd_oracle_compute_rhs(b, o, bgp, t + c2, stage, ncomp, cb, k,
+                                     NULL, err);
+        if (rc)
+            break;
+        add_scaled_conserved(acc, &ss, cb, k, &rs, b, c3);
+        copy_interior_conserved(stage, u0, &ss, cb, b);
+        add_scaled_conserved(stage, &ss, cb, k, &rs, b, c2);
+        grhd_oracle_recover_primitives(b, o, bgp, t + c2, stage, ncomp, cb,
+                                       NULL, &dg);
+        total.num_cells += dg.num_cells;
+        total.num_failed_recoveries += dg.num_failed_recoveries;
+        total.num_floored_primitives += dg.num_floored_primitives;
+        total.num_conserved_resets += dg.num_conserved_resets;
+        /* stage 3 */
+        rc = grhd_oracle_compute_rhs(b, o, bgp, t + c2, stage, ncomp, cb, k,
+                                     NULL, err);
+        if (rc)
+            break;
+        add_scaled_conserved(acc, &ss, cb, k, &rs, b, c3);
+        copy_interior_conserved(stage, u0, &ss, cb, b);
+        add_scaled_conserved(stage, &ss, cb, k, &rs, b, dt);
+        grhd_oracle_recover_primitives(b, o, bgp, t + dt, stage, ncomp, cb,
+                                       NULL, &dg);
+        total.num_cells += dg.num_cells;
+        total.num_failed_recoveries += dg.num_failed_recoveries;
+        total.num_floored_primitives += dg.num_floored_primitives;
+        total.num_conserved_resets += dg.num_conserved_resets;
+        /* stage 4 */
+        rc = grhd_oracle_compute_rhs(b, o, bgp, t + dt, stage, ncomp, cb, k,
+                                     NULL, err);
+        if (rc)
+            break;
+        add_scaled_conserved(acc, &ss, cb, k, &rs, b, c6);
+        copy_interior_conserved(state, acc, &ss, cb, b);
+        grhd_oracle_recover_primitives(b, o, bgp, t + dt, state, ncomp, cb,
+                                       mask, &dg);
+        total.num_cells += dg.num_cells;
+        total.num_failed_recoveries += dg.num_failed_recoveries;
+        total.num_floored_primitives += dg.num_floored_primitives;
+        total.num_conserved_resets += dg.num_conserved_resets;
+    }
+    if (diag)
+        *diag = total;
+    free(u0);
+    free(acc);
+    free(stage);
+    free(k);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Point functions for the unit tests                                 */
+/* ------------------------------------------------------------------ */
+
+/* Full ADM vars at one point.  out (88 doubles): gamma[9], K_tensor[9], K,
+ * lapse, shift[3], d1_gamma[27] ([i][j][k]), d1_lapse[3], d1_shift[9]
+ * ([i][j]), then geometry gu[9] and derivs log_sqrt_det[3], and
+ * log_sqrt_det (1), determinant (1) -> 9+9+1+1+3+27+3+9+9+3+1+1 = 76. */
+int grhd_oracle_adm_point(const grhd_background *bgp, double time,
+                          const double x[3], double *out)
+{
+    bg_eval_t bg;
+    background_at(bgp, time, &bg);
+    adm_t v;
+    background_metric(&bg, x, &v);
+    geom_t g;
+    geometry_from_adm(&v, &g);
+    derivs_t d;
+    derivs_from_adm(&v, &d);
+    double *o = out;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            *o++ = v.gamma[i][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            *o++ = v.K_tensor[i][j];
+    *o++ = v.K;
+    *o++ = v.lapse;
+    for (int i = 0; i < 3; ++i)
+        *o++ = v.shift[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+                *o++ = v.d1_gamma[i][j][k];
+    for (int i = 0; i < 3; ++i)
+        *o++ = v.d1_lapse[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            *o++ = v.d1_shift[i][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            *o++ = g.gu[i][j];
+    for (int i = 0; i < 3; ++i)
+        *o++ = d.log_sqrt_det[i];
+    *o++ = spatial_metric_log_sqrt_det(&g);
+    *o++ = compute_determinant_sym(g.g);
+    return (int)(o - out);
+}
+
+/* Cell geometry over the FAB storage box, same layout as
+ * grhd_cell_geometry_box. */
+int grhd_oracle_cell_geometry_box(const grhd_box *b,
+                                  const grhd_background *bgp, double time,
+                                  double *out)
+{
+    bg_eval_t bg;
+    background_at(bgp, time, &bg);
+    const fab_shape_t ss = shape_of(b->glo, b->ghi);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b->glo[2]; k <= b->ghi[2]; ++k)
+        for (int j = b->glo[1]; j <= b->ghi[1]; ++j)
+            for (int i = b->glo[0]; i <= b->ghi[0]; ++i)
+            {
+                geom_t g;
+                load_geometry(&bg, i, j, k, b->dx, b->center, &g);
+                const long long off = fab_off(&ss, i, j, k);
+                const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2,
+                                                               2};
+                for (int n = 0; n < 6; ++n)
+                {
+                    out[(size_t)n * ss.npts + off] = g.g[ii[n]][jj[n]];
+                    out[(size_t)(6 + n) * ss.npts + off] = g.gu[ii[n]][jj[n]];
+                }
+                out[(size_t)12 * ss.npts + off] = g.lapse;
+                for (int n = 0; n < 3; ++n)
+                    out[(size_t)(13 + n) * ss.npts + off] = g.shift[n];
+            }
+    return GRHD_OK;
+}
+
+static void unpack_geom(const double *g9, const double *gu9, double lapse,
+                        const double *shift3, geom_t *g)
+{
+    memset(g, 0, sizeof(*g));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            g->g[i][j] = g9[3 * i + j];
+            g->gu[i][j] = gu9[3 * i + j];
+        }
+    g->lapse = lapse;
+    for (int i = 0; i < 3; ++i)
+        g->shift[i] = shift3[i];
+}
+
+/* prim (6: rho eps P v1 v2 v3) -> cons (5: D S1 S2 S3 tau) */
+void grhd_oracle_prim2con(const double *prim, double gamma, const double *g9,
+                          double *cons)
+{
+    prim_t p = {prim[0], prim[1], prim[2], {prim[3], prim[4], prim[5]}};
+    double g[3][3];
+    for (int i = 0; i < 9; ++i)
+        g[i / 3][i % 3] = g9[i];
+    cons_t c;
+    compute_conserved(&p, gamma, g, &c);
+    cons[0] = c.D;
+    cons[1] = c.S[0];
+    cons[2] = c.S[1];
+    cons[3] = c.S[2];
+    cons[4] = c.tau;
+}
+
+/* cons (5) -> prim (6); returns success; info = {iterations, residual} */
+int grhd_oracle_con2prim(const double *cons, const grhd_params *o,
+                         const double *gu9, double *prim, double *info)
+{
+    cons_t c = {cons[0], {cons[1], cons[2], cons[3]}, cons[4]};
+    double gu[3][3];
+    for (int i = 0; i < 9; ++i)
+        gu[i / 3][i % 3] = gu9[i];
+    recovery_t r;
+    recover_primitive(&c, o->adiabatic_index, gu, o, &r);
+    prim[0] = r.prim.rho;
+    prim[1] = r.prim.eps;
+    prim[2] = r.prim.pressure;
+    prim[3] = r.prim.v[0];
+    prim[4] = r.prim.v[1];
+    prim[5] = r.prim.v[2];
+    info[0] = r.iterations;
+    info[1] = r.residual;
+    return r.success;
+}
+
+/* floors applied in place to prim (6) */
+void grhd_oracle_floors(double *prim, const grhd_params *o, const double *g9)
+{
+    prim_t p = {prim[0], prim[1], prim[2], {prim[3], prim[4], prim[5]}};
+    double g[3][3];
+    for (int i = 0; i < 9; ++i)
+        g[i / 3][i % 3] = g9[i];
+    enforce_primitive_floors(&p, o->adiabatic_index, g, o);
+    prim[0] = p.rho;
+    prim[1] = p.eps;
+    prim[2] = p.pressure;
+    prim[3] = p.v[0];
+    prim[4] = p.v[1];
+    prim[5] = p.v[2];
+}
+
+double grhd_oracle_limited_slope(double l, double c, double r, double theta)
+{
+    return limited_slope(l, c, r, theta);
+}
+
+double grhd_oracle_minmod(double a, double b) { return minmod2(a, b); }
+
+/* HLLE flux between two primitive states on a face geometry.
+ * face = g9, gu9, lapse, shift3.  speeds (4) = {lL, uL, lR, uR}. */
+void grhd_oracle_hlle(const double *pl6, const double *pr6, double gamma,
+                      const double *g9, const double *gu9, double lapse,
+                      const double *shift3, int d, double *flux5,
+                      double *speeds4)
+{
+    geom_t g;
+    unpack_geom(g9, gu9, lapse, shift3, &g);
+    prim_t pl = {pl6[0], pl6[1], pl6[2], {pl6[3], pl6[4], pl6[5]}};
+    prim_t pr = {pr6[0], pr6[1], pr6[2], {pr6[3], pr6[4], pr6[5]}};
+    cons_t cl, cr, f;
+    compute_conserved(&pl, gamma, g.g, &cl);
+    compute_conserved(&pr, gamma, g.g, &cr);
+    hlle_flux(&pl, &pr, &cl, &cr, gamma, g.g, g.gu, d, g.lapse, g.shift, &f);
+    characteristic_speeds(&pl, gamma, g.g, g.gu, d, g.lapse, g.shift,
+                          &speeds4[0], &speeds4[1]);
+    characteristic_speeds(&pr, gamma, g.g, g.gu, d, g.lapse, g.shift,
+                          &speeds4[2], &speeds4[3]);
+    flux5[0] = f.D;
+    flux5[1] = f.S[0];
+    flux5[2] = f.S[1];
+    flux5[3] = f.S[2];
+    flux5[4] = f.tau;
+}
+
+/* Directional flux (compute_flux, Valencia.hpp:230-249) of a primitive
+ * state with lapse/shift: flux5 = {D, S1, S2, S3, tau}. */
+void grhd_oracle_flux(const double *p6, double gamma, const double *g9,
+                      double lapse, const double *shift3, int d,
+                      double *flux5)
+{
+    double g[3][3];
+    for (int i = 0; i < 9; ++i)
+        g[i / 3][i % 3] = g9[i];
+    prim_t p = {p6[0], p6[1], p6[2], {p6[3], p6[4], p6[5]}};
+    cons_t c, f;
+    compute_conserved(&p, gamma, g, &c);
+    compute_flux(&p, &c, d, lapse, shift3, &f);
+    flux5[0] = f.D;
+    flux5[1] = f.S[0];
+    flux5[2] = f.S[1];
+    flux5[3] = f.S[2];
+    flux5[4] = f.tau;
+}
+
+/* Static-metric source terms at a point of the background: prim (6),
+ * cons (5) given; use_K = 0 zeroes K_ij (the reference's
+ * geometry_without_k check, GRHDFixedBGTest.cpp:682-712).  out = {D, S1,
+ * S2, S3, tau, tau_K_contraction}. */
+void grhd_oracle_sources_point(const grhd_background *bgp, double time,
+                               const double x[3], const double *p6,
+                               const double *c5, double gamma, int use_K,
+                               double *out)
+{
+    bg_eval_t bg;
+    background_at(bgp, time, &bg);
+    adm_t v;
+    background_metric(&bg, x, &v);
+    geom_t g;
+    geometry_from_adm(&v, &g);
+    derivs_t d;
+    derivs_from_adm(&v, &d);
+    if (!use_K)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                g.K[i][j] = 0.0;
+    prim_t p = {p6[0], p6[1], p6[2], {p6[3], p6[4], p6[5]}};
+    cons_t c = {c5[0], {c5[1], c5[2], c5[3]}, c5[4]};
+    cons_t s;
+    static_metric_source_terms(&p, &c, gamma, &g, &d, &s);
+    double T[3][3];
+    geom_t gk;
+    geometry_from_adm(&v, &gk);
+    spatial_stress_UU(&p, gamma, &gk, T);
+    double contraction = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            contraction += gk.lapse * T[i][j] * gk.K[i][j];
+    out[0] = s.D;
+    out[1] = s.S[0];
+    out[2] = s.S[1];
+    out[3] = s.S[2];
+    out[4] = s.tau;
+    out[5] = contraction;
+}
+
+/* d/dt log sqrt det gamma by the reference's central difference
+ * (fixed_background_log_sqrt_det_time_derivative, :616-631). */
+double grhd_oracle_log_sqrt_det_dt(const grhd_background *bgp, double time,
+                                   const int iv[3], double dx,
+                                   const double center[3], double step)
+{
+    const double dt = std_max(step, 1.0e-12);
+    bg_eval_t plus, minus;
+    background_at(bgp, time + dt, &plus);
+    background_at(bgp, time - dt, &minus);
+    geom_t gp, gm;
+    load_geometry(&plus, iv[0], iv[1], iv[2], dx, center, &gp);
+    load_geometry(&minus, iv[0], iv[1], iv[2], dx, center, &gm);
+    return (spatial_metric_log_sqrt_det(&gp) - spatial_metric_log_sqrt_det(&gm)) /
+           (2.0 * dt);
+}
+
+int grhd_oracle_num_threads(void) { return omp_get_max_threads(); }
