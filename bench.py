@@ -51,6 +51,18 @@ def units_per_launch(cls, n, g):
     return 0
 
 
+def measured_traffic(cls, n):
+    """HBM bytes per launch of kernel class `cls` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), for
+    the 128^3 box it was measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if n != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data.get("bytes_per_launch", {}).get(cls)
+
+
 def cpu_baseline(w):
     """Oracle RK4 step of the same box on this host's cores."""
     import numpy as np
@@ -69,6 +81,22 @@ def cpu_baseline(w):
                       "structure" % (w.box.interior_shape[0], cells, t1 - t0)}
 
 
+def calibrate_copy():
+    """Five dst.copy_(src) of 64 Mi float64 (512 MiB read + 512 MiB write
+    per dispatch): the known-size kernel that converts rocprofv3's
+    FETCH_SIZE / WRITE_SIZE readings into bytes."""
+    import torch
+    n = 64 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print("copy calibration: 5 x %d bytes read + written" % (8 * n),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +104,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 512 MiB device copy (x5), the "
+                         "reference kernel for scaling FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not "
               "supported (the box would shard by z-slabs across GPUs, which "
@@ -128,7 +162,8 @@ def main():
     achieved = ALG_BYTES[dom] * units / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(dom, args.n),
                 "avg_launch_ms": ms / nl,
                 "alg_bytes_per_unit": ALG_BYTES[dom], "units_per_launch": units}
 

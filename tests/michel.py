@@ -1,0 +1,219 @@
+"""Michel accretion profile and LevelData initial state, restated from the
+reference's tests (test infrastructure):
+
+  Tests/GRHDMichelAccretion/GRHDMichelAccretion.cpp:30-299 (profile)
+  Tests/GRHDLevelDataMichelAccretion/GRHDLevelDataMichelAccretion.cpp:32-436
+    (config, atmosphere/recovery options, cell fill)
+
+Scalar Python floats (IEEE binary64 with the C library's exp/pow/sqrt), in
+the reference's operation order.
+"""
+import math
+
+import numpy as np
+
+import oracle
+from grdzhadzha_amd.options import (AtmosphereOptions, BoxSpec,
+                                    KerrSchildParams, OperatorOptions,
+                                    RecoveryOptions)
+
+
+class MichelConfig:
+    def __init__(self, gamma=5.0 / 3.0, sonic_radius=8.0, sonic_density=1e-6,
+                 mass=1.0, root_tolerance=1e-13):
+        self.mass = mass
+        self.gamma = gamma
+        self.sonic_radius = sonic_radius
+        self.sonic_density = sonic_density
+        self.root_tolerance = root_tolerance
+
+
+def schwarzschild_f(mass, radius):
+    return 1.0 - 2.0 * mass / radius
+
+
+def pressure_from_density(K, cfg, rho):
+    return K * math.pow(rho, cfg.gamma)
+
+
+def specific_enthalpy(K, cfg, rho):
+    p = pressure_from_density(K, cfg, rho)
+    return 1.0 + cfg.gamma * p / ((cfg.gamma - 1.0) * rho)
+
+
+def make_constants(cfg):
+    c = {}
+    c["cs2"] = cfg.mass / (2.0 * cfg.sonic_radius - 3.0 * cfg.mass)
+    c["ur"] = -math.sqrt(cfg.mass / (2.0 * cfg.sonic_radius))
+    q = c["cs2"] / (cfg.gamma - c["cs2"] * cfg.gamma / (cfg.gamma - 1.0))
+    c["K"] = q / math.pow(cfg.sonic_density, cfg.gamma - 1.0)
+    h_c = specific_enthalpy(c["K"], cfg, cfg.sonic_density)
+    f_c = schwarzschild_f(cfg.mass, cfg.sonic_radius)
+    c["bernoulli"] = h_c * math.sqrt(f_c + c["ur"] * c["ur"])
+    c["mass_flux"] = (cfg.sonic_radius * cfg.sonic_radius *
+                      cfg.sonic_density * c["ur"])
+    return c
+
+
+def bernoulli_residual(c, cfg, radius, rho):
+    f = schwarzschild_f(cfg.mass, radius)
+    ur = c["mass_flux"] / (radius * radius * rho)
+    h = specific_enthalpy(c["K"], cfg, rho)
+    return h * h * (f + ur * ur) - c["bernoulli"] * c["bernoulli"]
+
+
+def bisect(c, cfg, radius, lower, upper):
+    f_lower = bernoulli_residual(c, cfg, radius, lower)
+    f_upper = bernoulli_residual(c, cfg, radius, upper)
+    if f_lower == 0.0:
+        return lower
+    if f_upper == 0.0:
+        return upper
+    if f_lower * f_upper > 0.0:
+        raise RuntimeError("Michel density bracket has no sign change")
+    for _ in range(200):
+        mid = 0.5 * (lower + upper)
+        f_mid = bernoulli_residual(c, cfg, radius, mid)
+        if abs(f_mid) < cfg.root_tolerance or \
+                abs(upper - lower) <= cfg.root_tolerance * max(1e-300,
+                                                               abs(mid)):
+            return mid
+        if f_lower * f_mid <= 0.0:
+            upper = mid
+        else:
+            lower = mid
+            f_lower = f_mid
+    return 0.5 * (lower + upper)
+
+
+def solve_density(c, cfg, radius):
+    if abs(radius - cfg.sonic_radius) < 1e-12:
+        return cfg.sonic_density
+    log_min = math.log(cfg.sonic_density * 1e-6)
+    log_max = math.log(cfg.sonic_density * 1e6)
+    samples = 6000
+    brackets = []
+    last_rho = math.exp(log_min)
+    last_value = bernoulli_residual(c, cfg, radius, last_rho)
+    for i in range(1, samples + 1):
+        w = float(i) / samples
+        rho = math.exp(log_min + w * (log_max - log_min))
+        value = bernoulli_residual(c, cfg, radius, rho)
+        if last_value == 0.0 or value == 0.0 or last_value * value < 0.0:
+            brackets.append((last_rho, rho, 0.5 * (last_rho + rho)))
+        last_rho, last_value = rho, value
+    if not brackets:
+        raise RuntimeError("Michel density solve found no root bracket")
+    inner = radius < cfg.sonic_radius
+    sel = [b for b in brackets
+           if (b[2] > cfg.sonic_density if inner else b[2] < cfg.sonic_density)]
+    b = sel[0] if sel else brackets[0]
+    return bisect(c, cfg, radius, b[0], b[1])
+
+
+def geometry_on_x_axis(cfg, radius):
+    a = oracle.adm_point(KerrSchildParams(cfg.mass, 0.0), [radius, 0.0, 0.0])
+    return a["gamma"], a["gamma_UU"], a["lapse"], a["shift"]
+
+
+def make_point(c, cfg, radius):
+    """GRHDMichelAccretion.cpp make_point: rho, P, eps, u^r, v^r (KS)."""
+    f = schwarzschild_f(cfg.mass, radius)
+    if f <= 0.0:
+        raise RuntimeError("Michel profile radius is inside the horizon")
+    rho = solve_density(c, cfg, radius)
+    p = pressure_from_density(c["K"], cfg, rho)
+    eps = p / ((cfg.gamma - 1.0) * rho)
+    ur = c["mass_flux"] / (radius * radius * rho)
+    g, gu, lapse, shift = geometry_on_x_axis(cfg, radius)
+    shift_L = [0.0, 0.0, 0.0]
+    for i in range(3):
+        for j in range(3):
+            shift_L[i] += g[i][j] * shift[j]
+    shift_sq = 0.0
+    for m in range(3):
+        for n in range(3):
+            shift_sq += g[m][n] * shift[m] * shift[n]
+    g_tt = -lapse * lapse + shift_sq
+    u_t = -math.sqrt(f + ur * ur)
+    u_upper_t = (u_t - shift_L[0] * ur) / g_tt
+    W = lapse * u_upper_t
+    if not (W > 1.0) or not math.isfinite(W):
+        raise RuntimeError("Michel Kerr-Schild Lorentz factor invalid")
+    vr = ur / W + shift[0] / lapse
+    h = 1.0 + eps + p / rho
+    return {"radius": radius, "rho": rho, "pressure": p, "eps": eps,
+            "ur": ur, "vr": vr, "g": g, "gu": gu,
+            "mass_flux": radius * radius * rho * ur,
+            "bernoulli": h * math.sqrt(f + ur * ur)}
+
+
+# ------------------------------------------------------------------ LevelData
+def leveldata_setup(num_cells=18, dx=2.0, ghosts=2):
+    """GRHDLevelDataMichelAccretion.cpp:32-45, 142-171, 403-436, 572-596."""
+    cfg = MichelConfig()
+    c = make_constants(cfg)
+    atm = AtmosphereOptions(rho_floor=1e-12, pressure_floor=1e-16,
+                            max_velocity_squared=0.8)
+    rec = RecoveryOptions(max_iterations=100, tolerance=1e-12,
+                          pressure_floor=atm.pressure_floor,
+                          max_velocity_squared=atm.max_velocity_squared)
+    opts = OperatorOptions(adiabatic_index=cfg.gamma, atmosphere=atm,
+                           recovery=rec, limiter_theta=1.2)
+    half = 0.5 * num_cells * dx
+    box = BoxSpec((0, 0, 0), (num_cells - 1,) * 3, ghosts, dx,
+                  (half, half, half))
+    bg = KerrSchildParams(cfg.mass, 0.0)
+    geo = oracle.Oracle(bg, box, opts).cell_geometry()
+    state = np.zeros((11,) + box.storage_shape)
+    gamma = cfg.gamma
+    cache = {}
+    nz, ny, nx = box.storage_shape
+    glo = box.glo
+    atmosphere_radius = 4.0
+    for kk in range(nz):
+        for jj in range(ny):
+            for ii in range(nx):
+                i, j, k = glo[0] + ii, glo[1] + jj, glo[2] + kk
+                x = (i + 0.5) * dx - half
+                y = (j + 0.5) * dx - half
+                z = (k + 0.5) * dx - half
+                r = math.sqrt(x * x + y * y + z * z)
+                g6 = geo[0:6, kk, jj, ii]
+                g = np.array([[g6[0], g6[1], g6[2]], [g6[1], g6[3], g6[4]],
+                              [g6[2], g6[4], g6[5]]])
+                if r < atmosphere_radius:
+                    rho = atm.rho_floor
+                    p = atm.pressure_floor
+                    eps = p / ((gamma - 1.0) * rho)
+                    prim = np.array([rho, eps, p, 0.0, 0.0, 0.0])
+                else:
+                    if r not in cache:
+                        cache[r] = make_point(c, cfg, r)
+                    pt = cache[r]
+                    pos = (x, y, z)
+                    prim = np.array([pt["rho"], pt["eps"], pt["pressure"]] +
+                                    [pt["vr"] * pos[d] / r for d in range(3)])
+                    prim = oracle.floors(prim, opts, g)
+                cons = oracle.prim2con(prim, gamma, g)
+                state[0:5, kk, jj, ii] = cons
+                state[5:11, kk, jj, ii] = prim
+    return cfg, box, bg, opts, state
+
+
+def diagnostics(box, state, geo):
+    """compute_diagnostics (GRHDLevelDataMichelAccretion.cpp:479-530)."""
+    sl = box.interior_slices()
+    mass = 0.0
+    # BoxIterator order: i fastest -- sum in that order
+    D = state[0][sl].ravel()
+    for v in D:
+        mass += v
+    rho = state[5][sl]
+    p = state[7][sl]
+    v = state[8:11][(slice(None),) + sl]
+    g = geo[0:6][(slice(None),) + sl]
+    G = np.array([[g[0], g[1], g[2]], [g[1], g[3], g[4]], [g[2], g[4], g[5]]])
+    v2 = np.einsum("ij...,i...,j...->...", G, v, v)
+    return {"mass": mass, "max_rho": float(rho.max()),
+            "max_pressure": float(p.max()), "max_v2": float(v2.max())}
