@@ -179,6 +179,45 @@ __device__ void block_add_counts(DevStatus *st, unsigned long long c0,
     }
 }
 
+/* The e-th storage cell outside the interior box: whole z-planes below
+ * and above the interior, then per interior plane the rows below and
+ * above the interior in y, then the x-ghost cells of interior rows. */
+__device__ __forceinline__ void exterior_cell(const BoxDev &b, long long e,
+                                              int &i, int &j, int &k)
+{
+    const long long plane = (long long)b.gn[0] * b.gn[1];
+    const long long zlo = (long long)(b.lo[2] - b.glo[2]) * plane;
+    const long long zhi = (long long)(b.ghi[2] - b.hi[2]) * plane;
+    if (e < zlo + zhi)
+    {
+        const long long q = (e < zlo) ? e : e - zlo;
+        k = ((e < zlo) ? b.glo[2] : b.hi[2] + 1) + (int)(q / plane);
+        const long long r = q % plane;
+        j = b.glo[1] + (int)(r / b.gn[0]);
+        i = b.glo[0] + (int)(r % b.gn[0]);
+        return;
+    }
+    e -= zlo + zhi;
+    const long long per_plane = plane - (long long)b.in[0] * b.in[1];
+    k = b.lo[2] + (int)(e / per_plane);
+    long long r = e % per_plane;
+    const long long ylo = (long long)(b.lo[1] - b.glo[1]) * b.gn[0];
+    const long long yhi = (long long)(b.ghi[1] - b.hi[1]) * b.gn[0];
+    if (r < ylo + yhi)
+    {
+        const long long q = (r < ylo) ? r : r - ylo;
+        j = ((r < ylo) ? b.glo[1] : b.hi[1] + 1) + (int)(q / b.gn[0]);
+        i = b.glo[0] + (int)(q % b.gn[0]);
+        return;
+    }
+    r -= ylo + yhi;
+    const int gx = b.lo[0] - b.glo[0];
+    const int per_row = b.gn[0] - b.in[0];
+    j = b.lo[1] + (int)(r / per_row);
+    const int q = (int)(r % per_row);
+    i = (q < gx) ? b.glo[0] + q : b.hi[0] + 1 + (q - gx);
+}
+
 __global__ void k_status_init(DevStatus *st)
 {
     if (threadIdx.x == 0)
@@ -202,12 +241,23 @@ __global__ void __launch_bounds__(kBlock)
             double *W, unsigned char *wbad, DevStatus *st,
             unsigned long long eval)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= b.gnpts)
-        return;
-    const int i = b.glo[0] + (int)(idx % b.gn[0]);
-    const int j = b.glo[1] + (int)((idx / b.gn[0]) % b.gn[1]);
-    const int k = b.glo[2] + (int)(idx / ((long long)b.gn[0] * b.gn[1]));
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (EXTERIOR_ONLY)
+    {
+        if (tid >= b.gnpts - b.inpts)
+            return;
+        exterior_cell(b, tid, i, j, k);
+    }
+    else
+    {
+        if (tid >= b.gnpts)
+            return;
+        i = b.glo[0] + (int)(tid % b.gn[0]);
+        j = b.glo[1] + (int)((tid / b.gn[0]) % b.gn[1]);
+        k = b.glo[2] + (int)(tid / ((long long)b.gn[0] * b.gn[1]));
+    }
+    const long long idx = goff(b, i, j, k);
     const int iv[3] = {i, j, k};
     int src[3] = {i, j, k};
     bool physical = false;
@@ -297,71 +347,203 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 /* ------------------------------------------------------------------ */
-/* k_faces<D>: one MUSCL-HLLE flux per face of direction D.           */
-/* Face box: interior box with hi[D] + 1; face (f) sits between cells */
-/* f - e_D (left) and f (right).                                      */
+/* k_faces<D>: MUSCL-HLLE fluxes of direction D, marching.            */
+/* Face box: interior box with hi[D] + 1; face f sits between cells   */
+/* f - e_D (left) and f (right).  Each thread walks kSeg consecutive  */
+/* faces along D, carrying the right cell's primitives, the next      */
+/* face's reconstructed left state and the right cell's metric, so a  */
+/* cell's slope and metric are computed once per walk instead of once */
+/* per adjacent face.  Lanes of a wavefront are consecutive along the */
+/* fastest other axis (i for D = 1, 2; j for D = 0).                  */
 /* ------------------------------------------------------------------ */
+constexpr int kSeg = 8;
+
+__device__ __forceinline__ void slope_step(const Prim &l, const Prim &c,
+                                           const Prim &r, double theta,
+                                           Prim &minus_half, Prim &plus_half)
+{
+    /* add_scaled(c, slope, -0.5) and add_scaled(c, slope, +0.5)
+     * (Reconstruction.hpp:59-69) of the MC-theta slope of c */
+    double s = limited_slope(l.rho, c.rho, r.rho, theta);
+    minus_half.rho = c.rho + -0.5 * s;
+    plus_half.rho = c.rho + 0.5 * s;
+    s = limited_slope(l.eps, c.eps, r.eps, theta);
+    minus_half.eps = c.eps + -0.5 * s;
+    plus_half.eps = c.eps + 0.5 * s;
+    s = limited_slope(l.pressure, c.pressure, r.pressure, theta);
+    minus_half.pressure = c.pressure + -0.5 * s;
+    plus_half.pressure = c.pressure + 0.5 * s;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+    {
+        s = limited_slope(l.v[m], c.v[m], r.v[m], theta);
+        minus_half.v[m] = c.v[m] + -0.5 * s;
+        plus_half.v[m] = c.v[m] + 0.5 * s;
+    }
+}
+
 template <int D>
 __global__ void __launch_bounds__(kBlock)
     k_faces(BoxDev b, Options o, Background bg, const double *W, double *F,
-            long long nfaces)
+            long long nfaces, int nseg)
 {
+    constexpr int A = (D == 0) ? 1 : 0; /* fastest other axis */
+    constexpr int B = (D == 2) ? 1 : 2;
+    const int na = b.in[A], nb = b.in[B];
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nfaces)
+    if (idx >= (long long)na * nb * nseg)
         return;
-    const int fn0 = b.in[0] + (D == 0), fn1 = b.in[1] + (D == 1);
-    const int i = b.lo[0] + (int)(idx % fn0);
-    const int j = b.lo[1] + (int)((idx / fn0) % fn1);
-    const int k = b.lo[2] + (int)(idx / ((long long)fn0 * fn1));
+    const int ia = (int)(idx % na);
+    const int ib = (int)((idx / na) % nb);
+    const int sg = (int)(idx / ((long long)na * nb));
+    const int fnd = b.in[D] + 1;
+    const int p0 = sg * kSeg;
+    const int p1 = min(p0 + kSeg, fnd);
     const long long n = b.gnpts;
     const long long stride =
         (D == 0) ? 1 : ((D == 1) ? (long long)b.gn[0]
                                  : (long long)b.gn[0] * b.gn[1]);
-    const long long oR = goff(b, i, j, k);
-    const long long oL = oR - stride;
+    int c[3];
+    c[A] = b.lo[A] + ia;
+    c[B] = b.lo[B] + ib;
+    c[D] = b.lo[D] + p0; /* right cell of the first face */
+    long long oR = goff(b, c[0], c[1], c[2]);
+    const int fn0 = b.in[0] + (D == 0), fn1 = b.in[1] + (D == 1);
+    const long long f_stride =
+        (D == 0) ? 1 : ((D == 1) ? (long long)fn0 : (long long)fn0 * fn1);
+    long long fo = ((long long)(c[2] - b.lo[2]) * fn1 + (c[1] - b.lo[1])) *
+                       fn0 + (c[0] - b.lo[0]);
 
-    Prim pL = load_prim(W, n, oL);
-    Prim pR = load_prim(W, n, oR);
-    Prim lp = pL, rp = pR;
+    Prim wL = load_prim(W, n, oR - stride);
+    Prim wR = load_prim(W, n, oR);
+    Prim lp = wL;
     if (o.use_reconstruction)
     {
-        const Prim pLL = load_prim(W, n, oL - stride);
-        const Prim pRR = load_prim(W, n, oR + stride);
-        const double th = o.theta;
-        lp.rho = pL.rho + 0.5 * limited_slope(pLL.rho, pL.rho, pR.rho, th);
-        lp.eps = pL.eps + 0.5 * limited_slope(pLL.eps, pL.eps, pR.eps, th);
-        lp.pressure = pL.pressure + 0.5 * limited_slope(pLL.pressure,
-                                                        pL.pressure,
-                                                        pR.pressure, th);
-        rp.rho = pR.rho + -0.5 * limited_slope(pL.rho, pR.rho, pRR.rho, th);
-        rp.eps = pR.eps + -0.5 * limited_slope(pL.eps, pR.eps, pRR.eps, th);
-        rp.pressure = pR.pressure + -0.5 * limited_slope(pL.pressure,
-                                                         pR.pressure,
-                                                         pRR.pressure, th);
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
+        const Prim wLL = load_prim(W, n, oR - 2 * stride);
+        Prim unused;
+        slope_step(wLL, wL, wR, o.theta, unused, lp);
+    }
+    double x[3];
+    c[D] -= 1;
+    coords(c[0], c[1], c[2], b.dx, b.center, x);
+    Geo gL;
+    cell_geometry(bg, x, gL);
+    for (int p = p0; p < p1; ++p)
+    {
+        c[D] += 1;
+        Prim rp = wR, lp_next = wR;
+        Prim wRR;
+        if (o.use_reconstruction)
         {
-            lp.v[m] = pL.v[m] + 0.5 * limited_slope(pLL.v[m], pL.v[m],
-                                                    pR.v[m], th);
-            rp.v[m] = pR.v[m] + -0.5 * limited_slope(pL.v[m], pR.v[m],
-                                                     pRR.v[m], th);
+            wRR = load_prim(W, n, oR + stride);
+            slope_step(wL, wR, wRR, o.theta, rp, lp_next);
+        }
+        coords(c[0], c[1], c[2], b.dx, b.center, x);
+        Geo gR, face;
+        cell_geometry(bg, x, gR);
+        average_geo(gL, gR, face);
+        Cons f;
+        hlle_face(lp, rp, o, face, D, f);
+        F[fo] = f.D;
+        F[nfaces + fo] = f.S[0];
+        F[2 * nfaces + fo] = f.S[1];
+        F[3 * nfaces + fo] = f.S[2];
+        F[4 * nfaces + fo] = f.tau;
+        /* advance one cell along D */
+        if (p + 1 < p1)
+        {
+            lp = lp_next;
+            wL = wR;
+            wR = o.use_reconstruction ? wRR : load_prim(W, n, oR + stride);
+            gL = gR;
+            oR += stride;
+            fo += f_stride;
         }
     }
+}
 
-    double xl[3], xr[3];
-    coords(i - (D == 0), j - (D == 1), k - (D == 2), b.dx, b.center, xl);
-    coords(i, j, k, b.dx, b.center, xr);
-    Geo gl, gr, face;
-    cell_geometry(bg, xl, gl);
-    cell_geometry(bg, xr, gr);
-    average_geo(gl, gr, face);
+/* ------------------------------------------------------------------ */
+/* k_faces_x: MUSCL-HLLE fluxes of direction 0, one lane per cell.    */
+/* The cells lo-1 .. hi+1 of every (j, k) row are laid end to end; a  */
+/* wavefront takes 64 consecutive ones, overlapping the next wave by  */
+/* one.  Each lane evaluates its cell's metric and the two half-cell  */
+/* reconstructed states once; the face between a lane and the next is */
+/* formed with __shfl_down, so loads are unit-stride along i and no   */
+/* cell metric is evaluated twice within a wave.                      */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ double shfl_next(double v)
+{
+    return __shfl_down(v, 1, 64);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_faces_x(BoxDev b, Options o, Background bg, const double *W, double *F,
+              long long nfaces, long long next)
+{
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long e = wave * 63 + lane;
+    const int E = b.in[0] + 2;
+    const bool live = e < next;
+    const long long ee = live ? e : next - 1;
+    const int x = (int)(ee % E);
+    const long long row = ee / E;
+    const int j = b.lo[1] + (int)(row % b.in[1]);
+    const int k = b.lo[2] + (int)(row / b.in[1]);
+    const int i = b.lo[0] - 1 + x;
+    const long long n = b.gnpts;
+    const long long oc = goff(b, i, j, k);
+    const Prim wc = load_prim(W, n, oc);
+    Prim mh = wc, ph = wc; /* c - 1/2 (right state of the face below),
+                              c + 1/2 (left state of the face above) */
+    if (o.use_reconstruction)
+    {
+        const Prim wl = load_prim(W, n, oc - 1);
+        const Prim wr = load_prim(W, n, oc + 1);
+        slope_step(wl, wc, wr, o.theta, mh, ph);
+    }
+    double xc[3];
+    coords(i, j, k, b.dx, b.center, xc);
+    Geo gc;
+    cell_geometry(bg, xc, gc);
+
+    /* neighbour (lane + 1): its minus-half state and metric */
+    Prim rp;
+    rp.rho = shfl_next(mh.rho);
+    rp.eps = shfl_next(mh.eps);
+    rp.pressure = shfl_next(mh.pressure);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        rp.v[m] = shfl_next(mh.v[m]);
+    Geo gn;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            gn.g[a][c] = shfl_next(gc.g[a][c]);
+            gn.gu[a][c] = shfl_next(gc.gu[a][c]);
+        }
+    gn.lapse = shfl_next(gc.lapse);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        gn.shift[m] = shfl_next(gc.shift[m]);
+
+    /* face between this cell and the next one of the same row */
+    if (!live || lane == 63 || x == E - 1 || e + 1 >= next)
+        return;
+    Geo face;
+    average_geo(gc, gn, face);
     Cons f;
-    hlle_face(lp, rp, o, face, D, f);
-    F[idx] = f.D;
-    F[nfaces + idx] = f.S[0];
-    F[2 * nfaces + idx] = f.S[1];
-    F[3 * nfaces + idx] = f.S[2];
-    F[4 * nfaces + idx] = f.tau;
+    hlle_face(ph, rp, o, face, 0, f);
+    const int fn0 = b.in[0] + 1;
+    const long long fo =
+        ((long long)(k - b.lo[2]) * b.in[1] + (j - b.lo[1])) * fn0 + x;
+    F[fo] = f.D;
+    F[nfaces + fo] = f.S[0];
+    F[2 * nfaces + fo] = f.S[1];
+    F[3 * nfaces + fo] = f.S[2];
+    F[4 * nfaces + fo] = f.tau;
 }
 
 enum CellMode
@@ -447,18 +629,21 @@ __global__ void __launch_bounds__(kBlock)
         bool have_geo = false;
         if (o.use_static_sources)
         {
-            FullGeo fg;
-            cell_full(bg_s, x, fg);
-            geo_s = fg.geo;
+            /* geometry first; the derivative half of the metric only for
+             * cells whose sources are not skipped (:284-288) */
+            GeoCore core;
+            cell_geometry_core(bg_s, x, geo_s, core);
             have_geo = true;
             Prim p = load_prim(a.W, n, so);
-            floors(p, o, fg.geo.g);
+            floors(p, o, geo_s.g);
             if (is_atmosphere(p, o, o.source_atmosphere_factor))
             {
                 mbits |= GRHD_MASK_SOURCE_ATMOSPHERE;
             }
             else
             {
+                FullGeo fg;
+                cell_derivs(bg_s, x, core, geo_s, fg);
                 Cons src;
                 static_sources(p, us, o.gamma, fg, src);
                 rhs.D += src.D;
@@ -1061,7 +1246,7 @@ void launch_prims(grhd_ctx *c, const BoxDev &b, const Options &o,
                   unsigned long long eval, hipStream_t s)
 {
     if (exterior)
-        hipLaunchKernelGGL(k_prims<true>, dim3(grid_of(b.gnpts)),
+        hipLaunchKernelGGL(k_prims<true>, dim3(grid_of(b.gnpts - b.inpts)),
                            dim3(kBlock), 0, s, b, o, bg, U, P, c->W, c->wbad,
                            c->dstat, eval);
     else
@@ -1071,19 +1256,37 @@ void launch_prims(grhd_ctx *c, const BoxDev &b, const Options &o,
     tick(c, GRHD_KERNEL_PRIMS, s);
 }
 
+int face_segments(const BoxDev &b, int d)
+{
+    return (b.in[d] + 1 + kSeg - 1) / kSeg;
+}
+
+long long face_threads(const BoxDev &b, int d)
+{
+    return b.inpts / b.in[d] * face_segments(b, d);
+}
+
 void launch_faces(grhd_ctx *c, const BoxDev &b, const Options &o,
                   const Background &bg, hipStream_t s, long long nf[3])
 {
     for (int d = 0; d < 3; ++d)
         nf[d] = b.inpts / b.in[d] * (b.in[d] + 1);
-    hipLaunchKernelGGL(k_faces<0>, dim3(grid_of(nf[0])), dim3(kBlock), 0, s,
-                       b, o, bg, c->W, c->F[0], nf[0]);
+    {
+        /* one lane per cell lo-1 .. hi+1 of each row, 63 new cells per
+         * 64-lane wave */
+        const long long next = (long long)(b.in[0] + 2) * b.in[1] * b.in[2];
+        const long long waves = (next - 1 + 62) / 63;
+        hipLaunchKernelGGL(k_faces_x, dim3(grid_of(waves * 64)), dim3(kBlock),
+                           0, s, b, o, bg, c->W, c->F[0], nf[0], next);
+    }
     tick(c, GRHD_KERNEL_FACES_X, s);
-    hipLaunchKernelGGL(k_faces<1>, dim3(grid_of(nf[1])), dim3(kBlock), 0, s,
-                       b, o, bg, c->W, c->F[1], nf[1]);
+    hipLaunchKernelGGL(k_faces<1>, dim3(grid_of(face_threads(b, 1))),
+                       dim3(kBlock), 0, s, b, o, bg, c->W, c->F[1], nf[1],
+                       face_segments(b, 1));
     tick(c, GRHD_KERNEL_FACES_Y, s);
-    hipLaunchKernelGGL(k_faces<2>, dim3(grid_of(nf[2])), dim3(kBlock), 0, s,
-                       b, o, bg, c->W, c->F[2], nf[2]);
+    hipLaunchKernelGGL(k_faces<2>, dim3(grid_of(face_threads(b, 2))),
+                       dim3(kBlock), 0, s, b, o, bg, c->W, c->F[2], nf[2],
+                       face_segments(b, 2));
     tick(c, GRHD_KERNEL_FACES_Z, s);
 }
 

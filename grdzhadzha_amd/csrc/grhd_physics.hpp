@@ -191,12 +191,12 @@ __device__ __forceinline__ void ks_core(double M, double a, const double x[3],
     c.el[2] = z / c.r;
 }
 
-/* gamma, gamma^-1, lapse, shift only (derivatives are not needed by the
- * flux/recovery paths and do not feed these values). */
-__device__ __forceinline__ void ks_geometry(double M, double a,
-                                            const double x[3], Geo &g)
+/* gamma, gamma^-1, lapse, shift (KerrSchild.hpp:79-116 with
+ * FixedBGMetric.hpp:23-24); `c` keeps r, H and l_i for ks_derivs. */
+__device__ __forceinline__ void ks_geometry_core(double M, double a,
+                                                 const double x[3], Geo &g,
+                                                 KSCore &c)
 {
-    KSCore c;
     ks_core(M, a, x, c);
     const double el_t = 1.0;
     g.lapse = 1.0 / sqrt(1.0 + 2.0 * c.H * el_t * el_t);
@@ -218,14 +218,25 @@ __device__ __forceinline__ void ks_geometry(double M, double a,
     }
 }
 
-/* Full ADM evaluation converted to geometry + source derivatives
- * (KerrSchild.hpp:65-242 + FixedBGMetric.hpp:17-55). */
-__device__ __forceinline__ void ks_full(double M, double a, const double xc[3],
-                                        FullGeo &f)
+/* gamma, gamma^-1, lapse, shift only (derivatives are not needed by the
+ * flux/recovery paths and do not feed these values). */
+__device__ __forceinline__ void ks_geometry(double M, double a,
+                                            const double x[3], Geo &g)
 {
-    const double a2 = a * a;
     KSCore c;
-    ks_core(M, a, xc, c);
+    ks_geometry_core(M, a, x, g, c);
+}
+
+/* The derivative half of the full ADM evaluation, given the geometry half
+ * (KerrSchild.hpp:118-242 + FixedBGMetric.hpp:32-55): dgamma, dalpha,
+ * dbeta, K_ij and d ln sqrt gamma. */
+__device__ __forceinline__ void ks_derivs(double M, double a,
+                                          const double xc[3],
+                                          const KSCore &c, const Geo &gin,
+                                          FullGeo &f)
+{
+    (void)M;
+    const double a2 = a * a;
     const double z = xc[2];
     const double el_t = 1.0;
     const double H = c.H;
@@ -267,22 +278,8 @@ __device__ __forceinline__ void ks_full(double M, double a, const double xc[3],
     }
     const double dltdx = 0.0;
 
-    Geo &g = f.geo;
-    g.lapse = 1.0 / sqrt(1.0 + 2.0 * H * el_t * el_t);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            g.g[i][j] = kdelta(i, j) + 2.0 * H * c.el[i] * c.el[j];
-    inverse_sym(g.g, g.gu);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-    {
-        g.shift[i] = 0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            g.shift[i] += g.gu[i][j] * 2.0 * H * c.el[j] * el_t;
-    }
+    f.geo = gin;
+    const Geo &g = f.geo;
 
     /* d1_gamma[i][j][k] stored as dg[k][i][j] */
 #pragma unroll
@@ -353,6 +350,17 @@ __device__ __forceinline__ void ks_full(double M, double a, const double xc[3],
     }
 }
 
+/* Full ADM evaluation converted to geometry + source derivatives
+ * (KerrSchild.hpp:65-242 + FixedBGMetric.hpp:17-55). */
+__device__ __forceinline__ void ks_full(double M, double a, const double xc[3],
+                                        FullGeo &f)
+{
+    KSCore c;
+    Geo g;
+    ks_geometry_core(M, a, xc, g, c);
+    ks_derivs(M, a, xc, c, g, f);
+}
+
 /* ---------------- Circular binary PN (CircularBinaryPN.hpp) ---------------- */
 struct Potentials
 {
@@ -412,10 +420,12 @@ __device__ __forceinline__ void pn_potentials(const Background &bg,
     pn_add_point(p, bg.m2, bg.soft, bg.p2, bg.u2, x);
 }
 
-__device__ __forceinline__ void pn_geometry(const Background &bg,
-                                            const double x[3], Geo &g)
+/* Conformally flat geometry of the binary (CircularBinaryPN.hpp:80-130);
+ * `p` keeps the potentials for pn_derivs. */
+__device__ __forceinline__ void pn_geometry_pot(const Background &bg,
+                                                const double x[3], Geo &g,
+                                                Potentials &p)
 {
-    Potentials p;
     pn_potentials(bg, x, p);
     const double U = p.scalar;
     const double cf = 1.0 + 2.0 * U;
@@ -439,12 +449,20 @@ __device__ __forceinline__ void pn_geometry(const Background &bg,
     inverse_sym(g.g, g.gu);
 }
 
-__device__ __forceinline__ void pn_full(const Background &bg,
-                                        const double x[3], FullGeo &f)
+__device__ __forceinline__ void pn_geometry(const Background &bg,
+                                            const double x[3], Geo &g)
 {
     Potentials p;
-    pn_potentials(bg, x, p);
-    Geo &g = f.geo;
+    pn_geometry_pot(bg, x, g, p);
+}
+
+/* The derivative half of the binary's ADM evaluation given its geometry
+ * (CircularBinaryPN.hpp:132-188 + FixedBGMetric.hpp:32-55). */
+__device__ __forceinline__ void pn_derivs(const Potentials &p, const Geo &gin,
+                                          FullGeo &f)
+{
+    f.geo = gin;
+    const Geo &g = f.geo;
     const double U = p.scalar;
     const double dcf_dt = 2.0 * p.scalar_dt;
     const double cf = 1.0 + 2.0 * U;
@@ -460,21 +478,13 @@ __device__ __forceinline__ void pn_full(const Background &bg,
 #pragma unroll
         for (int j = 0; j < 3; ++j)
         {
-            g.g[i][j] = 0.0;
             f.dshift[i][j] = 0.0;
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 f.dg[k][i][j] = 0.0;
         }
     }
-    g.lapse = sqrt(lapse_squared);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-    {
-        g.g[i][i] = cf;
-        g.shift[i] = -4.0 * p.vec[i] / cf;
-    }
-    inverse_sym(g.g, g.gu);
+    (void)lapse_squared;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
@@ -534,6 +544,44 @@ __device__ __forceinline__ void pn_full(const Background &bg,
                 s += 0.5 * g.gu[i][j] * f.dg[d][i][j];
         f.dlogsqrtdet[d] = s;
     }
+}
+
+__device__ __forceinline__ void pn_full(const Background &bg,
+                                        const double x[3], FullGeo &f)
+{
+    Potentials p;
+    Geo g;
+    pn_geometry_pot(bg, x, g, p);
+    pn_derivs(p, g, f);
+}
+
+/* What the derivative half of either background needs from its geometry
+ * half. */
+struct GeoCore
+{
+    KSCore ks;
+    Potentials pn;
+};
+
+__device__ __forceinline__ void cell_geometry_core(const Background &bg,
+                                                   const double x[3], Geo &g,
+                                                   GeoCore &core)
+{
+    if (bg.kind == 1)
+        pn_geometry_pot(bg, x, g, core.pn);
+    else
+        ks_geometry_core(bg.M, bg.a, x, g, core.ks);
+}
+
+__device__ __forceinline__ void cell_derivs(const Background &bg,
+                                            const double x[3],
+                                            const GeoCore &core, const Geo &g,
+                                            FullGeo &f)
+{
+    if (bg.kind == 1)
+        pn_derivs(core.pn, g, f);
+    else
+        ks_derivs(bg.M, bg.a, x, core.ks, g, f);
 }
 
 __device__ __forceinline__ void cell_geometry(const Background &bg,
