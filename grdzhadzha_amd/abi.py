@@ -76,6 +76,10 @@ class GrhdDiagnostics(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class GrhdLevelBox(ctypes.Structure):
+    _fields_ = [("box", GrhdBox), ("state_offset", ctypes.c_longlong)]
+
+
 class GrhdError(ctypes.Structure):
     _fields_ = [("code", ctypes.c_int), ("iv", _I3),
                 ("residual", ctypes.c_double)]
@@ -116,6 +120,28 @@ SIGNATURES = {
                                   _vp]),
     "grhd_cell_geometry_box": (_i, [_vp, _P(GrhdBox), _P(GrhdBackground), _d,
                                     _vp, _vp]),
+    "grhd_level_compute_rhs": (_i, [_vp, _P(GrhdLevelBox), _i,
+                                    _P(GrhdParams), _P(GrhdBackground), _d,
+                                    _vp, _i, _i, _vp, _vp, _P(GrhdError),
+                                    _vp]),
+    "grhd_level_recover_primitives": (_i, [_vp, _P(GrhdLevelBox), _i,
+                                           _P(GrhdParams), _P(GrhdBackground),
+                                           _d, _vp, _i, _i, _vp,
+                                           _P(GrhdDiagnostics), _vp]),
+    "grhd_level_compute_max_inverse_dt": (_i, [_vp, _P(GrhdLevelBox), _i,
+                                               _P(GrhdParams),
+                                               _P(GrhdBackground), _d, _vp,
+                                               _i, _i, _P(_d), _P(GrhdError),
+                                               _vp]),
+    "grhd_level_advance_ssprk2": (_i, [_vp, _P(GrhdLevelBox), _i,
+                                       _P(GrhdParams), _P(GrhdBackground), _d,
+                                       _d, _vp, _i, _i, _vp,
+                                       _P(GrhdDiagnostics), _P(GrhdError),
+                                       _vp]),
+    "grhd_level_advance_rk4": (_i, [_vp, _P(GrhdLevelBox), _i,
+                                    _P(GrhdParams), _P(GrhdBackground), _d,
+                                    _d, _i, _vp, _i, _i, _vp,
+                                    _P(GrhdDiagnostics), _P(GrhdError), _vp]),
     "grhd_ctx_set_timing": (_i, [_vp, _i]),
     "grhd_ctx_get_timing": (_i, [_vp, _P(_d), _P(ctypes.c_longlong)]),
 }

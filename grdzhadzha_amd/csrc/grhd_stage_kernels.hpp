@@ -1,0 +1,1157 @@
+/*
+ * grhd_stage_kernels.hpp -- the CDNA4 kernels of one right-hand side /
+ * stage over every box of a level.
+ *
+ * A launch covers all boxes of a level.  The host writes a device box
+ * table (BoxDev: geometry of each box and its offsets into the shared
+ * pools) and, per kind of work item, a block table: block b processes
+ * items first .. first+255 of box `box` (blocks never straddle boxes), so
+ * one launch per kernel serves 1 or 540 boxes alike.  Box parameters are
+ * block-uniform and load into scalar registers.
+ *
+ * Work decomposition of one right-hand side (the reference's three
+ * directional sweeps, FixedBGFArrayBoxFiniteVolume.hpp:199-363, with every
+ * duplicated quantity computed once):
+ *   k_exchange   internal ghosts copied from the owning box (LevelData
+ *                exchange(), :310, :355)
+ *   k_prims      outflow ghost fill (:113-177) and con2prim over the
+ *                ghosted box (:33-58) -> primitives W (6 x npts) + a
+ *                failure byte per cell
+ *   k_faces_x,   MUSCL slopes (FArrayBoxFiniteVolume.hpp:573-589), face
+ *   k_faces<1,2> metric = mean of the two cell metrics (:179-197), floors,
+ *                prim2con, HLLE -> face fluxes F_d (5 x nfaces), each
+ *                face once
+ *   k_cells<M>   flux divergence in the reference's summation order,
+ *                static-metric sources (:255-298), volume-time sources
+ *                (:616-657), then per mode: write rhs, or an RK4 / SSPRK2
+ *                stage update fused with the post-update recovery (:60-110)
+ *                that also produces W of the next stage
+ *
+ * Memory follows the FArrayBox layout (component-major, i fastest), so a
+ * wavefront's loads of one component are unit-stride along i.  Reductions
+ * (reset counts, max 1/dt) are wavefront-64 shuffles, one LDS step across
+ * the block's waves and one atomic per block.  No MFMA: nothing here is a
+ * dense contraction.  Compiled with -ffp-contract=off.
+ */
+#ifndef GRHD_STAGE_KERNELS_HPP_
+#define GRHD_STAGE_KERNELS_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/grhd_abi.h"
+#include "grhd_physics.hpp"
+
+namespace grhd
+{
+
+constexpr int kBlock = 256;
+constexpr unsigned long long kNoFail = ~0ull;
+constexpr int kKeyBits = 43; /* 2 x (cells of a level) < 2^43 */
+constexpr int kSeg = 8;      /* faces per thread of the marching kernels */
+
+/* Device-side status block of one call.  The failure key, which every
+ * wavefront of the stage kernels reads, has a 128-byte line of its own,
+ * apart from the counters that every block updates with atomics. */
+struct DevStatus
+{
+    /* first RHS recovery failure: (evaluation << 43) | position, where a
+     * box owns positions [2 off_g, 2 (off_g + gnpts)): its ghost-fill
+     * failures first, then its recovery failures, each in BoxIterator
+     * order -- the cell the reference would throw at first */
+    alignas(128) unsigned long long fail_key;
+    alignas(128) unsigned long long cfl_key; /* off_i + interior index */
+    unsigned long long counts[4];
+    unsigned long long max_bits;
+};
+
+/* One box of a level as the kernels see it. */
+struct BoxDev
+{
+    int lo[3], hi[3];
+    int glo[3], ghi[3];
+    int dlo[3], dhi[3];
+    int periodic[3];
+    int gn[3], in[3];
+    long long gnpts, inpts;
+    double dx;
+    double center[3];
+    long long off_u;    /* this box's FAB in the state pool (doubles) */
+    long long off_g;    /* storage cells of the boxes before it */
+    long long off_i;    /* interior cells of the boxes before it */
+    long long off_f[3]; /* faces of direction d of the boxes before it */
+    long long nface[3]; /* faces of direction d of this box */
+    double inv_dx;      /* 1 / dx, rounded once on the host */
+};
+
+/* Kinds of work item, one block table each. */
+enum ItemClass
+{
+    C_ALL = 0, /* storage cells                          */
+    C_EXT = 1, /* storage cells outside the interior     */
+    C_INT = 2, /* interior cells                         */
+    C_FX = 3,  /* x-face lanes: 64 per 63 cells of a row */
+    C_FY = 4,  /* marching y-face walkers                */
+    C_FZ = 5,  /* marching z-face walkers                */
+    NCLASS = 6
+};
+
+__host__ __device__ inline int face_segments(const BoxDev &b, int d)
+{
+    return (b.in[d] + 1 + kSeg - 1) / kSeg;
+}
+
+__host__ __device__ inline long long fx_cells(const BoxDev &b)
+{
+    return (long long)(b.in[0] + 2) * b.in[1] * b.in[2];
+}
+
+__host__ __device__ inline long long class_items(const BoxDev &b, int c)
+{
+    switch (c)
+    {
+    case C_ALL:
+        return b.gnpts;
+    case C_EXT:
+        return b.gnpts - b.inpts;
+    case C_INT:
+        return b.inpts;
+    case C_FX:
+        return (fx_cells(b) - 1 + 62) / 63 * 64;
+    case C_FY:
+        return (long long)b.in[0] * b.in[2] * face_segments(b, 1);
+    default:
+        return (long long)b.in[0] * b.in[1] * face_segments(b, 2);
+    }
+}
+
+__host__ __device__ inline long long faces(const BoxDev &b, int d)
+{
+    return b.inpts / b.in[d] * (b.in[d] + 1);
+}
+
+/* Pools shared by every box of the level. */
+enum PoolId
+{
+    POOL_STATE = 0,
+    POOL_A = 1,
+    POOL_B = 2
+};
+
+struct Pools
+{
+    double *state; /* caller's FABs (ncomp components each) */
+    int cb;        /* comp_base */
+    double *UA, *UB;
+    double *acc;
+    double *W;
+    unsigned char *wbad;
+    double *F[3];
+    double *rhs; /* rhs FABs: rhs_ncomp x inpts per box */
+    int rhs_ncomp;
+    unsigned char *mask;
+    DevStatus *st;
+};
+
+__device__ __forceinline__ double *pool_cons(const Pools &p, int which,
+                                             const BoxDev &b)
+{
+    if (which == POOL_STATE)
+        return p.state + b.off_u + (long long)p.cb * b.gnpts;
+    return (which == POOL_A ? p.UA : p.UB) + 5 * b.off_g;
+}
+
+__device__ __forceinline__ long long goff(const BoxDev &b, int i, int j, int k)
+{
+    return ((long long)(k - b.glo[2]) * b.gn[1] + (j - b.glo[1])) * b.gn[0] +
+           (i - b.glo[0]);
+}
+
+__device__ __forceinline__ Cons load_cons(const double *U, long long n,
+                                          long long o)
+{
+    Cons c;
+    c.D = U[o];
+    c.S[0] = U[n + o];
+    c.S[1] = U[2 * n + o];
+    c.S[2] = U[3 * n + o];
+    c.tau = U[4 * n + o];
+    return c;
+}
+
+__device__ __forceinline__ void store_cons(double *U, long long n, long long o,
+                                           const Cons &c)
+{
+    U[o] = c.D;
+    U[n + o] = c.S[0];
+    U[2 * n + o] = c.S[1];
+    U[3 * n + o] = c.S[2];
+    U[4 * n + o] = c.tau;
+}
+
+/* primitives: rho, eps, pressure, v1, v2, v3 */
+__device__ __forceinline__ Prim load_prim(const double *P, long long n,
+                                          long long o)
+{
+    Prim p;
+    p.rho = P[o];
+    p.eps = P[n + o];
+    p.pressure = P[2 * n + o];
+    p.v[0] = P[3 * n + o];
+    p.v[1] = P[4 * n + o];
+    p.v[2] = P[5 * n + o];
+    return p;
+}
+
+__device__ __forceinline__ void store_prim(double *P, long long n, long long o,
+                                           const Prim &p)
+{
+    P[o] = p.rho;
+    P[n + o] = p.eps;
+    P[2 * n + o] = p.pressure;
+    P[3 * n + o] = p.v[0];
+    P[4 * n + o] = p.v[1];
+    P[5 * n + o] = p.v[2];
+}
+
+__device__ __forceinline__ unsigned long long
+fail_key(unsigned long long eval, int phase, const BoxDev &b, long long idx)
+{
+    return (eval << kKeyBits) |
+           (unsigned long long)(2 * b.off_g + (phase ? b.gnpts : 0) + idx);
+}
+
+/* The block's box and the item this thread processes (-1 past the end).
+ * The box is bound by reference, not copied: its fields are block-uniform
+ * scalar loads the compiler may issue where they are used, which keeps the
+ * scalar register file from spilling in the large kernels. */
+__device__ __forceinline__ long long block_item(const int2 *__restrict__ items,
+                                                const BoxDev *__restrict__ boxes,
+                                                int cls, const BoxDev *&b)
+{
+    const int2 bt = items[blockIdx.x];
+    b = boxes + bt.x;
+    const long long it = (long long)bt.y + threadIdx.x;
+    return it < class_items(*b, cls) ? it : -1;
+}
+
+/* True once a recovery failure of an evaluation before `eval` is
+ * recorded.  A kernel writing primitives of evaluation `eval` then drops
+ * its stores: the call reports the first failure, and the scratch slot
+ * holding that failure's residual must survive later evaluations.  Failures
+ * of `eval` itself do not stop it, since a ghost-fill failure of an
+ * evaluation precedes its interior failures in the reference's order.  The
+ * word is read once per wavefront at kernel start and consulted only at the
+ * stores, so its latency hides behind the arithmetic; every lane of the
+ * wavefront sees the same value. */
+__device__ __forceinline__ bool failed_before(const DevStatus *st,
+                                              unsigned long long eval)
+{
+    return __hip_atomic_load(&st->fail_key, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT) <
+           (eval << kKeyBits);
+}
+
+/* wavefront-64 + block reductions */
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+__device__ inline void block_add_counts(DevStatus *st, unsigned long long c0,
+                                        unsigned long long c1,
+                                        unsigned long long c2,
+                                        unsigned long long c3)
+{
+    __shared__ unsigned long long part[4][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    c0 = wave_sum(c0);
+    c1 = wave_sum(c1);
+    c2 = wave_sum(c2);
+    c3 = wave_sum(c3);
+    if (lane == 0)
+    {
+        part[0][wave] = c0;
+        part[1][wave] = c1;
+        part[2][wave] = c2;
+        part[3][wave] = c3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4)
+    {
+        unsigned long long s = 0;
+        for (int w = 0; w < kBlock / 64; ++w)
+            s += part[threadIdx.x][w];
+        if (s)
+            atomicAdd(&st->counts[threadIdx.x], s);
+    }
+}
+
+/* The e-th storage cell outside the interior box: whole z-planes below
+ * and above the interior, then per interior plane the rows below and
+ * above the interior in y, then the x-ghost cells of interior rows. */
+__device__ __forceinline__ void exterior_cell(const BoxDev &b, long long e,
+                                              int &i, int &j, int &k)
+{
+    const long long plane = (long long)b.gn[0] * b.gn[1];
+    const long long zlo = (long long)(b.lo[2] - b.glo[2]) * plane;
+    const long long zhi = (long long)(b.ghi[2] - b.hi[2]) * plane;
+    if (e < zlo + zhi)
+    {
+        const long long q = (e < zlo) ? e : e - zlo;
+        k = ((e < zlo) ? b.glo[2] : b.hi[2] + 1) + (int)(q / plane);
+        const long long r = q % plane;
+        j = b.glo[1] + (int)(r / b.gn[0]);
+        i = b.glo[0] + (int)(r % b.gn[0]);
+        return;
+    }
+    e -= zlo + zhi;
+    const long long per_plane = plane - (long long)b.in[0] * b.in[1];
+    k = b.lo[2] + (int)(e / per_plane);
+    long long r = e % per_plane;
+    const long long ylo = (long long)(b.lo[1] - b.glo[1]) * b.gn[0];
+    const long long yhi = (long long)(b.ghi[1] - b.hi[1]) * b.gn[0];
+    if (r < ylo + yhi)
+    {
+        const long long q = (r < ylo) ? r : r - ylo;
+        j = ((r < ylo) ? b.glo[1] : b.hi[1] + 1) + (int)(q / b.gn[0]);
+        i = b.glo[0] + (int)(q % b.gn[0]);
+        return;
+    }
+    r -= ylo + yhi;
+    const int gx = b.lo[0] - b.glo[0];
+    const int per_row = b.gn[0] - b.in[0];
+    j = b.lo[1] + (int)(r / per_row);
+    const int q = (int)(r % per_row);
+    i = (q < gx) ? b.glo[0] + q : b.hi[0] + 1 + (q - gx);
+}
+
+__global__ void k_status_init(DevStatus *st)
+{
+    if (threadIdx.x == 0)
+    {
+        st->fail_key = kNoFail;
+        st->cfl_key = kNoFail;
+        st->counts[0] = st->counts[1] = st->counts[2] = st->counts[3] = 0;
+        st->max_bits = 0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* k_exchange: internal ghost cells of every box copied from the box  */
+/* whose interior owns them (host-built list: dst box, dst index, src */
+/* box, src index), `ncopy` components of the pool.                   */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(kBlock)
+    k_exchange(const BoxDev *__restrict__ boxes, const int4 *list, long long n, Pools p,
+               int which, int ncopy)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int4 e = list[t];
+    const BoxDev &bd = boxes[e.x];
+    const BoxDev &bs = boxes[e.z];
+    double *dst = (which == POOL_STATE)
+                      ? p.state + bd.off_u
+                      : pool_cons(p, which, bd);
+    const double *src = (which == POOL_STATE)
+                            ? p.state + bs.off_u
+                            : pool_cons(p, which, bs);
+    for (int c = 0; c < ncopy; ++c)
+        dst[c * bd.gnpts + e.y] = src[c * bs.gnpts + e.w];
+}
+
+/* ------------------------------------------------------------------ */
+/* k_prims: outflow ghost fill + con2prim over the storage box.       */
+/* EXTERIOR: interior W already holds con2prim of the interior state  */
+/* (written by the previous k_cells); only exterior cells are done.   */
+/* ------------------------------------------------------------------ */
+template <bool EXTERIOR>
+__global__ void __launch_bounds__(kBlock)
+    k_prims(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o, Background bg,
+            Pools p, int which, int store_prims, unsigned long long eval)
+{
+    const BoxDev *bp;
+    const long long it =
+        block_item(items, boxes, EXTERIOR ? C_EXT : C_ALL, bp);
+    const BoxDev &b = *bp;
+    if (it < 0)
+        return;
+    const bool dead = failed_before(p.st, eval);
+    int i, j, k;
+    if (EXTERIOR)
+    {
+        exterior_cell(b, it, i, j, k);
+    }
+    else
+    {
+        i = b.glo[0] + (int)(it % b.gn[0]);
+        j = b.glo[1] + (int)((it / b.gn[0]) % b.gn[1]);
+        k = b.glo[2] + (int)(it / ((long long)b.gn[0] * b.gn[1]));
+    }
+    const long long idx = goff(b, i, j, k);
+    const int iv[3] = {i, j, k};
+    int src[3] = {i, j, k};
+    bool physical = false;
+    bool src_interior = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        if (iv[d] < b.dlo[d])
+        {
+            if (!b.periodic[d])
+            {
+                physical = true;
+                src[d] = b.lo[d];
+            }
+        }
+        else if (iv[d] > b.dhi[d])
+        {
+            if (!b.periodic[d])
+            {
+                physical = true;
+                src[d] = b.hi[d];
+            }
+        }
+        if (src[d] < b.lo[d] || src[d] > b.hi[d])
+            src_interior = false;
+    }
+
+    const long long n = b.gnpts;
+    double *U = pool_cons(p, which, b);
+    double *W = p.W + 6 * b.off_g;
+    unsigned char *wbad = p.wbad + b.off_g;
+    double x[3];
+    coords(i, j, k, b.dx, b.center, x);
+    Geo gt;
+    cell_geometry(bg, x, gt);
+    Cons c;
+    if (physical)
+    {
+        const long long so = goff(b, src[0], src[1], src[2]);
+        Prim pr;
+        bool src_ok;
+        double res = 0.0;
+        if (EXTERIOR && src_interior)
+        {
+            /* con2prim of the source was done by the previous stage */
+            src_ok = wbad[so] == 0;
+            if (src_ok)
+                pr = load_prim(W, n, so);
+            else
+                res = W[so];
+        }
+        else
+        {
+            double xs[3];
+            coords(src[0], src[1], src[2], b.dx, b.center, xs);
+            Geo gs;
+            cell_geometry(bg, xs, gs);
+            const Cons cs = load_cons(U, n, so);
+            src_ok = con2prim(cs, o, gs.gu, pr, res);
+        }
+        if (!src_ok)
+        {
+            if (!dead)
+            {
+                atomicMin(&p.st->fail_key, fail_key(eval, 0, b, idx));
+                W[idx] = res;
+                wbad[idx] = 1;
+            }
+            return;
+        }
+        floors(pr, o, gt.g);
+        prim2con(pr, o.gamma, gt.g, c);
+        if (!dead)
+        {
+            if (store_prims)
+                store_prim(U + 5 * n, n, idx, pr);
+            store_cons(U, n, idx, c);
+        }
+    }
+    else
+    {
+        c = load_cons(U, n, idx);
+    }
+    Prim w;
+    double res;
+    const bool ok = con2prim(c, o, gt.gu, w, res);
+    if (dead)
+        return;
+    if (!ok)
+    {
+        atomicMin(&p.st->fail_key, fail_key(eval, 1, b, idx));
+        W[idx] = res;
+        wbad[idx] = 1;
+        return;
+    }
+    store_prim(W, n, idx, w);
+    wbad[idx] = 0;
+}
+
+/* add_scaled(c, slope, -1/2) and add_scaled(c, slope, +1/2)
+ * (Reconstruction.hpp:59-69) of the MC-theta slope of c */
+__device__ __forceinline__ void slope_step(const Prim &l, const Prim &c,
+                                           const Prim &r, double theta,
+                                           Prim &minus_half, Prim &plus_half)
+{
+    double s = limited_slope(l.rho, c.rho, r.rho, theta);
+    minus_half.rho = c.rho + -0.5 * s;
+    plus_half.rho = c.rho + 0.5 * s;
+    s = limited_slope(l.eps, c.eps, r.eps, theta);
+    minus_half.eps = c.eps + -0.5 * s;
+    plus_half.eps = c.eps + 0.5 * s;
+    s = limited_slope(l.pressure, c.pressure, r.pressure, theta);
+    minus_half.pressure = c.pressure + -0.5 * s;
+    plus_half.pressure = c.pressure + 0.5 * s;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+    {
+        s = limited_slope(l.v[m], c.v[m], r.v[m], theta);
+        minus_half.v[m] = c.v[m] + -0.5 * s;
+        plus_half.v[m] = c.v[m] + 0.5 * s;
+    }
+}
+
+__device__ __forceinline__ void store_flux(double *F, long long nf,
+                                           long long fo, const Cons &f)
+{
+    F[fo] = f.D;
+    F[nf + fo] = f.S[0];
+    F[2 * nf + fo] = f.S[1];
+    F[3 * nf + fo] = f.S[2];
+    F[4 * nf + fo] = f.tau;
+}
+
+/* ------------------------------------------------------------------ */
+/* k_faces_x: fluxes of direction 0, one lane per cell.  The cells    */
+/* lo-1 .. hi+1 of every (j, k) row are laid end to end; a wavefront  */
+/* takes 64 consecutive ones, overlapping the next wave by one.  Each */
+/* lane evaluates its cell's metric and two half-cell states once;    */
+/* the face between a lane and the next is formed with __shfl_down,   */
+/* so loads are unit-stride along i and no metric is evaluated twice. */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ double shfl_next(double v)
+{
+    return __shfl_down(v, 1, 64);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_faces_x(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
+              Background bg, Pools p)
+{
+    const int2 bt = items[blockIdx.x];
+    const BoxDev &b = boxes[bt.x];
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)bt.y + threadIdx.x) >> 6;
+    const long long e = wave * 63 + lane;
+    const long long next = fx_cells(b);
+    const int E = b.in[0] + 2;
+    const bool live = e < next;
+    const long long ee = live ? e : next - 1;
+    const int x = (int)(ee % E);
+    const long long row = ee / E;
+    const int j = b.lo[1] + (int)(row % b.in[1]);
+    const int k = b.lo[2] + (int)(row / b.in[1]);
+    const int i = b.lo[0] - 1 + x;
+    const long long n = b.gnpts;
+    const double *W = p.W + 6 * b.off_g;
+    const long long oc = goff(b, i, j, k);
+    const Prim wc = load_prim(W, n, oc);
+    Prim mh = wc, ph = wc;
+    if (o.use_reconstruction)
+    {
+        const Prim wl = load_prim(W, n, oc - 1);
+        const Prim wr = load_prim(W, n, oc + 1);
+        slope_step(wl, wc, wr, o.theta, mh, ph);
+    }
+    double xc[3];
+    coords(i, j, k, b.dx, b.center, xc);
+    Geo gc;
+    cell_geometry(bg, xc, gc);
+
+    Prim rp;
+    rp.rho = shfl_next(mh.rho);
+    rp.eps = shfl_next(mh.eps);
+    rp.pressure = shfl_next(mh.pressure);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        rp.v[m] = shfl_next(mh.v[m]);
+    Geo gn;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            gn.g[a][c] = shfl_next(gc.g[a][c]);
+            gn.gu[a][c] = shfl_next(gc.gu[a][c]);
+        }
+    gn.lapse = shfl_next(gc.lapse);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        gn.shift[m] = shfl_next(gc.shift[m]);
+
+    if (!live || lane == 63 || x == E - 1 || e + 1 >= next)
+        return;
+    Geo face;
+    average_geo(gc, gn, face);
+    Cons f;
+    hlle_face(ph, rp, o, face, 0, f);
+    const long long nf = b.nface[0];
+    const long long fo =
+        ((long long)(k - b.lo[2]) * b.in[1] + (j - b.lo[1])) * (b.in[0] + 1) +
+        x;
+    store_flux(p.F[0] + 5 * b.off_f[0], nf, fo, f);
+}
+
+/* ------------------------------------------------------------------ */
+/* k_faces<D> (D = 1, 2): each thread walks kSeg consecutive faces     */
+/* along D carrying the right cell's primitives, the next face's left  */
+/* state and the right cell's metric, so a cell's slope and metric are */
+/* computed once per walk; lanes are consecutive along i.              */
+/* ------------------------------------------------------------------ */
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+    k_faces(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o, Background bg,
+            Pools p)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, D == 1 ? C_FY : C_FZ, bp);
+    const BoxDev &b = *bp;
+    if (it < 0)
+        return;
+    constexpr int B = (D == 2) ? 1 : 2; /* the other slow axis */
+    const int na = b.in[0], nb = b.in[B];
+    const int ia = (int)(it % na);
+    const int ib = (int)((it / na) % nb);
+    const int sg = (int)(it / ((long long)na * nb));
+    const int fnd = b.in[D] + 1;
+    const int p0 = sg * kSeg;
+    const int p1 = min(p0 + kSeg, fnd);
+    const long long n = b.gnpts;
+    const double *W = p.W + 6 * b.off_g;
+    const long long stride =
+        (D == 1) ? (long long)b.gn[0] : (long long)b.gn[0] * b.gn[1];
+    int c[3];
+    c[0] = b.lo[0] + ia;
+    c[B] = b.lo[B] + ib;
+    c[D] = b.lo[D] + p0; /* right cell of the first face */
+    long long oR = goff(b, c[0], c[1], c[2]);
+    const int fn0 = b.in[0], fn1 = b.in[1] + (D == 1);
+    const long long f_stride =
+        (D == 1) ? (long long)fn0 : (long long)fn0 * fn1;
+    long long fo = ((long long)(c[2] - b.lo[2]) * fn1 + (c[1] - b.lo[1])) *
+                       fn0 + (c[0] - b.lo[0]);
+    const long long nf = b.nface[D];
+    double *F = p.F[D] + 5 * b.off_f[D];
+
+    Prim wL = load_prim(W, n, oR - stride);
+    Prim wR = load_prim(W, n, oR);
+    Prim lp = wL;
+    if (o.use_reconstruction)
+    {
+        const Prim wLL = load_prim(W, n, oR - 2 * stride);
+        Prim unused;
+        slope_step(wLL, wL, wR, o.theta, unused, lp);
+    }
+    double x[3];
+    c[D] -= 1;
+    coords(c[0], c[1], c[2], b.dx, b.center, x);
+    Geo gL;
+    cell_geometry(bg, x, gL);
+    for (int q = p0; q < p1; ++q)
+    {
+        c[D] += 1;
+        Prim rp = wR, lp_next = wR;
+        Prim wRR;
+        if (o.use_reconstruction)
+        {
+            wRR = load_prim(W, n, oR + stride);
+            slope_step(wL, wR, wRR, o.theta, rp, lp_next);
+        }
+        coords(c[0], c[1], c[2], b.dx, b.center, x);
+        Geo gR, face;
+        cell_geometry(bg, x, gR);
+        average_geo(gL, gR, face);
+        Cons f;
+        hlle_face(lp, rp, o, face, D, f);
+        store_flux(F, nf, fo, f);
+        if (q + 1 < p1)
+        {
+            lp = lp_next;
+            wL = wR;
+            wR = o.use_reconstruction ? wRR : load_prim(W, n, oR + stride);
+            gL = gR;
+            oR += stride;
+            fo += f_stride;
+        }
+    }
+}
+
+enum CellMode
+{
+    MODE_RHS = 0,
+    MODE_RK4_1 = 1,
+    MODE_RK4_2 = 2,
+    MODE_RK4_3 = 3,
+    MODE_RK4_4 = 4,
+    MODE_SSP_1 = 5,
+    MODE_SSP_2 = 6
+};
+
+struct CellArgs
+{
+    int us;          /* pool whose RHS is evaluated */
+    int u0;          /* RK: step start / SSP: old state */
+    int unext;       /* pool receiving the stage result */
+    int store_prims; /* write floored primitives into the state FAB */
+    int write_mask;
+    int count;       /* add recovery counts */
+    double c_stage;  /* U0 coefficient of k */
+    double c_acc;    /* accumulator coefficient of k */
+    double dt;
+    double vol_h;    /* max(step, 1e-12) */
+    unsigned long long next_eval;
+};
+
+/* ------------------------------------------------------------------ */
+/* k_cells<MODE>: one thread per interior cell.                        */
+/* ------------------------------------------------------------------ */
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_cells(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
+            Background bg_s, Background bg_next, Background bg_plus,
+            Background bg_minus, Pools p, CellArgs a)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_INT, bp);
+    const BoxDev &b = *bp;
+    const bool dead = failed_before(p.st, a.next_eval);
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    if (it >= 0)
+    {
+        const int ii = (int)(it % b.in[0]);
+        const int jj = (int)((it / b.in[0]) % b.in[1]);
+        const int kk = (int)(it / ((long long)b.in[0] * b.in[1]));
+        const int i = b.lo[0] + ii, j = b.lo[1] + jj, k = b.lo[2] + kk;
+        const long long n = b.gnpts;
+        const long long so = goff(b, i, j, k);
+        double *W = p.W + 6 * b.off_g;
+
+        /* flux divergence: rhs = ((0 + r_x) + r_y) + r_z */
+        Cons rhs;
+        rhs.D = 0.0;
+        rhs.tau = 0.0;
+        rhs.S[0] = rhs.S[1] = rhs.S[2] = 0.0;
+        const double mi = -b.inv_dx;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            const double *F = p.F[d] + 5 * b.off_f[d];
+            const long long nf = b.nface[d];
+            const int fn0 = b.in[0] + (d == 0), fn1 = b.in[1] + (d == 1);
+            const long long flo = ((long long)kk * fn1 + jj) * fn0 + ii;
+            const long long fhi =
+                flo + ((d == 0) ? 1
+                                : ((d == 1) ? (long long)fn0
+                                            : (long long)fn0 * fn1));
+            rhs.D += mi * (F[fhi] - F[flo]);
+            rhs.tau += mi * (F[4 * nf + fhi] - F[4 * nf + flo]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                rhs.S[m] += mi * (F[(1 + m) * nf + fhi] - F[(1 + m) * nf + flo]);
+        }
+
+        const Cons us = load_cons(pool_cons(p, a.us, b), n, so);
+        double x[3];
+        coords(i, j, k, b.dx, b.center, x);
+        unsigned char mbits = 0;
+        Geo geo_s;
+        bool have_geo = false;
+        if (o.use_static_sources)
+        {
+            /* geometry first; the derivative half of the metric only for
+             * cells whose sources are not skipped (:284-288) */
+            GeoCore core;
+            cell_geometry_core(bg_s, x, geo_s, core);
+            have_geo = true;
+            Prim pw = load_prim(W, n, so);
+            floors(pw, o, geo_s.g);
+            if (is_atmosphere(pw, o, o.source_atmosphere_factor))
+            {
+                mbits |= GRHD_MASK_SOURCE_ATMOSPHERE;
+            }
+            else
+            {
+                FullGeo fg;
+                cell_derivs(bg_s, x, core, geo_s, fg);
+                Cons src;
+                static_sources(pw, us, o.gamma, fg, src);
+                rhs.D += src.D;
+                rhs.tau += src.tau;
+                rhs.S[0] += src.S[0];
+                rhs.S[1] += src.S[1];
+                rhs.S[2] += src.S[2];
+            }
+        }
+        if (o.use_volume_sources)
+        {
+            Geo gp, gm;
+            cell_geometry(bg_plus, x, gp);
+            cell_geometry(bg_minus, x, gm);
+            const double lp = log_sqrt_det(gp), lm = log_sqrt_det(gm);
+            const double dlog = (lp - lm) / (2.0 * a.vol_h);
+            rhs.D += 0.0 - us.D * dlog;
+            rhs.tau += 0.0 - us.tau * dlog;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                rhs.S[m] += 0.0 - us.S[m] * dlog;
+        }
+
+        if (MODE == MODE_RHS)
+        {
+            if (!dead)
+            {
+                const long long ni = b.inpts;
+                double *r =
+                    p.rhs + p.rhs_ncomp * b.off_i + (long long)p.cb * ni;
+                r[it] = rhs.D;
+                r[ni + it] = rhs.S[0];
+                r[2 * ni + it] = rhs.S[1];
+                r[3 * ni + it] = rhs.S[2];
+                r[4 * ni + it] = rhs.tau;
+                if (a.write_mask)
+                    p.mask[b.off_i + it] = mbits;
+            }
+        }
+        else
+        {
+            /* stage update */
+            Cons un;
+            const long long ni = b.inpts;
+            double *acc = p.acc + 5 * b.off_i;
+            if (MODE == MODE_RK4_1 || MODE == MODE_RK4_2 ||
+                MODE == MODE_RK4_3)
+            {
+                const Cons u0 = (MODE == MODE_RK4_1)
+                                    ? us
+                                    : load_cons(pool_cons(p, a.u0, b), n, so);
+                Cons ac = (MODE == MODE_RK4_1) ? u0 : load_cons(acc, ni, it);
+                ac.D += a.c_acc * rhs.D;
+                ac.tau += a.c_acc * rhs.tau;
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    ac.S[m] += a.c_acc * rhs.S[m];
+                store_cons(acc, ni, it, ac);
+                un.D = u0.D + a.c_stage * rhs.D;
+                un.tau = u0.tau + a.c_stage * rhs.tau;
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    un.S[m] = u0.S[m] + a.c_stage * rhs.S[m];
+            }
+            else if (MODE == MODE_RK4_4)
+            {
+                const Cons ac = load_cons(acc, ni, it);
+                un.D = ac.D + a.c_acc * rhs.D;
+                un.tau = ac.tau + a.c_acc * rhs.tau;
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    un.S[m] = ac.S[m] + a.c_acc * rhs.S[m];
+            }
+            else if (MODE == MODE_SSP_1)
+            {
+                un.D = us.D + a.dt * rhs.D;
+                un.tau = us.tau + a.dt * rhs.tau;
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    un.S[m] = us.S[m] + a.dt * rhs.S[m];
+            }
+            else /* MODE_SSP_2: 0.5 (old + stage + dt k) */
+            {
+                const Cons old = load_cons(pool_cons(p, a.u0, b), n, so);
+                un.D = 0.5 * (old.D + us.D + a.dt * rhs.D);
+                un.tau = 0.5 * (old.tau + us.tau + a.dt * rhs.tau);
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    un.S[m] = 0.5 * (old.S[m] + us.S[m] + a.dt * rhs.S[m]);
+            }
+
+            /* recovery with atmosphere (FixedBGFArrayBoxFiniteVolume.hpp:60-110) */
+            Geo g;
+            if (have_geo && bg_next.kind == 0)
+                g = geo_s;
+            else
+                cell_geometry(bg_next, x, g);
+            Prim rec;
+            double res;
+            const bool ok = con2prim(un, o, g.gu, rec, res);
+            bool reset = !ok;
+            Prim prim;
+            unsigned char m = 0;
+            cnt[0] = 1;
+            if (ok)
+            {
+                prim = rec;
+                floors(prim, o, g.g);
+                reset = prim_differs(rec, prim);
+                if (reset)
+                {
+                    cnt[2] = 1;
+                    m |= GRHD_MASK_FLOORED;
+                }
+            }
+            else
+            {
+                cnt[1] = 1;
+                m |= GRHD_MASK_FAILED;
+                atmosphere_prim(o, prim);
+            }
+            if (dead)
+            {
+                cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
+            }
+            else
+            {
+                double *Un = pool_cons(p, a.unext, b);
+                if (a.store_prims)
+                    store_prim(Un + 5 * n, n, so, prim);
+                Prim wn = rec;
+                bool wok = ok;
+                if (reset)
+                {
+                    cnt[3] = 1;
+                    m |= GRHD_MASK_RESET;
+                    prim2con(prim, o.gamma, g.g, un);
+                    wok = con2prim(un, o, g.gu, wn, res);
+                }
+                store_cons(Un, n, so, un);
+                unsigned char *wbad = p.wbad + b.off_g;
+                if (wok)
+                {
+                    store_prim(W, n, so, wn);
+                    wbad[so] = 0;
+                }
+                else
+                {
+                    atomicMin(&p.st->fail_key, fail_key(a.next_eval, 1, b, so));
+                    W[so] = res;
+                    wbad[so] = 1;
+                }
+                if (a.write_mask)
+                    p.mask[b.off_i + it] = m;
+            }
+        }
+    }
+    if (MODE != MODE_RHS && a.count)
+        block_add_counts(p.st, cnt[0], cnt[1], cnt[2], cnt[3]);
+}
+
+/* Post-update recovery of the interior of the state pool. */
+__global__ void __launch_bounds__(kBlock)
+    k_recover(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
+              Background bg, Pools p, int write_mask)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_INT, bp);
+    const BoxDev &b = *bp;
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    if (it >= 0)
+    {
+        const int i = b.lo[0] + (int)(it % b.in[0]);
+        const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+        const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+        const long long n = b.gnpts;
+        const long long so = goff(b, i, j, k);
+        double *U = pool_cons(p, POOL_STATE, b);
+        double x[3];
+        coords(i, j, k, b.dx, b.center, x);
+        Geo g;
+        cell_geometry(bg, x, g);
+        Cons c = load_cons(U, n, so);
+        Prim rec, prim;
+        double res;
+        const bool ok = con2prim(c, o, g.gu, rec, res);
+        bool reset = !ok;
+        unsigned char m = 0;
+        cnt[0] = 1;
+        if (ok)
+        {
+            prim = rec;
+            floors(prim, o, g.g);
+            reset = prim_differs(rec, prim);
+            if (reset)
+            {
+                cnt[2] = 1;
+                m |= GRHD_MASK_FLOORED;
+            }
+        }
+        else
+        {
+            cnt[1] = 1;
+            m |= GRHD_MASK_FAILED;
+            atmosphere_prim(o, prim);
+        }
+        store_prim(U + 5 * n, n, so, prim);
+        if (reset)
+        {
+            cnt[3] = 1;
+            m |= GRHD_MASK_RESET;
+            prim2con(prim, o.gamma, g.g, c);
+            store_cons(U, n, so, c);
+        }
+        if (write_mask)
+            p.mask[b.off_i + it] = m;
+    }
+    block_add_counts(p.st, cnt[0], cnt[1], cnt[2], cnt[3]);
+}
+
+/* add_scaled_conserved_rhs (FArrayBoxFiniteVolume.hpp:927-940) */
+__global__ void __launch_bounds__(kBlock)
+    k_add_scaled(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Pools p,
+                 double scale)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_INT, bp);
+    const BoxDev &b = *bp;
+    if (it < 0)
+        return;
+    const int i = b.lo[0] + (int)(it % b.in[0]);
+    const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+    const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+    const long long n = b.gnpts, ni = b.inpts;
+    double *U = pool_cons(p, POOL_STATE, b);
+    const double *r = p.rhs + p.rhs_ncomp * b.off_i + (long long)p.cb * ni;
+    const long long so = goff(b, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+        U[c * n + so] += scale * r[c * ni + it];
+}
+
+/* rhs.setVal(0.0) over every component of every box */
+__global__ void __launch_bounds__(kBlock)
+    k_zero_rhs(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Pools p)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_INT, bp);
+    const BoxDev &b = *bp;
+    if (it < 0)
+        return;
+    double *r = p.rhs + p.rhs_ncomp * b.off_i;
+    for (int c = 0; c < p.rhs_ncomp; ++c)
+        r[c * b.inpts + it] = 0.0;
+}
+
+/* max over non-atmosphere cells of sum_d max|lambda| / dx */
+__global__ void __launch_bounds__(kBlock)
+    k_max_inv_dt(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
+                 Background bg, Pools p)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_INT, bp);
+    const BoxDev &b = *bp;
+    double best = 0.0;
+    if (it >= 0)
+    {
+        const int i = b.lo[0] + (int)(it % b.in[0]);
+        const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+        const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+        const long long so = goff(b, i, j, k);
+        double x[3];
+        coords(i, j, k, b.dx, b.center, x);
+        Geo g;
+        cell_geometry(bg, x, g);
+        const Cons c = load_cons(pool_cons(p, POOL_STATE, b), b.gnpts, so);
+        Prim pr;
+        double res;
+        if (!con2prim(c, o, g.gu, pr, res))
+        {
+            const unsigned long long key =
+                (unsigned long long)(b.off_i + it);
+            atomicMin(&p.st->cfl_key, key);
+            p.W[key] = res;
+        }
+        else
+        {
+            floors(pr, o, g.g);
+            if (!is_atmosphere(pr, o, 1.0 + 1.0e-12))
+            {
+                const double inv_dx = b.inv_dx;
+                double cell = 0.0;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    double lo, hi;
+                    char_speeds(pr, o.gamma, g, d, lo, hi);
+                    const double ms = std_max(fabs(lo), fabs(hi));
+                    cell += ms * inv_dx;
+                }
+                best = std_max(best, cell);
+            }
+        }
+    }
+    /* non-negative doubles order like their bit patterns */
+    unsigned long long bits = __double_as_longlong(best);
+    bits = wave_max(bits);
+    __shared__ unsigned long long part[kBlock / 64];
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long m = 0;
+        for (int w = 0; w < kBlock / 64; ++w)
+            m = part[w] > m ? part[w] : m;
+        if (m)
+            atomicMax(&p.st->max_bits, m);
+    }
+}
+
+/* analytic cell geometry over the storage box of box 0 */
+__global__ void __launch_bounds__(kBlock)
+    k_geometry(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Background bg,
+               double *out)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_ALL, bp);
+    const BoxDev &b = *bp;
+    if (it < 0)
+        return;
+    const int i = b.glo[0] + (int)(it % b.gn[0]);
+    const int j = b.glo[1] + (int)((it / b.gn[0]) % b.gn[1]);
+    const int k = b.glo[2] + (int)(it / ((long long)b.gn[0] * b.gn[1]));
+    double x[3];
+    coords(i, j, k, b.dx, b.center, x);
+    Geo g;
+    cell_geometry(bg, x, g);
+    const long long n = b.gnpts;
+    double *o = out + 16 * b.off_g;
+    const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+    {
+        o[m * n + it] = g.g[ii[m]][jj[m]];
+        o[(6 + m) * n + it] = g.gu[ii[m]][jj[m]];
+    }
+    o[12 * n + it] = g.lapse;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        o[(13 + m) * n + it] = g.shift[m];
+}
+
+} // namespace grhd
+
+#endif /* GRHD_STAGE_KERNELS_HPP_ */

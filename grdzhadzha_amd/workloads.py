@@ -17,6 +17,8 @@ region) and use a vectorised Kerr-Schild metric.
   torus_box(n)    n^3 + 3 ghosts on [-n/4, n/4]^3 (dx = 0.5), Kerr a = 0.9,
                   Gaussian torus at R = 12, Keplerian Omega = 1/(R^1.5 + a),
                   Gamma = 4/3, minmod limiter; n = 128 is the 128^3 case.
+  binary_pn_levels  3 refinement levels x 180 boxes of seeded uneven sizes
+                  on the circular binary PN background, circumbinary ring.
 """
 import numpy as np
 
@@ -184,6 +186,113 @@ def torus_box(n=128, seed=2, ghosts=3, spin=0.9, R0=12.0, sigma=3.0,
     half = 0.25 * n
     box = cube_box(n, ghosts, half)
     M = 1.0
+    opts, state = torus_state(box, M, spin, R0, sigma, seed)
+    return Workload("torus_%d^3_kerr_a%.1f" % (n, spin), box,
+                    KerrSchildParams(M, spin), opts, state, 0.25 * box.dx,
+                    nsteps)
+
+
+def binary_pn_levels(seed=3, nlevels=3, nbox=180, base_cells=64,
+                     base_dx=2.0, sides=(8, 16, 24, 32, 48, 64), ghosts=3,
+                     ring_R=40.0, ring_sigma=8.0, nsteps=50):
+    """Batched box set: circular binary PN background (m1 = m2 = 0.5,
+    separation 20 M, circular PN orbit), `nlevels` refinement levels
+    (ratio 2) of `nbox` boxes each.  Level l has dx = base_dx / 2^l and an
+    index region of base_cells^3 cells centred on the origin; each box's
+    side lengths are drawn independently per axis from `sides` and its
+    corner uniformly in that region (SplitMix64 stream `seed`).  Every box
+    is its own ProblemDomain (outflow ghosts).  Circumbinary ring
+    rho0 = 1e-2 exp(-((R-40)^2 + z^2) / (2 8^2)) floored at 1e-10,
+    P = 0.01 rho0^(4/3), Keplerian Omega = (M/R^3)^(1/2) about the total
+    mass, Eulerian v^i = (Omega (-y, x, 0) + beta^i) / alpha, Gamma = 4/3,
+    MC limiter (theta = 2), volume-time sources on.  Returns
+    (background, options, levels) with levels = [(boxes, fabs, dt)],
+    dt = 0.25 dx of the level."""
+    from .options import CircularBinaryPNParams
+    bg = CircularBinaryPNParams(mass_1=0.5, mass_2=0.5, separation=20.0)
+    gamma = 4.0 / 3.0
+    atm = AtmosphereOptions(rho_floor=1e-10, pressure_floor=1e-12,
+                            max_velocity_squared=0.95)
+    rec = RecoveryOptions(max_iterations=100, tolerance=1e-12,
+                          pressure_floor=1e-12,
+                          max_velocity_squared=1.0 - 1.0e-12)
+    opts = OperatorOptions(adiabatic_index=gamma, atmosphere=atm,
+                           recovery=rec, limiter_theta=2.0,
+                           use_metric_volume_time_sources=True,
+                           metric_time_derivative_step=1.0e-3)
+    draws = splitmix64(np.arange(nlevels * nbox * 6, dtype=np.uint64) ^
+                       np.uint64(seed))
+    draws = draws.reshape(nlevels, nbox, 6)
+    levels = []
+    for lev in range(nlevels):
+        dx = base_dx / (2 ** lev)
+        half = 0.5 * base_cells * dx
+        boxes, fabs = [], []
+        for b in range(nbox):
+            side = [sides[int(draws[lev, b, d] % np.uint64(len(sides)))]
+                    for d in range(3)]
+            lo = [int(draws[lev, b, 3 + d] %
+                      np.uint64(base_cells - side[d] + 1)) for d in range(3)]
+            hi = [lo[d] + side[d] - 1 for d in range(3)]
+            box = BoxSpec(tuple(lo), tuple(hi), ghosts, dx, (half,) * 3)
+            X, Y, Z = cell_centres(box)
+            fabs.append(_pn_ring_state(box, bg, X, Y, Z, gamma, atm,
+                                       ring_R, ring_sigma))
+            boxes.append(box)
+        levels.append((boxes, fabs, 0.25 * dx))
+    return bg, opts, levels, nsteps
+
+
+def binary_pn_metric(bg, X, Y, Z, time=0.0):
+    """gamma_ij, lapse, shift^i of the softened circular binary
+    (Source/Background/CircularBinaryPN.hpp:70-130, 209-297), vectorised."""
+    M = bg.mass_1 + bg.mass_2
+    om = bg.angular_velocity if bg.angular_velocity != 0.0 else \
+        np.sqrt(M / bg.separation ** 3)
+    ph = bg.phase + om * (bg.time + time)
+    r1 = bg.mass_2 / M * bg.separation
+    r2 = bg.mass_1 / M * bg.separation
+    srcs = [(bg.mass_1, (r1 * np.cos(ph), r1 * np.sin(ph)),
+             (-om * r1 * np.sin(ph), om * r1 * np.cos(ph))),
+            (bg.mass_2, (-r2 * np.cos(ph), -r2 * np.sin(ph)),
+             (om * r2 * np.sin(ph), -om * r2 * np.cos(ph)))]
+    U = np.zeros_like(X)
+    V = np.zeros((3,) + X.shape)
+    for m, (px, py), (vx, vy) in srcs:
+        R = np.sqrt(bg.softening_radius ** 2 + (X - px) ** 2 + (Y - py) ** 2 +
+                    Z ** 2)
+        U += m / R
+        V[0] += m * vx / R
+        V[1] += m * vy / R
+    cf = 1.0 + 2.0 * U
+    g = np.zeros((3, 3) + X.shape)
+    for i in range(3):
+        g[i, i] = cf
+    lapse = np.sqrt(1.0 - 2.0 * U + 2.0 * U * U +
+                    16.0 * np.einsum("i...,i...->...", V, V) / cf)
+    shift = -4.0 * V / cf
+    return g, lapse, shift
+
+
+def _pn_ring_state(box, bg, X, Y, Z, gamma, atm, ring_R, ring_sigma):
+    R = np.sqrt(X * X + Y * Y)
+    rho = 1e-2 * np.exp(-((R - ring_R) ** 2 + Z * Z) /
+                        (2.0 * ring_sigma ** 2))
+    rho = np.maximum(rho, 1e-10)
+    p = 0.01 * rho ** (4.0 / 3.0)
+    M = bg.mass_1 + bg.mass_2
+    omega = np.sqrt(M / np.maximum(R, 1e-6) ** 3)
+    g, lapse, shift = binary_pn_metric(bg, X, Y, Z)
+    V = np.stack([-omega * Y, omega * X, np.zeros_like(X)])
+    v = (V + shift) / lapse
+    eps = p / ((gamma - 1.0) * rho)
+    rho, eps, p, v = apply_floors(rho, eps, p, v, g, gamma, atm)
+    D, S, tau = prim_to_cons(rho, eps, p, v, g, gamma)
+    return pack_state(box, rho, eps, p, v, D, S, tau)
+
+
+def torus_state(box, M=1.0, spin=0.9, R0=12.0, sigma=3.0, seed=2):
+    """Options and state FAB of torus_box's torus on any box."""
     gamma = 4.0 / 3.0
     atm = AtmosphereOptions(rho_floor=1e-10, pressure_floor=1e-12,
                             max_velocity_squared=0.95)
@@ -207,6 +316,4 @@ def torus_box(n=128, seed=2, ghosts=3, spin=0.9, R0=12.0, sigma=3.0,
     rho, eps, p, v = apply_floors(rho, eps, p, v, g, gamma, atm)
     D, S, tau = prim_to_cons(rho, eps, p, v, g, gamma)
     state = pack_state(box, rho, eps, p, v, D, S, tau)
-    return Workload("torus_%d^3_kerr_a%.1f" % (n, spin), box,
-                    KerrSchildParams(M, spin), opts, state, 0.25 * box.dx,
-                    nsteps)
+    return opts, state

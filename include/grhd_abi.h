@@ -25,7 +25,9 @@
  *
  * Return value: GRHD_OK (0) or one of the GRHD_ERR_* codes below; on a
  * recovery error `err` (if non-NULL) names the first failing cell in the
- * reference's BoxIterator order and the con2prim residual there.
+ * reference's BoxIterator order and the con2prim residual there.  After a
+ * recovery error the contents of the state, rhs and mask buffers are
+ * unspecified (the reference throws part-way through the same update).
  */
 #ifndef GRHD_ABI_H_
 #define GRHD_ABI_H_
@@ -271,6 +273,81 @@ int grhd_cell_geometry_box(grhd_ctx *ctx, const grhd_box *box,
                            const grhd_background *background, double time,
                            double *out, void *stream);
 
+
+
+/* ------------------------------------------------------------------ */
+/* Levels: every box of an AMR level in one launch per kernel.        */
+/* ------------------------------------------------------------------ */
+
+/* One box of a level and where its FArrayBox starts in the level's state
+ * pool (in doubles).  The pool holds each box's FAB (ncomp components
+ * over its storage box) at its offset. */
+typedef struct grhd_level_box
+{
+    grhd_box box;
+    long long state_offset;
+} grhd_level_box;
+
+/* Level RHS: LevelData exchange() of internal ghosts (cells of the
+ * ProblemDomain -- periodic directions wrapped -- that another box of the
+ * level owns), outflow fill of physical ghosts, then the RHS of every box.
+ * rhs_pool holds the boxes' interior FABs back to back in box order
+ * (ncomp x interior cells each); source_mask (nullable) one byte per
+ * interior cell, boxes back to back.  Replaces
+ * compute_leveldata_flux_rhs_from_fixed_background
+ * (FixedBGFArrayBoxFiniteVolume.hpp:322-363) with its exchange() calls
+ * (:310, :355) and the DataIterator loops over the level's boxes.
+ * GRHD_ERR_INVALID_ARGUMENT if a ghost inside the domain has no owner (a
+ * coarse-fine boundary, which needs interpolation from the coarser level). */
+int grhd_level_compute_rhs(grhd_ctx *ctx, const grhd_level_box *boxes,
+                           int nbox, const grhd_params *params,
+                           const grhd_background *background, double time,
+                           double *state_pool, int ncomp, int comp_base,
+                           double *rhs_pool, unsigned char *source_mask,
+                           grhd_error *err, void *stream);
+
+/* recover_primitives over the level (FixedBGFArrayBoxFiniteVolume.hpp:506-520,
+ * :728-743); diagnostics are level totals, as after the reference's
+ * MPI_Allreduce (FArrayBoxFiniteVolume.hpp:94-116). */
+int grhd_level_recover_primitives(grhd_ctx *ctx, const grhd_level_box *boxes,
+                                  int nbox, const grhd_params *params,
+                                  const grhd_background *background,
+                                  double time, double *state_pool, int ncomp,
+                                  int comp_base, unsigned char *mask,
+                                  grhd_diagnostics *diagnostics, void *stream);
+
+/* compute_leveldata_max_inverse_dt_from_fixed_background
+ * (FixedBGFArrayBoxFiniteVolume.hpp:414-444): maximum over the level. */
+int grhd_level_compute_max_inverse_dt(grhd_ctx *ctx,
+                                      const grhd_level_box *boxes, int nbox,
+                                      const grhd_params *params,
+                                      const grhd_background *background,
+                                      double time, const double *state_pool,
+                                      int ncomp, int comp_base,
+                                      double *max_inverse_dt, grhd_error *err,
+                                      void *stream);
+
+/* advance_ssprk2 of a whole level (FixedBGFArrayBoxFiniteVolume.hpp:563-589,
+ * :782-809), exchanging internal ghosts before each RHS. */
+int grhd_level_advance_ssprk2(grhd_ctx *ctx, const grhd_level_box *boxes,
+                              int nbox, const grhd_params *params,
+                              const grhd_background *background, double time,
+                              double dt, double *state_pool, int ncomp,
+                              int comp_base, unsigned char *mask,
+                              grhd_diagnostics *diagnostics, grhd_error *err,
+                              void *stream);
+
+/* `nsteps` RK4 steps of a whole level, as grhd_advance_rk4_box does for
+ * one box, exchanging internal ghosts before each stage's RHS: the level
+ * RK driver launches per level, not per box.  No counterpart in the
+ * reference (it steps GRHD with SSPRK2 only). */
+int grhd_level_advance_rk4(grhd_ctx *ctx, const grhd_level_box *boxes,
+                           int nbox, const grhd_params *params,
+                           const grhd_background *background, double time,
+                           double dt, int nsteps, double *state_pool,
+                           int ncomp, int comp_base, unsigned char *mask,
+                           grhd_diagnostics *diagnostics, grhd_error *err,
+                           void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Instrumentation (no counterpart in the reference).                 */

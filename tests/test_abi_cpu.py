@@ -37,6 +37,7 @@ STRUCTS = {
     "grhd_background": abi.GrhdBackground,
     "grhd_diagnostics": abi.GrhdDiagnostics,
     "grhd_error": abi.GrhdError,
+    "grhd_level_box": abi.GrhdLevelBox,
 }
 
 

@@ -1,0 +1,174 @@
+"""Level (batched) path on the GPU, through the level C-ABI.
+
+- A level of uneven boxes advanced in one launch per kernel equals each box
+  advanced alone, bit for bit, and matches the oracle per box.
+- A box split in two that share a ProblemDomain, with internal ghosts
+  exchanged on the device, gives the single-box RHS and steps bit for bit;
+  poisoned internal ghosts are repaired by the exchange
+  (GRHDLevelDataBinaryPNSmoke.cpp:204-233, logged error 0).
+- The reference's own 2-box LevelData Michel run (GRHD_PROGRESS.md:423-428)
+  reproduces its printed numbers on the GPU.
+"""
+import numpy as np
+import pytest
+import torch
+
+import michel
+import oracle
+from grdzhadzha_amd import workloads
+from grdzhadzha_amd.level import LevelOperator, split_box, sub_fab
+from grdzhadzha_amd.operator import FixedBGLevelDataFiniteVolumeOperator
+from helpers import assert_conserved_close
+
+pytestmark = pytest.mark.gpu
+
+
+def small_pn_level(nbox=10):
+    bg, opts, levels, _ = workloads.binary_pn_levels(
+        nlevels=2, nbox=nbox, base_cells=32, sides=(8, 16), base_dx=2.0)
+    boxes, fabs, dt = levels[1]
+    return bg, opts, boxes, fabs, dt
+
+
+def test_level_rk4_equals_each_box_alone(hip_context):
+    bg, opts, boxes, fabs, dt = small_pn_level()
+    lev = LevelOperator(bg, boxes, opts, time_dependent=True,
+                        context=hip_context)
+    pool = lev.pack(fabs)
+    mask = torch.zeros(lev.num_interior, dtype=torch.uint8, device="cuda")
+    diag = lev.advance_rk4(pool, dt, nsteps=2, time=0.0, mask=mask)
+    got = lev.unpack(pool)
+    masks = lev.split_interior(mask)
+    total = {k: 0 for k in diag}
+    for n, (box, fab) in enumerate(zip(boxes, fabs)):
+        from grdzhadzha_amd.operator import \
+            TimeDependentFixedBGLevelDataFiniteVolumeOperator as TD
+        op = TD(bg, box, opts, context=hip_context)
+        st = torch.from_numpy(np.ascontiguousarray(fab)).cuda()
+        m1 = torch.zeros(box.interior_shape, dtype=torch.uint8, device="cuda")
+        d1 = op.advance_rk4(st, dt, 2, time=0.0, mask=m1)
+        for k in total:
+            total[k] += d1[k]
+        assert np.array_equal(st.cpu().numpy(), got[n]), n
+        assert np.array_equal(m1.cpu().numpy(), masks[n]), n
+    assert total == diag
+    # and the oracle agrees on two of the boxes
+    for n in (0, len(boxes) - 1):
+        ref = np.ascontiguousarray(fabs[n]).copy()
+        dref, mref = oracle.Oracle(bg, boxes[n], opts,
+                                   time_dependent=True).advance_rk4(
+            ref, dt, 2, time=0.0)
+        assert_conserved_close(got[n], ref, 1e-12, what="box %d" % n)
+        assert np.array_equal(masks[n], mref)
+
+
+@pytest.fixture(scope="module")
+def michel_split():
+    cfg, box, bg, opts, state = michel.leveldata_setup()
+    O = oracle.Oracle(bg, box, opts)
+    O.recover_primitives(state)
+    halves = split_box(box, 0, 9)
+    return box, bg, opts, state, halves
+
+
+def test_level_rhs_split_equals_single_box(hip_context, michel_split):
+    box, bg, opts, state, halves = michel_split
+    single = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
+                                                  context=hip_context)
+    st = torch.from_numpy(state.copy()).cuda()
+    rhs = torch.zeros((11,) + box.interior_shape, dtype=torch.float64,
+                      device="cuda")
+    single.compute_rhs(rhs, st)
+    rhs_full = rhs.cpu().numpy()
+    lev = LevelOperator(bg, halves, opts, context=hip_context)
+    fabs = [sub_fab(state, box, h) for h in halves]
+    pool = lev.pack(fabs)
+    r = torch.zeros(11 * lev.num_interior, dtype=torch.float64, device="cuda")
+    lev.compute_rhs(r, pool)
+    parts = lev.split_interior(r, 11)
+    assert np.array_equal(parts[0][:5], rhs_full[:5, :, :, :9])
+    assert np.array_equal(parts[1][:5], rhs_full[:5, :, :, 9:])
+
+    # poisoned internal ghosts: the exchange repairs them
+    poisoned = []
+    for h, f in zip(halves, fabs):
+        f = f.copy()
+        x0 = h.glo[0]
+        for x in range(h.glo[0], h.ghi[0] + 1):
+            if (x < h.lo[0] or x > h.hi[0]) and 0 <= x <= 17:
+                f[:, :, :, x - x0] = 1e30 + np.arange(11)[:, None, None]
+        poisoned.append(f)
+    pool = lev.pack(poisoned)
+    r2 = torch.zeros_like(r)
+    lev.compute_rhs(r2, pool)
+    assert torch.equal(r2, r)
+
+
+def test_level_michel_anchor_on_gpu(hip_context, michel_split):
+    """The reference's 2-box LevelData Michel run: 18^3, dx 2, theta 1.2,
+    SSPRK2 with dt = min(CFL 0.2 stable dt, 0.02) to t = 0.1."""
+    box, bg, opts, state, halves = michel_split
+    lev = LevelOperator(bg, halves, opts, context=hip_context)
+    pool = lev.pack([sub_fab(state, box, h) for h in halves])
+
+    def mass(p):
+        return sum(f[0][h.interior_slices()].sum()
+                   for f, h in zip(lev.unpack(p), halves))
+
+    m0 = mass(pool)
+    t, steps = 0.0, 0
+    while t < 0.1 - 1e-14:
+        dt = min(min(lev.compute_stable_dt(pool, 0.2), 0.02), 0.1 - t)
+        lev.advance_ssprk2(pool, dt)
+        t += dt
+        steps += 1
+    m1 = mass(pool)
+    fabs = lev.unpack(pool)
+    max_rho = max(f[5][h.interior_slices()].max()
+                  for f, h in zip(fabs, halves))
+    max_p = max(f[7][h.interior_slices()].max()
+                for f, h in zip(fabs, halves))
+    assert steps == 5
+    assert float("%.6g" % m0) == 0.00172117
+    assert float("%.6g" % m1) == 0.00172335
+    assert float("%.6g" % ((m1 - m0) / m0)) == 0.00126778
+    assert float("%.6g" % max_rho) == 2.06653e-06
+    assert float("%.6g" % max_p) == 1.75256e-07
+    # and the split level equals the single box stepped the same way
+    single = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
+                                                  context=hip_context)
+    st = torch.from_numpy(state.copy()).cuda()
+    t = 0.0
+    while t < 0.1 - 1e-14:
+        dt = min(min(single.compute_stable_dt(st, 0.2), 0.02), 0.1 - t)
+        single.advance_ssprk2(st, dt)
+        t += dt
+    full = st.cpu().numpy()
+    for f, h in zip(fabs, halves):
+        sl = (slice(None),) + h.interior_slices()
+        ref = sub_fab(full, box, h)[sl]
+        assert np.array_equal(f[sl], ref)
+
+
+def test_level_rk4_split_equals_single_box(hip_context, michel_split):
+    box, bg, opts, state, halves = michel_split
+    single = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
+                                                  context=hip_context)
+    st = torch.from_numpy(state.copy()).cuda()
+    d_single = single.advance_rk4(st, 0.01, 3)
+    lev = LevelOperator(bg, halves, opts, context=hip_context)
+    pool = lev.pack([sub_fab(state, box, h) for h in halves])
+    d_level = lev.advance_rk4(pool, 0.01, 3)
+    assert d_single == d_level
+    full = st.cpu().numpy()
+    for f, h in zip(lev.unpack(pool), halves):
+        sl = (slice(None),) + h.interior_slices()
+        assert np.array_equal(f[sl], sub_fab(full, box, h)[sl])
+
+
+def test_level_rejects_unowned_internal_ghosts(hip_context, michel_split):
+    box, bg, opts, state, halves = michel_split
+    lev = LevelOperator(bg, halves[:1], opts, context=hip_context)
+    pool = lev.pack([sub_fab(state, box, halves[0])])
+    with pytest.raises(ValueError):
+        lev.advance_rk4(pool, 0.01, 1)

@@ -255,3 +255,39 @@ def test_ghost_fill_error_names_source(hip_context):
     with pytest.raises(oracle.OracleError) as e:
         oracle.Oracle(w.background, w.box, w.options).compute_rhs(st.copy())
     assert e.value.code == 1 and e.value.iv == (9, 4, 0)
+
+
+def _horizon_box_with_strict_floor():
+    """8^3 torus-state box whose x-low face touches the Kerr horizon; the
+    recovery's pressure floor (2e-12) sits above the atmosphere pressure
+    (1e-12).  The first RHS recovers everywhere; cells reset to atmosphere
+    by the first update cannot be recovered again, so the next evaluation's
+    ghost fill fails at an outflow source before its interior recovery."""
+    import dataclasses
+    box = BoxSpec((0, 0, 0), (7, 7, 7), 3, 0.5, (0.0, 2.0, 2.0))
+    opts, st = workloads.torus_state(box)
+    rec = dataclasses.replace(opts.recovery, pressure_floor=2.0e-12)
+    return box, dataclasses.replace(opts, recovery=rec), st
+
+
+@pytest.mark.parametrize("scheme", ["rk4", "ssprk2"])
+def test_later_evaluation_ghost_fill_error(hip_context, scheme):
+    box, opts, st = _horizon_box_with_strict_floor()
+    bg = KerrSchildParams(1.0, 0.9)
+    O = oracle.Oracle(bg, box, opts)
+    O.compute_rhs(st.copy())  # evaluation 0 recovers everywhere
+    with pytest.raises(oracle.OracleError) as e:
+        if scheme == "rk4":
+            O.advance_rk4(st.copy(), 0.125, 1)
+        else:
+            O.advance_ssprk2(st.copy(), 0.125)
+    assert e.value.code == 1
+    i, j, k = e.value.iv
+    op = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
+                                              context=hip_context)
+    with pytest.raises(RuntimeError,
+                       match=r"outflow source \(%d,%d,%d\)" % (i, j, k)):
+        if scheme == "rk4":
+            op.advance_rk4(to_dev(st.copy()), 0.125, 1)
+        else:
+            op.advance_ssprk2(to_dev(st.copy()), 0.125)
