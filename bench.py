@@ -13,6 +13,10 @@ dominant kernel (its average launch time measured with HIP events on the
 launch stream during the timed steps) and a CPU baseline (the oracle -- a C
 restatement of the reference's CPU path with its call structure -- timed on
 this host's cores on one RK4 step of the same box).
+
+The stage kernels are FP64 vector-ALU bound (no dense contraction, so no
+MFMA), so the roofline is FP64 VALU FLOP/s against the MI355X FP64 vector
+peak; the HBM fraction of the same launch is reported beside it.
 """
 import argparse
 import json
@@ -27,6 +31,7 @@ if ROOT not in sys.path:
 METRIC = ("GRHD cell-updates/s per RK4 step, plain FP64 vs mixed/batched "
           "at fixed error")
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 # Algorithmic HBM bytes per unit of each kernel class: every value the
 # kernel must read or write once, in FP64.
@@ -63,22 +68,36 @@ def measured_traffic(cls, n):
     return data.get("bytes_per_launch", {}).get(cls)
 
 
-def cpu_baseline(w):
-    """Oracle RK4 step of the same box on this host's cores."""
+def measured_flops(cls, n):
+    """FP64 FLOPs per launch of kernel class `cls` from the committed
+    rocprofv3 summary (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes), for the 128^3
+    box it was measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "fp64_flops.json")
+    if n != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data.get("flops_per_launch", {}).get(cls)
+
+
+def cpu_baseline(w, steps=3):
+    """Oracle RK4 steps of the same box on this host's cores (about 10 s on
+    16 cores for the 128^3 box)."""
     import numpy as np
 
     import oracle
     O = oracle.Oracle(w.background, w.box, w.options)
     st = np.ascontiguousarray(w.state).copy()
     t0 = time.perf_counter()
-    O.advance_rk4(st, w.dt, 1)
+    O.advance_rk4(st, w.dt, steps)
     t1 = time.perf_counter()
     cells = w.box.num_interior
-    return {"value": cells / (t1 - t0), "unit": "cell-updates/s",
+    return {"value": cells * steps / (t1 - t0), "unit": "cell-updates/s",
             "cores": oracle.num_threads(), "kind": "port",
-            "sample": "1 RK4 step of the same %d^3 box (%d interior cells), "
-                      "%.1f s wall, OpenMP over cells, reference call "
-                      "structure" % (w.box.interior_shape[0], cells, t1 - t0)}
+            "sample": "%d RK4 steps of the same %d^3 box (%d interior cells) "
+                      "from the same initial state, %.1f s wall, OpenMP over "
+                      "cells, reference call structure"
+                      % (steps, w.box.interior_shape[0], cells, t1 - t0)}
 
 
 def calibrate_copy():
@@ -159,13 +178,21 @@ def main():
     ms, nl = timing[dom]
     avg_s = ms / nl / 1e3
     units = units_per_launch(dom, args.n, w.box.ghosts)
-    achieved = ALG_BYTES[dom] * units / avg_s / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(dom, args.n),
-                "avg_launch_ms": ms / nl,
-                "alg_bytes_per_unit": ALG_BYTES[dom], "units_per_launch": units}
+    gbs = ALG_BYTES[dom] * units / avg_s / 1e9
+    hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": gbs / HBM_PEAK_GBS, "alg_bytes_per_unit": ALG_BYTES[dom]}
+    flops = measured_flops(dom, args.n)
+    common = {"kernel": dom, "traffic": measured_traffic(dom, args.n),
+              "avg_launch_ms": ms / nl, "units_per_launch": units}
+    if flops:
+        tf = flops / avg_s / 1e12
+        roofline = dict(bound="valu", achieved=tf, peak=FP64_PEAK_TFLOPS,
+                        unit="TFLOP/s", frac=tf / FP64_PEAK_TFLOPS,
+                        fp64_flops_per_unit=flops / units, hbm=hbm, **common)
+    else:
+        roofline = dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS,
+                        unit="GB/s", frac=gbs / HBM_PEAK_GBS,
+                        alg_bytes_per_unit=ALG_BYTES[dom], **common)
 
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -178,7 +205,7 @@ def main():
                       "dt": w.dt, "parallelism": "single GPU"},
            "roofline": roofline}
     if not args.no_cpu_baseline:
-        print("bench: timing the CPU oracle on one RK4 step ...",
+        print("bench: timing the CPU oracle on 3 RK4 steps ...",
               file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

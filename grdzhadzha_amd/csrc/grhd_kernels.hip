@@ -335,6 +335,8 @@ struct grhd_ctx
     long long cap_u = 0;
     double *acc = nullptr;
     long long cap_i = 0;
+    double *diag = nullptr; /* per-block partials of the level diagnostics */
+    long long cap_diag = 0;
     DevStatus *dstat = nullptr;
     DevStatus *hstat = nullptr;
     /* instrumentation */
@@ -759,6 +761,29 @@ int level_max_inv_dt(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
     return GRHD_OK;
 }
 
+/* Conserved sums and the count of interior cells holding a non-finite
+ * conserved value, over the level (LevelDataDriver.hpp:22-92). */
+int level_diagnostics(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
+                      const double *state, int ncomp, int cb, void *st)
+{
+    if (!state || cb < 0 || cb + GRHD_NUM_COMPS > ncomp)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    int rc = prepare(c, boxes, nbox, ncomp);
+    if (rc)
+        return rc;
+    const long long nb = c->plan.nblocks[C_INT];
+    if ((rc = grow(&c->diag, &c->cap_diag, kDiagWords * nb)))
+        return rc;
+    hipStream_t s = (hipStream_t)st;
+    Pools p = make_pools(c, const_cast<double *>(state), cb);
+    launch_status_init(c, s);
+    GRHD_LAUNCH(c, C_INT, k_diag_cells, s, p, c->diag);
+    hipLaunchKernelGGL(k_diag_finish, dim3(1), dim3(kBlock), 0, s, c->diag,
+                       nb, c->dstat);
+    tick(c, GRHD_KERNEL_OTHER, s);
+    return finish(c, s, nullptr, nullptr, false);
+}
+
 int level_ssprk2(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
                  const grhd_params *params, const grhd_background *bg,
                  double time, double dt, double *state, int ncomp, int cb,
@@ -904,6 +929,7 @@ void grhd_ctx_destroy(grhd_ctx *c)
     (void)hipFree(c->UA);
     (void)hipFree(c->UB);
     (void)hipFree(c->acc);
+    (void)hipFree(c->diag);
     for (hipEvent_t e : c->events)
         (void)hipEventDestroy(e);
     (void)hipFree(c->dstat);
@@ -1113,6 +1139,37 @@ int grhd_level_compute_max_inverse_dt(grhd_ctx *ctx,
     return level_max_inv_dt(ctx, boxes, nbox, params, background, time,
                             state_pool, ncomp, comp_base, max_inverse_dt, err,
                             stream);
+}
+
+int grhd_level_conserved_sum(grhd_ctx *ctx, const grhd_level_box *boxes,
+                             int nbox, const double *state_pool, int ncomp,
+                             int comp_base, double *sum, void *stream)
+{
+    if (!sum)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    const int rc = level_diagnostics(ctx, boxes, nbox, state_pool, ncomp,
+                                     comp_base, stream);
+    if (rc)
+        return rc;
+    for (int m = 0; m < 5; ++m)
+        sum[m] = ctx->hstat->sums[m];
+    return GRHD_OK;
+}
+
+int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
+                                         const grhd_level_box *boxes,
+                                         int nbox, const double *state_pool,
+                                         int ncomp, int comp_base,
+                                         int *is_finite, void *stream)
+{
+    if (!is_finite)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    const int rc = level_diagnostics(ctx, boxes, nbox, state_pool, ncomp,
+                                     comp_base, stream);
+    if (rc)
+        return rc;
+    *is_finite = ctx->hstat->nonfinite == 0 ? 1 : 0;
+    return GRHD_OK;
 }
 
 int grhd_level_advance_ssprk2(grhd_ctx *ctx, const grhd_level_box *boxes,

@@ -62,6 +62,8 @@ struct DevStatus
     alignas(128) unsigned long long cfl_key; /* off_i + interior index */
     unsigned long long counts[4];
     unsigned long long max_bits;
+    double sums[5];              /* level conserved sums */
+    unsigned long long nonfinite; /* interior cells with a non-finite value */
 };
 
 /* One box of a level as the kernels see it. */
@@ -1150,6 +1152,185 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int m = 0; m < 3; ++m)
         o[(13 + m) * n + it] = g.shift[m];
+}
+
+/* ------------------------------------------------------------------ */
+/* Level diagnostics: sums of D, S_1, S_2, S_3, tau over every interior */
+/* cell and their finiteness (LevelDataDriver.hpp:22-92).  The         */
+/* reference adds sequentially; here every cell value enters a         */
+/* double-double accumulator (Knuth's TwoSum, the QD library's         */
+/* accurate addition) in a fixed order -- wavefront butterfly, waves   */
+/* of a block in order, blocks in order -- so the result is the exact  */
+/* sum rounded once in all but contrived cases, and the same on every  */
+/* run.  A non-finite cell makes the plain sum the result.             */
+/* ------------------------------------------------------------------ */
+struct DD
+{
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD two_sum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+
+__device__ __forceinline__ DD fast_two_sum(double a, double b)
+{
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    DD s = two_sum(a.hi, b.hi);
+    const DD t = two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = fast_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return fast_two_sum(s.hi, s.lo);
+}
+
+constexpr int kDiagWords = 16; /* per block: hi[5], lo[5], plain[5], bad */
+
+__device__ __forceinline__ DD shfl_xor_dd(DD v, int off)
+{
+    return {__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)};
+}
+
+/* Reduce (acc[5], plain[5], bad) over the block in a fixed order; thread 0
+ * returns the block's values.  dd_add is commutative bit for bit (TwoSum
+ * is exact in either order), so both lanes of a butterfly pair agree. */
+__device__ inline void block_reduce_diag(DD acc[5], double plain[5],
+                                         double &bad)
+{
+    __shared__ double part[kBlock / 64][kDiagWords];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            acc[m] = dd_add(acc[m], shfl_xor_dd(acc[m], off));
+            plain[m] += __shfl_xor(plain[m], off, 64);
+        }
+        bad += __shfl_xor(bad, off, 64);
+    }
+    if (lane == 0)
+    {
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            part[wave][m] = acc[m].hi;
+            part[wave][5 + m] = acc[m].lo;
+            part[wave][10 + m] = plain[m];
+        }
+        part[wave][15] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int w = 1; w < kBlock / 64; ++w)
+        {
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+            {
+                acc[m] = dd_add(acc[m], DD{part[w][m], part[w][5 + m]});
+                plain[m] += part[w][10 + m];
+            }
+            bad += part[w][15];
+        }
+    }
+}
+
+/* Per block of interior cells: partial sums into diag[block * 16 ..]. */
+__global__ void __launch_bounds__(kBlock)
+    k_diag_cells(const BoxDev *__restrict__ boxes,
+                 const int2 *__restrict__ items, Pools p, double *diag)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_INT, bp);
+    const BoxDev &b = *bp;
+    DD acc[5];
+    double plain[5];
+    double bad = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+    {
+        acc[m] = DD{0.0, 0.0};
+        plain[m] = 0.0;
+    }
+    if (it >= 0)
+    {
+        const int i = b.lo[0] + (int)(it % b.in[0]);
+        const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+        const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+        const Cons c =
+            load_cons(pool_cons(p, POOL_STATE, b), b.gnpts, goff(b, i, j, k));
+        const double v[5] = {c.D, c.S[0], c.S[1], c.S[2], c.tau};
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            acc[m].hi = v[m];
+            plain[m] = v[m];
+            if (!isfinite(v[m]))
+                bad = 1.0;
+        }
+    }
+    block_reduce_diag(acc, plain, bad);
+    if (threadIdx.x == 0)
+    {
+        double *d = diag + (long long)blockIdx.x * kDiagWords;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            d[m] = acc[m].hi;
+            d[5 + m] = acc[m].lo;
+            d[10 + m] = plain[m];
+        }
+        d[15] = bad;
+    }
+}
+
+/* One block: the partials of `nblocks` blocks, thread t taking blocks
+ * t, t + 256, ... in order, then the block reduction. */
+__global__ void __launch_bounds__(kBlock)
+    k_diag_finish(const double *diag, long long nblocks, DevStatus *st)
+{
+    DD acc[5];
+    double plain[5];
+    double bad = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+    {
+        acc[m] = DD{0.0, 0.0};
+        plain[m] = 0.0;
+    }
+    for (long long q = threadIdx.x; q < nblocks; q += kBlock)
+    {
+        const double *d = diag + q * kDiagWords;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            acc[m] = dd_add(acc[m], DD{d[m], d[5 + m]});
+            plain[m] += d[10 + m];
+        }
+        bad += d[15];
+    }
+    block_reduce_diag(acc, plain, bad);
+    if (threadIdx.x == 0)
+    {
+        bool finite = true;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            finite = finite && isfinite(acc[m].hi) && isfinite(plain[m]);
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            st->sums[m] = finite ? acc[m].hi : plain[m];
+        st->nonfinite = (unsigned long long)bad;
+    }
 }
 
 } // namespace grhd

@@ -160,6 +160,40 @@ class LevelOperator:
         return diag.as_dict()
 
 
+    # ------------------------------------------------------- diagnostics
+    def conserved_sum(self, pool, stream=None):
+        """Level sums of D, S_1, S_2, S_3, tau over interior cells
+        (compute_leveldata_conserved_sum, LevelDataDriver.hpp:22-57), as a
+        5-tuple of floats."""
+        self._check(pool)
+        out = (ctypes.c_double * 5)()
+        rc = self.lib.grhd_level_conserved_sum(
+            self.ctx.handle, self._cboxes, len(self.boxes), _ptr(pool),
+            self.ncomp, self.comp_base, out, _stream(stream))
+        _raise(rc, None, "level conserved_sum")
+        return tuple(out)
+
+    def conserved_state_is_finite(self, pool, stream=None):
+        """leveldata_conserved_state_is_finite (LevelDataDriver.hpp:59-92)."""
+        self._check(pool)
+        out = ctypes.c_int()
+        rc = self.lib.grhd_level_conserved_state_is_finite(
+            self.ctx.handle, self._cboxes, len(self.boxes), _ptr(pool),
+            self.ncomp, self.comp_base, ctypes.byref(out), _stream(stream))
+        _raise(rc, None, "level conserved_state_is_finite")
+        return bool(out.value)
+
+
+def compute_leveldata_conserved_sum(level: "LevelOperator", pool):
+    """Reference-named free function (LevelDataDriver.hpp:22-57)."""
+    return level.conserved_sum(pool)
+
+
+def leveldata_conserved_state_is_finite(level: "LevelOperator", pool):
+    """Reference-named free function (LevelDataDriver.hpp:67-92)."""
+    return level.conserved_state_is_finite(pool)
+
+
 def split_box(box: BoxSpec, axis: int, at: int):
     """Two boxes splitting `box` along `axis` (cells < at, cells >= at),
     sharing its ProblemDomain -- the reference's x-split layouts

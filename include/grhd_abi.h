@@ -327,6 +327,27 @@ int grhd_level_compute_max_inverse_dt(grhd_ctx *ctx,
                                       double *max_inverse_dt, grhd_error *err,
                                       void *stream);
 
+/* compute_leveldata_conserved_sum (LevelDataDriver.hpp:22-57): sum[0..4] =
+ * sums of D, S_1, S_2, S_3, tau over every interior cell of the level.  The
+ * reference adds sequentially in box and BoxIterator order; this adds in a
+ * fixed order in double-double arithmetic and rounds once, so the result is
+ * deterministic and within (cells - 1) eps sum|x| of the reference's (the
+ * bound of its own sequential rounding).  If any value is not finite the
+ * plain floating-point sum is returned. */
+int grhd_level_conserved_sum(grhd_ctx *ctx, const grhd_level_box *boxes,
+                             int nbox, const double *state_pool, int ncomp,
+                             int comp_base, double *sum, void *stream);
+
+/* leveldata_conserved_state_is_finite + conserved_is_finite
+ * (LevelDataDriver.hpp:59-92):
+ * *is_finite = 1 if D, S_i and tau are finite in every interior cell of
+ * the level, else 0. */
+int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
+                                         const grhd_level_box *boxes,
+                                         int nbox, const double *state_pool,
+                                         int ncomp, int comp_base,
+                                         int *is_finite, void *stream);
+
 /* advance_ssprk2 of a whole level (FixedBGFArrayBoxFiniteVolume.hpp:563-589,
  * :782-809), exchanging internal ghosts before each RHS. */
 int grhd_level_advance_ssprk2(grhd_ctx *ctx, const grhd_level_box *boxes,

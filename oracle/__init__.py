@@ -62,6 +62,10 @@ def lib():
             "grhd_oracle_log_sqrt_det_dt": (d, [Bg, d, vp, d, vp, d]),
             "grhd_oracle_log": (d, [d]),
             "grhd_oracle_num_threads": (i, []),
+            "grhd_oracle_leveldata_conserved_sum": (None, [B, i, P(vp), i, i,
+                                                           P(d)]),
+            "grhd_oracle_leveldata_conserved_state_is_finite": (
+                i, [B, i, P(vp), i, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -293,3 +297,27 @@ def minmod(a, b):
 
 def num_threads():
     return lib().grhd_oracle_num_threads()
+
+
+def _level_args(boxes, fabs):
+    cb = (abi.GrhdBox * len(boxes))(*[b.to_c() for b in boxes])
+    arrs = [np.ascontiguousarray(f, dtype=np.float64) for f in fabs]
+    ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    return cb, arrs, ptrs
+
+
+def leveldata_conserved_sum(boxes, fabs, comp_base=0):
+    """Sequential level sums of D, S_1, S_2, S_3, tau (reference order)."""
+    cb, arrs, ptrs = _level_args(boxes, fabs)
+    out = (ctypes.c_double * 5)()
+    ncomp = arrs[0].shape[0]
+    lib().grhd_oracle_leveldata_conserved_sum(cb, len(boxes), ptrs, ncomp,
+                                              comp_base, out)
+    return tuple(out)
+
+
+def leveldata_conserved_state_is_finite(boxes, fabs, comp_base=0):
+    cb, arrs, ptrs = _level_args(boxes, fabs)
+    ncomp = arrs[0].shape[0]
+    return bool(lib().grhd_oracle_leveldata_conserved_state_is_finite(
+        cb, len(boxes), ptrs, ncomp, comp_base))

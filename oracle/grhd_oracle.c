@@ -2201,3 +2201,64 @@ double grhd_oracle_log_sqrt_det_dt(const grhd_background *bgp, double time,
 }
 
 int grhd_oracle_num_threads(void) { return omp_get_max_threads(); }
+
+/* compute_leveldata_conserved_sum (LevelDataDriver.hpp:22-57): one
+ * sequential accumulator per component over boxes, then BoxIterator order. */
+void grhd_oracle_leveldata_conserved_sum(const grhd_box *boxes, int nbox,
+                                         const double *const *fabs, int ncomp,
+                                         int comp_base, double sum[5])
+{
+    (void)ncomp;
+    for (int m = 0; m < 5; ++m)
+        sum[m] = 0.0;
+    for (int n = 0; n < nbox; ++n)
+    {
+        const grhd_box *b = &boxes[n];
+        const long long nx = b->ghi[0] - b->glo[0] + 1;
+        const long long ny = b->ghi[1] - b->glo[1] + 1;
+        const long long nz = b->ghi[2] - b->glo[2] + 1;
+        const long long npts = nx * ny * nz;
+        const double *U = fabs[n] + (long long)comp_base * npts;
+        for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+            for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+                for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+                {
+                    const long long o = ((k - b->glo[2]) * ny +
+                                         (j - b->glo[1])) * nx +
+                                        (i - b->glo[0]);
+                    for (int m = 0; m < 5; ++m)
+                        sum[m] += U[m * npts + o];
+                }
+    }
+}
+
+/* leveldata_conserved_state_is_finite + conserved_is_finite
+ * (LevelDataDriver.hpp:59-92) */
+int grhd_oracle_leveldata_conserved_state_is_finite(
+    const grhd_box *boxes, int nbox, const double *const *fabs, int ncomp,
+    int comp_base)
+{
+    (void)ncomp;
+    int ok = 1;
+    for (int n = 0; n < nbox; ++n)
+    {
+        const grhd_box *b = &boxes[n];
+        const long long nx = b->ghi[0] - b->glo[0] + 1;
+        const long long ny = b->ghi[1] - b->glo[1] + 1;
+        const long long nz = b->ghi[2] - b->glo[2] + 1;
+        const long long npts = nx * ny * nz;
+        const double *U = fabs[n] + (long long)comp_base * npts;
+        for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+            for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+                for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+                {
+                    const long long o = ((k - b->glo[2]) * ny +
+                                         (j - b->glo[1])) * nx +
+                                        (i - b->glo[0]);
+                    for (int m = 0; m < 5; ++m)
+                        if (!isfinite(U[m * npts + o]))
+                            ok = 0;
+                }
+    }
+    return ok;
+}

@@ -79,6 +79,16 @@ double grhd_oracle_log_sqrt_det_dt(const grhd_background *bg, double time,
 double grhd_oracle_log(double x);
 int grhd_oracle_num_threads(void);
 
+/* Level diagnostics over `nbox` FABs (each ncomp components on its box's
+ * storage), interior cells only, in the reference's order: boxes in turn,
+ * BoxIterator (i fastest) within a box. */
+void grhd_oracle_leveldata_conserved_sum(const grhd_box *boxes, int nbox,
+                                         const double *const *fabs, int ncomp,
+                                         int comp_base, double sum[5]);
+int grhd_oracle_leveldata_conserved_state_is_finite(
+    const grhd_box *boxes, int nbox, const double *const *fabs, int ncomp,
+    int comp_base);
+
 #ifdef __cplusplus
 }
 #endif

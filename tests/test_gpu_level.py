@@ -172,3 +172,48 @@ def test_level_rejects_unowned_internal_ghosts(hip_context, michel_split):
     pool = lev.pack([sub_fab(state, box, halves[0])])
     with pytest.raises(ValueError):
         lev.advance_rk4(pool, 0.01, 1)
+
+
+# ------------------------------------------------------------ diagnostics
+def _interior(boxes, fabs, m):
+    return np.concatenate([f[m][b.interior_slices()].ravel()
+                           for b, f in zip(boxes, fabs)])
+
+
+def test_level_conserved_sum_and_finiteness(hip_context):
+    import math
+    from grdzhadzha_amd.level import (compute_leveldata_conserved_sum,
+                                      leveldata_conserved_state_is_finite)
+    bg, opts, boxes, fabs, dt = small_pn_level(nbox=24)
+    lev = LevelOperator(bg, boxes, opts, context=hip_context)
+    pool = lev.pack(fabs)
+    got = compute_leveldata_conserved_sum(lev, pool)
+    assert got == lev.conserved_sum(pool)  # deterministic
+    seq = oracle.leveldata_conserved_sum(boxes, fabs)
+    eps = np.finfo(float).eps
+    for m in range(5):
+        v = _interior(boxes, fabs, m)
+        exact = math.fsum(v)
+        # double-double accumulation rounds the exact sum once
+        assert abs(got[m] - exact) <= 0.5 * np.spacing(abs(exact)) + 1e-300
+        # and differs from the reference's sequential sum by no more than
+        # that sum's own rounding bound
+        bound = (len(v) - 1) * eps / 2 * np.abs(v).sum()
+        assert abs(got[m] - seq[m]) <= bound
+    assert leveldata_conserved_state_is_finite(lev, pool)
+
+    # a NaN in an interior cell of one box; a ghost-only inf does not count
+    bad = [f.copy() for f in fabs]
+    b = boxes[5]
+    bad[5][2, b.lo[2] - b.glo[2], b.hi[1] - b.glo[1], b.lo[0] - b.glo[0]] = \
+        np.nan
+    assert not lev.conserved_state_is_finite(lev.pack(bad))
+    assert not oracle.leveldata_conserved_state_is_finite(boxes, bad)
+    s = lev.conserved_sum(lev.pack(bad))
+    assert np.isnan(s[2]) and np.isnan(oracle.leveldata_conserved_sum(
+        boxes, bad)[2])
+    ghost = [f.copy() for f in fabs]
+    ghost[7][0, 0, 0, 0] = np.inf
+    p = lev.pack(ghost)
+    assert lev.conserved_state_is_finite(p)
+    assert lev.conserved_sum(p) == got
