@@ -138,6 +138,10 @@ SIGNATURES = {
     "grhd_level_conserved_state_is_finite": (_i, [_vp, _P(GrhdLevelBox), _i,
                                                   _vp, _i, _i, _P(_i),
                                                   _vp]),
+    "grhd_level_checkpoint_pack": (_i, [_vp, _P(GrhdLevelBox), _i, _vp, _i,
+                                        _P(_i), _P(_i), _i, _vp, _vp]),
+    "grhd_level_checkpoint_unpack": (_i, [_vp, _P(GrhdLevelBox), _i, _vp,
+                                          _i, _P(_i), _P(_i), _i, _vp, _vp]),
     "grhd_level_advance_ssprk2": (_i, [_vp, _P(GrhdLevelBox), _i,
                                        _P(GrhdParams), _P(GrhdBackground), _d,
                                        _d, _vp, _i, _i, _vp,

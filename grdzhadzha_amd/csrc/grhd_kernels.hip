@@ -784,6 +784,63 @@ int level_diagnostics(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
     return finish(c, s, nullptr, nullptr, false);
 }
 
+/* The checkpoint's grid: the ProblemDomain grown by `ghosts`.  Every box's
+ * storage must lie inside it ("state ghost box exceeds checkpoint storage",
+ * Main_GRHDLevelDataBinaryPNKITPDisk.cpp:1576-1579). */
+int ckpt_grid(const Plan &pl, const int *dlo, const int *dhi, int ghosts,
+              CkptGrid &g, long long &cells)
+{
+    if (!dlo || !dhi || ghosts < 0)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    cells = 1;
+    for (int d = 0; d < 3; ++d)
+    {
+        if (dhi[d] < dlo[d])
+            return GRHD_ERR_INVALID_ARGUMENT;
+        g.lo[d] = dlo[d] - ghosts;
+        g.n[d] = dhi[d] - dlo[d] + 1 + 2 * ghosts;
+        g.dlo[d] = dlo[d];
+        g.dhi[d] = dhi[d];
+        cells *= g.n[d];
+    }
+    for (const BoxDev &b : pl.hb)
+        for (int d = 0; d < 3; ++d)
+            if (b.glo[d] < g.lo[d] || b.ghi[d] > g.lo[d] + g.n[d] - 1)
+                return GRHD_ERR_INVALID_ARGUMENT;
+    return GRHD_OK;
+}
+
+int level_checkpoint(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
+                     double *state, int ncomp, const int *dlo, const int *dhi,
+                     int ghosts, double *global, bool pack, void *st)
+{
+    if (!state || !global || ncomp < 1)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    int rc = prepare(c, boxes, nbox, ncomp);
+    if (rc)
+        return rc;
+    CkptGrid g;
+    long long cells = 0;
+    if ((rc = ckpt_grid(c->plan, dlo, dhi, ghosts, g, cells)))
+        return rc;
+    hipStream_t s = (hipStream_t)st;
+    if (pack)
+    {
+        const long long n = cells * ncomp;
+        hipLaunchKernelGGL(k_fill, dim3(grid_of(n)), dim3(kBlock), 0, s,
+                           global, n, 0.0);
+        GRHD_LAUNCH(c, C_ALL, k_ckpt_pack, s, state, ncomp, g, global);
+    }
+    else
+    {
+        GRHD_LAUNCH(c, C_ALL, k_ckpt_unpack, s, global, ncomp, g, state);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+        return GRHD_ERR_DEVICE;
+    return GRHD_OK;
+}
+
 int level_ssprk2(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
                  const grhd_params *params, const grhd_background *bg,
                  double time, double dt, double *state, int ncomp, int cb,
@@ -1170,6 +1227,29 @@ int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
         return rc;
     *is_finite = ctx->hstat->nonfinite == 0 ? 1 : 0;
     return GRHD_OK;
+}
+
+int grhd_level_checkpoint_pack(grhd_ctx *ctx, const grhd_level_box *boxes,
+                               int nbox, const double *state_pool, int ncomp,
+                               const int *domain_lo, const int *domain_hi,
+                               int ghosts, double *checkpoint_values,
+                               void *stream)
+{
+    return level_checkpoint(ctx, boxes, nbox, const_cast<double *>(state_pool),
+                            ncomp, domain_lo, domain_hi, ghosts,
+                            checkpoint_values, true, stream);
+}
+
+int grhd_level_checkpoint_unpack(grhd_ctx *ctx, const grhd_level_box *boxes,
+                                 int nbox, const double *checkpoint_values,
+                                 int ncomp, const int *domain_lo,
+                                 const int *domain_hi, int ghosts,
+                                 double *state_pool, void *stream)
+{
+    return level_checkpoint(ctx, boxes, nbox, state_pool, ncomp, domain_lo,
+                            domain_hi, ghosts,
+                            const_cast<double *>(checkpoint_values), false,
+                            stream);
 }
 
 int grhd_level_advance_ssprk2(grhd_ctx *ctx, const grhd_level_box *boxes,

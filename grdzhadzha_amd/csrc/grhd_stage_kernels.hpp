@@ -1333,6 +1333,92 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* KITPCHK1 payload (Main_GRHDLevelDataBinaryPNKITPDisk.cpp:1348-1375,  */
+/* :1385-1425, :1516-1586): the ProblemDomain grown by the ghost width, */
+/* point-major with the ncomp components of a cell adjacent:            */
+/*   global[((k * ny + j) * nx + i) * ncomp + c],                       */
+/* indices relative to the grown box.  Pack writes every storage cell   */
+/* whose index clamped to the domain lies in the box's interior (so     */
+/* interior cells and the physical ghosts beside them, once each);      */
+/* unpack fills every storage cell of every box.                        */
+/* ------------------------------------------------------------------ */
+struct CkptGrid
+{
+    int lo[3];  /* corner of the grown domain */
+    int n[3];   /* its extent                 */
+    int dlo[3]; /* ProblemDomain              */
+    int dhi[3];
+};
+
+__device__ __forceinline__ long long ckpt_cell(const CkptGrid &g, int i,
+                                               int j, int k)
+{
+    return ((long long)(k - g.lo[2]) * g.n[1] + (j - g.lo[1])) * g.n[0] +
+           (i - g.lo[0]);
+}
+
+__device__ __forceinline__ void storage_cell(const BoxDev &b, long long it,
+                                             int &i, int &j, int &k)
+{
+    i = b.glo[0] + (int)(it % b.gn[0]);
+    j = b.glo[1] + (int)((it / b.gn[0]) % b.gn[1]);
+    k = b.glo[2] + (int)(it / ((long long)b.gn[0] * b.gn[1]));
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_fill(double *x, long long n, double v)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        x[t] = v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ckpt_pack(const BoxDev *__restrict__ boxes,
+                const int2 *__restrict__ items, const double *state,
+                int ncomp, CkptGrid g, double *global)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_ALL, bp);
+    const BoxDev &b = *bp;
+    if (it < 0)
+        return;
+    int i, j, k;
+    storage_cell(b, it, i, j, k);
+    const int iv[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const int o = iv[d] < g.dlo[d] ? g.dlo[d]
+                                       : (iv[d] > g.dhi[d] ? g.dhi[d] : iv[d]);
+        if (o < b.lo[d] || o > b.hi[d])
+            return;
+    }
+    const double *src = state + b.off_u + goff(b, i, j, k);
+    double *dst = global + ckpt_cell(g, i, j, k) * ncomp;
+    for (int c = 0; c < ncomp; ++c)
+        dst[c] = src[(long long)c * b.gnpts];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ckpt_unpack(const BoxDev *__restrict__ boxes,
+                  const int2 *__restrict__ items, const double *global,
+                  int ncomp, CkptGrid g, double *state)
+{
+    const BoxDev *bp;
+    const long long it = block_item(items, boxes, C_ALL, bp);
+    const BoxDev &b = *bp;
+    if (it < 0)
+        return;
+    int i, j, k;
+    storage_cell(b, it, i, j, k);
+    const double *src = global + ckpt_cell(g, i, j, k) * ncomp;
+    double *dst = state + b.off_u + goff(b, i, j, k);
+    for (int c = 0; c < ncomp; ++c)
+        dst[(long long)c * b.gnpts] = src[c];
+}
+
 } // namespace grhd
 
 #endif /* GRHD_STAGE_KERNELS_HPP_ */

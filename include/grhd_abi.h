@@ -348,6 +348,28 @@ int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
                                          int ncomp, int comp_base,
                                          int *is_finite, void *stream);
 
+/* KITPCHK1 checkpoint payload <-> level pool, on the device.  The payload
+ * covers the ProblemDomain [domain_lo, domain_hi] grown by `ghosts`, point
+ * major with the ncomp components of a cell adjacent (ncomp = NUM_VARS):
+ *   values[(((k - lo2) * ny + (j - lo1)) * nx + (i - lo0)) * ncomp + c].
+ * Pack zeroes the payload, then writes each storage cell whose index clamped
+ * to the domain lies in its box's interior (write_checkpoint,
+ * Main_GRHDLevelDataBinaryPNKITPDisk.cpp:1385-1425); unpack fills every
+ * storage cell of every box (read_checkpoint_state, :1516-1586), so a
+ * checkpoint written with one box layout restores another.  Each box's
+ * storage must lie inside the grown domain.  The header and file I/O stay
+ * on the host with the caller. */
+int grhd_level_checkpoint_pack(grhd_ctx *ctx, const grhd_level_box *boxes,
+                               int nbox, const double *state_pool, int ncomp,
+                               const int *domain_lo, const int *domain_hi,
+                               int ghosts, double *checkpoint_values,
+                               void *stream);
+int grhd_level_checkpoint_unpack(grhd_ctx *ctx, const grhd_level_box *boxes,
+                                 int nbox, const double *checkpoint_values,
+                                 int ncomp, const int *domain_lo,
+                                 const int *domain_hi, int ghosts,
+                                 double *state_pool, void *stream);
+
 /* advance_ssprk2 of a whole level (FixedBGFArrayBoxFiniteVolume.hpp:563-589,
  * :782-809), exchanging internal ghosts before each RHS. */
 int grhd_level_advance_ssprk2(grhd_ctx *ctx, const grhd_level_box *boxes,

@@ -66,6 +66,10 @@ def lib():
                                                            P(d)]),
             "grhd_oracle_leveldata_conserved_state_is_finite": (
                 i, [B, i, P(vp), i, i]),
+            "grhd_oracle_checkpoint_pack": (None, [B, i, P(vp), i, P(i), P(i),
+                                                   i, vp]),
+            "grhd_oracle_checkpoint_unpack": (None, [B, i, P(vp), i, P(i),
+                                                     P(i), i, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -321,3 +325,31 @@ def leveldata_conserved_state_is_finite(boxes, fabs, comp_base=0):
     ncomp = arrs[0].shape[0]
     return bool(lib().grhd_oracle_leveldata_conserved_state_is_finite(
         cb, len(boxes), ptrs, ncomp, comp_base))
+
+
+def checkpoint_pack(boxes, fabs, domain_lo, domain_hi, ghosts):
+    """KITPCHK1 payload (float64, point major) of a level of FABs."""
+    cb, arrs, ptrs = _level_args(boxes, fabs)
+    ncomp = arrs[0].shape[0]
+    n = ncomp
+    for d in range(3):
+        n *= domain_hi[d] - domain_lo[d] + 1 + 2 * ghosts
+    out = np.empty(n)
+    I3 = ctypes.c_int * 3
+    lib().grhd_oracle_checkpoint_pack(cb, len(boxes), ptrs, ncomp,
+                                      I3(*domain_lo), I3(*domain_hi), ghosts,
+                                      _p(out))
+    return out
+
+
+def checkpoint_unpack(boxes, fabs, values, domain_lo, domain_hi, ghosts):
+    """Fill every storage cell of `fabs` (in place) from a payload."""
+    for f in fabs:
+        assert f.flags["C_CONTIGUOUS"] and f.dtype == np.float64
+    cb = (abi.GrhdBox * len(boxes))(*[b.to_c() for b in boxes])
+    ptrs = (ctypes.c_void_p * len(fabs))(*[f.ctypes.data for f in fabs])
+    I3 = ctypes.c_int * 3
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    lib().grhd_oracle_checkpoint_unpack(cb, len(boxes), ptrs,
+                                        fabs[0].shape[0], I3(*domain_lo),
+                                        I3(*domain_hi), ghosts, _p(values))

@@ -2262,3 +2262,92 @@ int grhd_oracle_leveldata_conserved_state_is_finite(
     }
     return ok;
 }
+
+/* checkpoint_value_index (Main_GRHDLevelDataBinaryPNKITPDisk.cpp:1348-1358)
+ * over the domain grown by `ghosts` (:1360-1365). */
+static long long ckpt_index(const int iv[3], int comp, const int dlo[3],
+                            const int dhi[3], int ghosts, int ncomp)
+{
+    const long long nx = dhi[0] - dlo[0] + 1 + 2 * ghosts;
+    const long long ny = dhi[1] - dlo[1] + 1 + 2 * ghosts;
+    const long long i = iv[0] - (dlo[0] - ghosts);
+    const long long j = iv[1] - (dlo[1] - ghosts);
+    const long long k = iv[2] - (dlo[2] - ghosts);
+    return ((k * ny + j) * nx + i) * ncomp + comp;
+}
+
+/* write_checkpoint (:1385-1425): zeroed payload; each box writes the cells
+ * of its FAB storage whose index clamped to the domain lies in its
+ * interior, in BoxIterator order. */
+void grhd_oracle_checkpoint_pack(const grhd_box *boxes, int nbox,
+                                 const double *const *fabs, int ncomp,
+                                 const int dlo[3], const int dhi[3],
+                                 int ghosts, double *values)
+{
+    long long count = ncomp;
+    for (int d = 0; d < 3; ++d)
+        count *= dhi[d] - dlo[d] + 1 + 2 * ghosts;
+    for (long long q = 0; q < count; ++q)
+        values[q] = 0.0;
+    for (int n = 0; n < nbox; ++n)
+    {
+        const grhd_box *b = &boxes[n];
+        const long long nx = b->ghi[0] - b->glo[0] + 1;
+        const long long ny = b->ghi[1] - b->glo[1] + 1;
+        const long long nz = b->ghi[2] - b->glo[2] + 1;
+        const long long npts = nx * ny * nz;
+        for (int k = b->glo[2]; k <= b->ghi[2]; ++k)
+            for (int j = b->glo[1]; j <= b->ghi[1]; ++j)
+                for (int i = b->glo[0]; i <= b->ghi[0]; ++i)
+                {
+                    const int iv[3] = {i, j, k};
+                    int owned = 1;
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        int o = iv[d];
+                        if (o < dlo[d])
+                            o = dlo[d];
+                        if (o > dhi[d])
+                            o = dhi[d];
+                        if (o < b->lo[d] || o > b->hi[d])
+                            owned = 0;
+                    }
+                    if (!owned)
+                        continue;
+                    const long long off = ((k - b->glo[2]) * ny +
+                                           (j - b->glo[1])) * nx +
+                                          (i - b->glo[0]);
+                    for (int c = 0; c < ncomp; ++c)
+                        values[ckpt_index(iv, c, dlo, dhi, ghosts, ncomp)] =
+                            fabs[n][c * npts + off];
+                }
+    }
+}
+
+/* read_checkpoint_state (:1566-1586): every storage cell of every box. */
+void grhd_oracle_checkpoint_unpack(const grhd_box *boxes, int nbox,
+                                   double *const *fabs, int ncomp,
+                                   const int dlo[3], const int dhi[3],
+                                   int ghosts, const double *values)
+{
+    for (int n = 0; n < nbox; ++n)
+    {
+        const grhd_box *b = &boxes[n];
+        const long long nx = b->ghi[0] - b->glo[0] + 1;
+        const long long ny = b->ghi[1] - b->glo[1] + 1;
+        const long long nz = b->ghi[2] - b->glo[2] + 1;
+        const long long npts = nx * ny * nz;
+        for (int k = b->glo[2]; k <= b->ghi[2]; ++k)
+            for (int j = b->glo[1]; j <= b->ghi[1]; ++j)
+                for (int i = b->glo[0]; i <= b->ghi[0]; ++i)
+                {
+                    const int iv[3] = {i, j, k};
+                    const long long off = ((k - b->glo[2]) * ny +
+                                           (j - b->glo[1])) * nx +
+                                          (i - b->glo[0]);
+                    for (int c = 0; c < ncomp; ++c)
+                        fabs[n][c * npts + off] =
+                            values[ckpt_index(iv, c, dlo, dhi, ghosts, ncomp)];
+                }
+    }
+}

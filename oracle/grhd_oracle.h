@@ -89,6 +89,16 @@ int grhd_oracle_leveldata_conserved_state_is_finite(
     const grhd_box *boxes, int nbox, const double *const *fabs, int ncomp,
     int comp_base);
 
+/* KITPCHK1 payload of a level (write_checkpoint / read_checkpoint_state). */
+void grhd_oracle_checkpoint_pack(const grhd_box *boxes, int nbox,
+                                 const double *const *fabs, int ncomp,
+                                 const int dlo[3], const int dhi[3],
+                                 int ghosts, double *values);
+void grhd_oracle_checkpoint_unpack(const grhd_box *boxes, int nbox,
+                                   double *const *fabs, int ncomp,
+                                   const int dlo[3], const int dhi[3],
+                                   int ghosts, const double *values);
+
 #ifdef __cplusplus
 }
 #endif
