@@ -133,6 +133,10 @@ SIGNATURES = {
                                                _P(GrhdBackground), _d, _vp,
                                                _i, _i, _P(_d), _P(GrhdError),
                                                _vp]),
+    "grhd_level_update_conserved": (_i, [_vp, _P(GrhdLevelBox), _i,
+                                         _P(GrhdParams), _P(GrhdBackground),
+                                         _vp, _i, _i, _vp, _d, _i, _d, _vp,
+                                         _P(GrhdDiagnostics), _vp]),
     "grhd_level_conserved_sum": (_i, [_vp, _P(GrhdLevelBox), _i, _vp, _i,
                                       _i, _P(_d), _vp]),
     "grhd_level_conserved_state_is_finite": (_i, [_vp, _P(GrhdLevelBox), _i,

@@ -22,6 +22,39 @@ LIBS = {
 }
 
 
+ROOT = os.path.dirname(HERE)
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+# C++ host programs over the C-ABI (include/grhd_operator.hpp): host code
+# only, built with g++ against the HIP runtime and libgrhd_hip.so.
+HOST_PROGRAMS = {
+    os.path.join("tests", "cpp", "_build", "level_driver"):
+        [os.path.join("tests", "cpp", "level_driver.cpp")],
+}
+
+
+def build_host_programs(verbose=False):
+    out = []
+    for exe, srcs in HOST_PROGRAMS.items():
+        path = os.path.join(ROOT, exe)
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        rel = os.path.relpath(HERE, os.path.dirname(path))
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+               "-I" + os.path.join(ROCM, "include"),
+               "-I" + os.path.join(ROOT, "include"), "-o", path + ".tmp"] + \
+            [os.path.join(ROOT, s) for s in srcs] + \
+            ["-L" + HERE, "-lgrhd_hip", "-L" + os.path.join(ROCM, "lib"),
+             "-lamdhip64", "-Wl,-rpath,$ORIGIN/" + rel,
+             "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("g++ failed for %s:\n%s" % (exe, res.stderr))
+        os.replace(path + ".tmp", path)
+        out.append(path)
+    return out
+
+
 def build(verbose=False):
     for lib, srcs in LIBS.items():
         out = os.path.join(HERE, lib)
@@ -33,7 +66,8 @@ def build(verbose=False):
         if res.returncode != 0:
             raise RuntimeError("hipcc failed for %s:\n%s" % (lib, res.stderr))
         os.replace(out + ".tmp", out)
-    return [os.path.join(HERE, lib) for lib in LIBS]
+    return [os.path.join(HERE, lib) for lib in LIBS] + \
+        build_host_programs(verbose)
 
 
 if __name__ == "__main__":

@@ -841,6 +841,47 @@ int level_checkpoint(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
     return GRHD_OK;
 }
 
+/* add_scaled_conserved_rhs over the level, then (optionally) the
+ * post-update recovery at `recovery_time`. */
+int level_update(grhd_ctx *ctx, const grhd_level_box *boxes, int nbox,
+                 const grhd_params *params, const grhd_background *background,
+                 double *state, int ncomp, int comp_base, const double *rhs,
+                 double dt, int recover, double recovery_time,
+                 unsigned char *mask, grhd_diagnostics *diagnostics,
+                 void *stream)
+{
+    if (!state || !rhs)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    int rc = check_common(params, background, ncomp, comp_base);
+    if (rc || (rc = prepare(ctx, boxes, nbox, ncomp)))
+        return rc;
+    hipStream_t s = (hipStream_t)stream;
+    Pools p = make_pools(ctx, state, comp_base);
+    p.rhs = const_cast<double *>(rhs);
+    p.rhs_ncomp = ncomp;
+    p.mask = mask;
+    launch_status_init(ctx, s);
+    GRHD_LAUNCH(ctx, C_INT, k_add_scaled, s, p, dt);
+    tick(ctx, GRHD_KERNEL_OTHER, s);
+    if (recover)
+    {
+        GRHD_LAUNCH(ctx, C_INT, k_recover, s, make_options(params),
+                    make_background(background, recovery_time), p,
+                    (int)(mask != nullptr));
+        tick(ctx, GRHD_KERNEL_OTHER, s);
+    }
+    grhd_diagnostics d;
+    rc = finish(ctx, s, nullptr, &d, false);
+    if (diagnostics)
+    {
+        if (recover)
+            *diagnostics = d;
+        else
+            std::memset(diagnostics, 0, sizeof(*diagnostics));
+    }
+    return rc;
+}
+
 int level_ssprk2(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
                  const grhd_params *params, const grhd_background *bg,
                  double time, double dt, double *state, int ncomp, int cb,
@@ -1033,37 +1074,12 @@ int grhd_update_conserved_box(grhd_ctx *ctx, const grhd_box *box,
                               double recovery_time, unsigned char *mask,
                               grhd_diagnostics *diagnostics, void *stream)
 {
-    if (!box || !state || !rhs)
+    if (!box)
         return GRHD_ERR_INVALID_ARGUMENT;
     const grhd_level_box lb = one_box(box);
-    int rc = check_common(params, background, ncomp, comp_base);
-    if (rc || (rc = prepare(ctx, &lb, 1, ncomp)))
-        return rc;
-    hipStream_t s = (hipStream_t)stream;
-    Pools p = make_pools(ctx, state, comp_base);
-    p.rhs = const_cast<double *>(rhs);
-    p.rhs_ncomp = ncomp;
-    p.mask = mask;
-    launch_status_init(ctx, s);
-    GRHD_LAUNCH(ctx, C_INT, k_add_scaled, s, p, dt);
-    tick(ctx, GRHD_KERNEL_OTHER, s);
-    if (recover)
-    {
-        GRHD_LAUNCH(ctx, C_INT, k_recover, s, make_options(params),
-                    make_background(background, recovery_time), p,
-                    (int)(mask != nullptr));
-        tick(ctx, GRHD_KERNEL_OTHER, s);
-    }
-    grhd_diagnostics d;
-    rc = finish(ctx, s, nullptr, &d, false);
-    if (diagnostics)
-    {
-        if (recover)
-            *diagnostics = d;
-        else
-            std::memset(diagnostics, 0, sizeof(*diagnostics));
-    }
-    return rc;
+    return level_update(ctx, &lb, 1, params, background, state, ncomp,
+                        comp_base, rhs, dt, recover, recovery_time, mask,
+                        diagnostics, stream);
 }
 
 int grhd_compute_max_inverse_dt_box(grhd_ctx *ctx, const grhd_box *box,
@@ -1227,6 +1243,19 @@ int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
         return rc;
     *is_finite = ctx->hstat->nonfinite == 0 ? 1 : 0;
     return GRHD_OK;
+}
+
+int grhd_level_update_conserved(grhd_ctx *ctx, const grhd_level_box *boxes,
+                                int nbox, const grhd_params *params,
+                                const grhd_background *background,
+                                double *state_pool, int ncomp, int comp_base,
+                                const double *rhs_pool, double dt, int recover,
+                                double recovery_time, unsigned char *mask,
+                                grhd_diagnostics *diagnostics, void *stream)
+{
+    return level_update(ctx, boxes, nbox, params, background, state_pool,
+                        ncomp, comp_base, rhs_pool, dt, recover,
+                        recovery_time, mask, diagnostics, stream);
 }
 
 int grhd_level_checkpoint_pack(grhd_ctx *ctx, const grhd_level_box *boxes,

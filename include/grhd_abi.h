@@ -316,6 +316,18 @@ int grhd_level_recover_primitives(grhd_ctx *ctx, const grhd_level_box *boxes,
                                   int comp_base, unsigned char *mask,
                                   grhd_diagnostics *diagnostics, void *stream);
 
+/* update_conserved over the level (FixedBGFArrayBoxFiniteVolume.hpp:522-532,
+ * :745-758): add_scaled_conserved_rhs (FArrayBoxFiniteVolume.hpp:927-940)
+ * of the rhs pool into every box, then, if `recover`, the post-update
+ * recovery at `recovery_time` with level-total diagnostics. */
+int grhd_level_update_conserved(grhd_ctx *ctx, const grhd_level_box *boxes,
+                                int nbox, const grhd_params *params,
+                                const grhd_background *background,
+                                double *state_pool, int ncomp, int comp_base,
+                                const double *rhs_pool, double dt, int recover,
+                                double recovery_time, unsigned char *mask,
+                                grhd_diagnostics *diagnostics, void *stream);
+
 /* compute_leveldata_max_inverse_dt_from_fixed_background
  * (FixedBGFArrayBoxFiniteVolume.hpp:414-444): maximum over the level. */
 int grhd_level_compute_max_inverse_dt(grhd_ctx *ctx,
