@@ -116,6 +116,152 @@ def calibrate_copy():
           flush=True)
 
 
+def bench_batched(args):
+    """The 3-level x 180-box circular-binary PN box set: every box of a
+    level advanced by one level call per RK4 run (box table, one launch per
+    kernel per level), against the plain path (one call per box) on the
+    same boxes.  Returns the JSON record."""
+    import torch
+
+    from grdzhadzha_amd import workloads
+    from grdzhadzha_amd.level import LevelOperator
+    from grdzhadzha_amd.operator import (
+        Context, TimeDependentFixedBGLevelDataFiniteVolumeOperator as TD)
+
+    bg, opts, levels, nsteps = workloads.binary_pn_levels()
+    steps = args.steps if args.steps_given else nsteps
+    ctx = Context()
+    ops = [LevelOperator(bg, boxes, opts, time_dependent=True, context=ctx)
+           for boxes, _, _ in levels]
+    init = [lev.pack(fabs) for lev, (_, fabs, _) in zip(ops, levels)]
+    cells = sum(lev.num_interior for lev in ops)
+    nbox = sum(len(b) for b, _, _ in levels)
+    print("bench: binary-PN box set, %d levels, %d boxes, %d interior cells"
+          % (len(levels), nbox, cells), file=sys.stderr, flush=True)
+
+    def run_batched(k):
+        pools = [p.clone() for p in init]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for lev, pool, (_, _, dt) in zip(ops, pools, levels):
+            lev.advance_rk4(pool, dt, k, time=0.0)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    run_batched(args.warmup or 1)
+    tb = run_batched(steps)
+    batched = cells * steps / tb
+
+    # plain path: one call per box, the same boxes and steps; each box has
+    # its own context, so its launch plan is built once, not per call
+    plain_steps = max(1, min(steps, args.plain_steps))
+    per_box = []
+    for lev, pool, (boxes, _, dt) in zip(ops, init, levels):
+        for n, (b, fab) in enumerate(zip(boxes, lev.unpack(pool))):
+            st = torch.from_numpy(fab.copy()).cuda()
+            per_box.append((TD(bg, b, opts, context=Context()), st, dt))
+    for op, st, dt in per_box:  # warm every per-box plan
+        op.advance_rk4(st.clone(), dt, 1, time=0.0)
+    sts = [st.clone() for _, st, _ in per_box]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (op, _, dt), st in zip(per_box, sts):
+        op.advance_rk4(st, dt, plain_steps, time=0.0)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    plain = cells * plain_steps / tp
+    print("bench: batched %.3f s for %d steps (%.3e cell-updates/s); plain "
+          "%.3f s for %d steps (%.3e)" % (tb, steps, batched, tp,
+                                           plain_steps, plain),
+          file=sys.stderr, flush=True)
+    return {"metric": METRIC, "value": batched, "unit": "cell-updates/s",
+            "n_gpus": 1, "steps": steps, "warmup": args.warmup or 1,
+            "ms_per_step": tb / steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "binary-PN box set: 3 levels x 180 "
+                       "seeded boxes (sides 8..64), circumbinary ring, "
+                       "batched FP64 RK4 per level", "boxes": nbox,
+                       "interior_cells": cells, "parallelism": "single GPU"},
+            "plain_per_box": {"value": plain, "steps": plain_steps,
+                              "ms_per_step": tp / plain_steps * 1e3},
+            "speedup_vs_plain": batched / plain}
+
+
+def bench_mixed(args):
+    """Mixed precision on the 256^3 torus (the 128^3 recipe scaled 2x in
+    radius: [-64, 64]^3, R = 24, sigma = 6, seed 4), 200 RK4 steps: FP32
+    reconstruction, face metric and HLLE against the FP64 path (which is
+    bit-exact with the oracle), both on this GPU from the same state.
+    Returns the JSON record: throughput of both, relative L1 / Linf error
+    per conserved variable and the accretion rate through |x| = 2 r_+."""
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from grdzhadzha_amd import workloads
+    from grdzhadzha_amd.operator import FixedBGLevelDataFiniteVolumeOperator
+
+    n = args.mixed_n
+    steps = args.mixed_steps
+    w = workloads.torus_box(n=n, seed=4, R0=12.0 * n / 128,
+                            sigma=3.0 * n / 128, nsteps=steps)
+    op64 = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box,
+                                                w.options)
+    o32 = dataclasses.replace(w.options, mixed_precision=True)
+    op32 = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, o32,
+                                                context=op64.ctx)
+    init = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
+    cells = w.box.num_interior
+    print("bench: mixed precision, %s, %d interior cells, %d steps"
+          % (w.name, cells, steps), file=sys.stderr, flush=True)
+
+    def run(op, k):
+        st = init.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.advance_rk4(st, w.dt, k)
+        torch.cuda.synchronize()
+        return st, time.perf_counter() - t0
+
+    run(op64, 1)
+    run(op32, 1)
+    s64, t64 = run(op64, steps)
+    s32, t32 = run(op32, steps)
+    a64 = s64.cpu().numpy()
+    a32 = s32.cpu().numpy()
+    del s64, s32
+    sl = (slice(None),) + w.box.interior_slices()
+    names = ("D", "S1", "S2", "S3", "tau")
+    err = {}
+    for c, name in enumerate(names):
+        ref = a64[c][sl[1:]]
+        diff = np.abs(a32[c][sl[1:]] - ref)
+        err[name] = {"l1_rel": float(diff.sum() / np.abs(ref).sum()),
+                     "linf_rel": float(diff.max() / np.abs(ref).max())}
+    spin = w.background.spin
+    r_plus = 1.0 + (1.0 - spin * spin) ** 0.5
+    md64 = workloads.mass_accretion_rate(w.box, a64, 1.0, spin, 2 * r_plus)
+    md32 = workloads.mass_accretion_rate(w.box, a32, 1.0, spin, 2 * r_plus)
+    err["mdot"] = {"fp64": md64, "mixed": md32,
+                   "rel": abs(md32 - md64) / max(abs(md64), 1e-300)}
+    v64, v32 = cells * steps / t64, cells * steps / t32
+    print("bench: FP64 %.3f s (%.3e cell-updates/s), mixed %.3f s (%.3e); "
+          "max L1 rel error %.3e" % (t64, v64, t32, v32,
+                                     max(e["l1_rel"] for k, e in err.items()
+                                         if k != "mdot")),
+          file=sys.stderr, flush=True)
+    return {"workload": "%s mixed FP32 faces / FP64 recovery+update RK4, "
+                        "%d steps" % (w.name, steps),
+            "interior_cells": cells, "steps": steps,
+            "value": v32, "fp64_value": v64, "speedup_vs_fp64": v32 / v64,
+            "ms_per_step": t32 / steps * 1e3,
+            "fp64_ms_per_step": t64 / steps * 1e3,
+            "error_vs_fp64": err,
+            "reference": "the FP64 GPU path, bit-exact with the oracle"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,10 +269,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["torus", "batched", "mixed"],
+                    default="torus",
+                    help="torus: the 128^3 plain FP64 box (default); "
+                         "batched: the 3-level binary-PN box set; mixed: "
+                         "the 256^3 torus in mixed precision")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision record of the default run")
+    ap.add_argument("--mixed-n", type=int, default=256)
+    ap.add_argument("--mixed-steps", type=int, default=200)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the binary-PN box-set record of the default "
+                         "run")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="steps of the one-call-per-box comparison run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 512 MiB device copy (x5), the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
+    args.steps_given = "--steps" in sys.argv
     if args.calibrate_copy:
         calibrate_copy()
         return
@@ -135,6 +296,13 @@ def main():
               "supported (the box would shard by z-slabs across GPUs, which "
               "is not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args)), flush=True)
+        return
+    if args.workload == "mixed":
+        print(json.dumps(bench_mixed(args)), flush=True)
+        return
 
     import numpy as np
     import torch
@@ -204,6 +372,21 @@ def main():
                       "interior_cells": cells, "ghosts": w.box.ghosts,
                       "dt": w.dt, "parallelism": "single GPU"},
            "roofline": roofline}
+    if not args.no_batched:
+        del op, st, init
+        torch.cuda.empty_cache()
+        args.steps_given = False
+        b = bench_batched(args)
+        out["batched"] = {"workload": b["config"]["workload"],
+                          "boxes": b["config"]["boxes"],
+                          "interior_cells": b["config"]["interior_cells"],
+                          "steps": b["steps"], "value": b["value"],
+                          "ms_per_step": b["ms_per_step"],
+                          "plain_per_box": b["plain_per_box"],
+                          "speedup_vs_plain": b["speedup_vs_plain"]}
+    if not args.no_mixed:
+        torch.cuda.empty_cache()
+        out["mixed"] = bench_mixed(args)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on 3 RK4 steps ...",
               file=sys.stderr, flush=True)

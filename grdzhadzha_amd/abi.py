@@ -54,7 +54,8 @@ class GrhdParams(ctypes.Structure):
                 ("use_reconstruction", ctypes.c_int),
                 ("use_static_metric_sources", ctypes.c_int),
                 ("use_metric_volume_time_sources", ctypes.c_int),
-                ("metric_time_derivative_step", ctypes.c_double)]
+                ("metric_time_derivative_step", ctypes.c_double),
+                ("mixed_precision", ctypes.c_int)]
 
 
 class GrhdBackground(ctypes.Structure):

@@ -43,6 +43,7 @@ Options make_options(const grhd_params *p)
     o.use_reconstruction = p->use_reconstruction;
     o.use_static_sources = p->use_static_metric_sources;
     o.use_volume_sources = p->use_metric_volume_time_sources;
+    o.mixed = p->mixed_precision != 0;
     return o;
 }
 
@@ -515,15 +516,66 @@ void launch_prims(grhd_ctx *c, const Options &o, const Background &bg,
     tick(c, GRHD_KERNEL_PRIMS, s);
 }
 
+template <class T>
+void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
+                    const Pools &p, hipStream_t s)
+{
+    GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
+    tick(c, GRHD_KERNEL_FACES_X, s);
+    GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), s, o, bg, p);
+    tick(c, GRHD_KERNEL_FACES_Y, s);
+    GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), s, o, bg, p);
+    tick(c, GRHD_KERNEL_FACES_Z, s);
+}
+
+/* FP64 faces, or FP32 faces when the options ask for mixed precision. */
 void launch_faces(grhd_ctx *c, const Options &o, const Background &bg,
                   const Pools &p, hipStream_t s)
 {
-    GRHD_LAUNCH(c, C_FX, k_faces_x, s, o, bg, p);
-    tick(c, GRHD_KERNEL_FACES_X, s);
-    GRHD_LAUNCH(c, C_FY, k_faces<1>, s, o, bg, p);
-    tick(c, GRHD_KERNEL_FACES_Y, s);
-    GRHD_LAUNCH(c, C_FZ, k_faces<2>, s, o, bg, p);
-    tick(c, GRHD_KERNEL_FACES_Z, s);
+    if (o.mixed)
+        launch_faces_t<float>(c, o, bg, p, s);
+    else
+        launch_faces_t<double>(c, o, bg, p, s);
+}
+
+template <class FT>
+void launch_cells_t(grhd_ctx *c, int mode, const Options &o,
+                    const Background &bs, const Background &bn,
+                    const Background &bp, const Background &bm,
+                    const Pools &p, const CellArgs &a, hipStream_t s)
+{
+    switch (mode)
+    {
+    case MODE_RHS:
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RHS, FT>), s, o, bs, bn, bp, bm,
+                    p, a);
+        break;
+    case MODE_RK4_1:
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_1, FT>), s, o, bs, bn, bp,
+                    bm, p, a);
+        break;
+    case MODE_RK4_2:
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_2, FT>), s, o, bs, bn, bp,
+                    bm, p, a);
+        break;
+    case MODE_RK4_3:
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_3, FT>), s, o, bs, bn, bp,
+                    bm, p, a);
+        break;
+    case MODE_RK4_4:
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_4, FT>), s, o, bs, bn, bp,
+                    bm, p, a);
+        break;
+    case MODE_SSP_1:
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_1, FT>), s, o, bs, bn, bp,
+                    bm, p, a);
+        break;
+    default:
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_2, FT>), s, o, bs, bn, bp,
+                    bm, p, a);
+        break;
+    }
+    tick(c, GRHD_KERNEL_CELLS, s);
 }
 
 void launch_cells(grhd_ctx *c, int mode, const Options &o,
@@ -531,37 +583,10 @@ void launch_cells(grhd_ctx *c, int mode, const Options &o,
                   const Background &bp, const Background &bm, const Pools &p,
                   const CellArgs &a, hipStream_t s)
 {
-    switch (mode)
-    {
-    case MODE_RHS:
-        GRHD_LAUNCH(c, C_INT, k_cells<MODE_RHS>, s, o, bs, bn, bp, bm, p, a);
-        break;
-    case MODE_RK4_1:
-        GRHD_LAUNCH(c, C_INT, k_cells<MODE_RK4_1>, s, o, bs, bn, bp, bm, p,
-                    a);
-        break;
-    case MODE_RK4_2:
-        GRHD_LAUNCH(c, C_INT, k_cells<MODE_RK4_2>, s, o, bs, bn, bp, bm, p,
-                    a);
-        break;
-    case MODE_RK4_3:
-        GRHD_LAUNCH(c, C_INT, k_cells<MODE_RK4_3>, s, o, bs, bn, bp, bm, p,
-                    a);
-        break;
-    case MODE_RK4_4:
-        GRHD_LAUNCH(c, C_INT, k_cells<MODE_RK4_4>, s, o, bs, bn, bp, bm, p,
-                    a);
-        break;
-    case MODE_SSP_1:
-        GRHD_LAUNCH(c, C_INT, k_cells<MODE_SSP_1>, s, o, bs, bn, bp, bm, p,
-                    a);
-        break;
-    default:
-        GRHD_LAUNCH(c, C_INT, k_cells<MODE_SSP_2>, s, o, bs, bn, bp, bm, p,
-                    a);
-        break;
-    }
-    tick(c, GRHD_KERNEL_CELLS, s);
+    if (o.mixed)
+        launch_cells_t<float>(c, mode, o, bs, bn, bp, bm, p, a, s);
+    else
+        launch_cells_t<double>(c, mode, o, bs, bn, bp, bm, p, a, s);
 }
 
 /* Volume-time source backgrounds at t +- max(step, 1e-12). */

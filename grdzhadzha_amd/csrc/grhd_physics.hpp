@@ -21,25 +21,33 @@
 namespace grhd
 {
 
-struct Prim
+/* The face-side arithmetic (metric for the face geometry, reconstruction,
+ * HLLE) is templated on its scalar type T: double for the bit-exact path,
+ * float for the mixed-precision faces.  Constants are written T(...), so
+ * the double instantiation is the same arithmetic as plain double code. */
+template <class T> struct PrimT
 {
-    double rho, eps, pressure, v[3];
+    T rho, eps, pressure, v[3];
 };
 
-struct Cons
+template <class T> struct ConsT
 {
-    double D, S[3], tau;
+    T D, S[3], tau;
 };
 
 /* CellGeometryData without K (StaticMetric.hpp:19-40); the flux and
  * recovery paths never read K_ij. */
-struct Geo
+template <class T> struct GeoT
 {
-    double g[3][3];
-    double gu[3][3];
-    double lapse;
-    double shift[3];
+    T g[3][3];
+    T gu[3][3];
+    T lapse;
+    T shift[3];
 };
+
+using Prim = PrimT<double>;
+using Cons = ConsT<double>;
+using Geo = GeoT<double>;
 
 /* Geometry plus everything the static-metric sources read
  * (StaticMetric.hpp:79-102, FixedBGMetric.hpp:17-55). */
@@ -62,6 +70,7 @@ struct Options
     double tolerance, rec_pressure_floor, rec_max_v2;
     double theta;
     int use_reconstruction, use_static_sources, use_volume_sources;
+    int mixed; /* FP32 reconstruction, face metric and HLLE */
 };
 
 /* A background evaluated at one time (background_factory(time)); the
@@ -74,32 +83,32 @@ struct Background
     double p1[3], p2[3], u1[3], u2[3];
 };
 
-__device__ __forceinline__ double std_max(double a, double b)
+template <class T> __device__ __forceinline__ T std_max(T a, T b)
 {
     return (a < b) ? b : a;
 }
-__device__ __forceinline__ double std_min(double a, double b)
+template <class T> __device__ __forceinline__ T std_min(T a, T b)
 {
     return (b < a) ? b : a;
 }
-__device__ __forceinline__ double kdelta(int i, int j)
+template <class T = double> __device__ __forceinline__ T kdelta(int i, int j)
 {
-    return (i == j) ? 1.0 : 0.0;
+    return (i == j) ? T(1) : T(0);
 }
 
 /* ---------------- GRChombo TensorAlgebra (restated) ---------------- */
-__device__ __forceinline__ double det_sym(const double m[3][3])
+template <class T> __device__ __forceinline__ T det_sym(const T m[3][3])
 {
     return m[0][0] * m[1][1] * m[2][2] + 2 * m[0][1] * m[0][2] * m[1][2] -
            m[0][0] * m[1][2] * m[1][2] - m[1][1] * m[0][2] * m[0][2] -
            m[2][2] * m[0][1] * m[0][1];
 }
 
-__device__ __forceinline__ void inverse_sym(const double m[3][3],
-                                            double out[3][3])
+template <class T>
+__device__ __forceinline__ void inverse_sym(const T m[3][3], T out[3][3])
 {
-    const double deth = det_sym(m);
-    const double inv = 1. / deth;
+    const T deth = det_sym(m);
+    const T inv = T(1) / deth;
     out[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[1][2]) * inv;
     out[0][1] = (m[0][2] * m[1][2] - m[0][1] * m[2][2]) * inv;
     out[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv;
@@ -160,29 +169,30 @@ __device__ __forceinline__ void coords(int i, int j, int k, double dx,
     x[2] = (k + 0.5) * dx - c[2];
 }
 
-__device__ __forceinline__ double get_radius(const double x[3])
+template <class T> __device__ __forceinline__ T get_radius(const T x[3])
 {
-    const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-    return std_max(r, 1e-6);
+    const T r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    return std_max(r, T(1e-6));
 }
 
 /* ---------------- Kerr-Schild (KerrSchild.hpp:65-166) ---------------- */
 
 /* Radius, H and l_i shared by the geometry-only and full evaluations. */
-struct KSCore
+template <class T> struct KSCoreT
 {
-    double rho, rho2, r2, r, cos_theta, H, el[3];
+    T rho, rho2, r2, r, cos_theta, H, el[3];
 };
+using KSCore = KSCoreT<double>;
 
-__device__ __forceinline__ void ks_core(double M, double a, const double x[3],
-                                        KSCore &c)
+template <class T>
+__device__ __forceinline__ void ks_core(T M, T a, const T x[3], KSCoreT<T> &c)
 {
-    const double a2 = a * a;
-    const double z = x[2];
+    const T a2 = a * a;
+    const T z = x[2];
     c.rho = get_radius(x);
     c.rho2 = c.rho * c.rho;
-    c.r2 = 0.5 * (c.rho2 - a2) +
-           sqrt(0.25 * (c.rho2 - a2) * (c.rho2 - a2) + a2 * z * z);
+    c.r2 = T(0.5) * (c.rho2 - a2) +
+           sqrt(T(0.25) * (c.rho2 - a2) * (c.rho2 - a2) + a2 * z * z);
     c.r = sqrt(c.r2);
     c.cos_theta = z / c.r;
     c.H = M * c.r / (c.r2 + a2 * c.cos_theta * c.cos_theta);
@@ -193,37 +203,59 @@ __device__ __forceinline__ void ks_core(double M, double a, const double x[3],
 
 /* gamma, gamma^-1, lapse, shift (KerrSchild.hpp:79-116 with
  * FixedBGMetric.hpp:23-24); `c` keeps r, H and l_i for ks_derivs. */
-__device__ __forceinline__ void ks_geometry_core(double M, double a,
-                                                 const double x[3], Geo &g,
-                                                 KSCore &c)
+template <class T>
+__device__ __forceinline__ void ks_geometry_core(T M, T a, const T x[3],
+                                                 GeoT<T> &g, KSCoreT<T> &c)
 {
     ks_core(M, a, x, c);
-    const double el_t = 1.0;
-    g.lapse = 1.0 / sqrt(1.0 + 2.0 * c.H * el_t * el_t);
+    const T el_t = T(1);
+    g.lapse = T(1) / sqrt(T(1) + T(2) * c.H * el_t * el_t);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            g.g[i][j] = kdelta(i, j) + 2.0 * c.H * c.el[i] * c.el[j];
-    /* KerrSchild.hpp:108 inverse; FixedBGMetric.hpp:23-24 inverse again:
-     * both are the same function of the same gamma. */
-    inverse_sym(g.g, g.gu);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
+            g.g[i][j] = kdelta<T>(i, j) + T(2) * c.H * c.el[i] * c.el[j];
+    if constexpr (sizeof(T) < sizeof(double))
     {
-        g.shift[i] = 0;
+        /* Reduced precision: the cofactor inverse of gamma sums terms of
+         * order one that cancel to order H; the Sherman-Morrison form of
+         * the inverse of delta + 2H l l^T has no cancellation:
+         * gamma^ij = delta^ij - q l^i l^j, beta^i = q l^i, with
+         * q = 2H / (1 + 2H l.l). */
+        const T ll = c.el[0] * c.el[0] + c.el[1] * c.el[1] + c.el[2] * c.el[2];
+        const T q = T(2) * c.H / (T(1) + T(2) * c.H * ll);
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            g.shift[i] += g.gu[i][j] * 2.0 * c.H * c.el[j] * el_t;
+        for (int i = 0; i < 3; ++i)
+        {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                g.gu[i][j] = kdelta<T>(i, j) - q * c.el[i] * c.el[j];
+            g.shift[i] = q * c.el[i] * el_t;
+        }
+    }
+    else
+    {
+        /* KerrSchild.hpp:108 inverse; FixedBGMetric.hpp:23-24 inverse
+         * again: both are the same function of the same gamma. */
+        inverse_sym(g.g, g.gu);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            g.shift[i] = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                g.shift[i] += g.gu[i][j] * T(2) * c.H * c.el[j] * el_t;
+        }
     }
 }
 
 /* gamma, gamma^-1, lapse, shift only (derivatives are not needed by the
  * flux/recovery paths and do not feed these values). */
-__device__ __forceinline__ void ks_geometry(double M, double a,
-                                            const double x[3], Geo &g)
+template <class T>
+__device__ __forceinline__ void ks_geometry(T M, T a, const T x[3],
+                                            GeoT<T> &g)
 {
-    KSCore c;
+    KSCoreT<T> c;
     ks_geometry_core(M, a, x, g, c);
 }
 
@@ -362,28 +394,29 @@ __device__ __forceinline__ void ks_full(double M, double a, const double xc[3],
 }
 
 /* ---------------- Circular binary PN (CircularBinaryPN.hpp) ---------------- */
-struct Potentials
+template <class T> struct PotentialsT
 {
-    double scalar, scalar_dt, dscalar[3], vec[3], dvec[3][3];
+    T scalar, scalar_dt, dscalar[3], vec[3], dvec[3][3];
 };
+using Potentials = PotentialsT<double>;
 
-__device__ __forceinline__ void pn_add_point(Potentials &p, double mass,
-                                             double soft, const double src[3],
-                                             const double vel[3],
-                                             const double pos[3])
+template <class T>
+__device__ __forceinline__ void pn_add_point(PotentialsT<T> &p, T mass, T soft,
+                                             const T src[3], const T vel[3],
+                                             const T pos[3])
 {
-    double disp[3];
-    double r2 = soft * soft;
+    T disp[3];
+    T r2 = soft * soft;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
         disp[i] = pos[i] - src[i];
         r2 += disp[i] * disp[i];
     }
-    const double ir = 1.0 / sqrt(r2);
-    const double ir3 = ir * ir * ir;
+    const T ir = T(1) / sqrt(r2);
+    const T ir3 = ir * ir * ir;
     p.scalar += mass * ir;
-    double ddv = 0.0;
+    T ddv = T(0);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         ddv += disp[i] * vel[i];
@@ -401,58 +434,65 @@ __device__ __forceinline__ void pn_add_point(Potentials &p, double mass,
         p.vec[c] += mass * vel[c] * ir;
 }
 
+template <class T>
 __device__ __forceinline__ void pn_potentials(const Background &bg,
-                                              const double x[3],
-                                              Potentials &p)
+                                              const T x[3], PotentialsT<T> &p)
 {
-    p.scalar = 0.0;
-    p.scalar_dt = 0.0;
+    p.scalar = T(0);
+    p.scalar_dt = T(0);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        p.dscalar[i] = 0.0;
-        p.vec[i] = 0.0;
+        p.dscalar[i] = T(0);
+        p.vec[i] = T(0);
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            p.dvec[i][j] = 0.0;
+            p.dvec[i][j] = T(0);
     }
-    pn_add_point(p, bg.m1, bg.soft, bg.p1, bg.u1, x);
-    pn_add_point(p, bg.m2, bg.soft, bg.p2, bg.u2, x);
+    const T p1[3] = {T(bg.p1[0]), T(bg.p1[1]), T(bg.p1[2])};
+    const T p2[3] = {T(bg.p2[0]), T(bg.p2[1]), T(bg.p2[2])};
+    const T u1[3] = {T(bg.u1[0]), T(bg.u1[1]), T(bg.u1[2])};
+    const T u2[3] = {T(bg.u2[0]), T(bg.u2[1]), T(bg.u2[2])};
+    pn_add_point(p, T(bg.m1), T(bg.soft), p1, u1, x);
+    pn_add_point(p, T(bg.m2), T(bg.soft), p2, u2, x);
 }
 
 /* Conformally flat geometry of the binary (CircularBinaryPN.hpp:80-130);
  * `p` keeps the potentials for pn_derivs. */
+template <class T>
 __device__ __forceinline__ void pn_geometry_pot(const Background &bg,
-                                                const double x[3], Geo &g,
-                                                Potentials &p)
+                                                const T x[3], GeoT<T> &g,
+                                                PotentialsT<T> &p)
 {
     pn_potentials(bg, x, p);
-    const double U = p.scalar;
-    const double cf = 1.0 + 2.0 * U;
-    double vsq = 0.0;
+    const T U = p.scalar;
+    const T cf = T(1) + T(2) * U;
+    T vsq = T(0);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         vsq += p.vec[i] * p.vec[i];
-    const double lapse_squared = 1.0 - 2.0 * U + 2.0 * U * U + 16.0 * vsq / cf;
+    const T lapse_squared =
+        T(1) - T(2) * U + T(2) * U * U + T(16) * vsq / cf;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            g.g[i][j] = 0.0;
+            g.g[i][j] = T(0);
     g.lapse = sqrt(lapse_squared);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
         g.g[i][i] = cf;
-        g.shift[i] = -4.0 * p.vec[i] / cf;
+        g.shift[i] = T(-4) * p.vec[i] / cf;
     }
     inverse_sym(g.g, g.gu);
 }
 
+template <class T>
 __device__ __forceinline__ void pn_geometry(const Background &bg,
-                                            const double x[3], Geo &g)
+                                            const T x[3], GeoT<T> &g)
 {
-    Potentials p;
+    PotentialsT<T> p;
     pn_geometry_pot(bg, x, g, p);
 }
 
@@ -584,13 +624,14 @@ __device__ __forceinline__ void cell_derivs(const Background &bg,
         ks_derivs(bg.M, bg.a, x, core.ks, g, f);
 }
 
+template <class T>
 __device__ __forceinline__ void cell_geometry(const Background &bg,
-                                              const double x[3], Geo &g)
+                                              const T x[3], GeoT<T> &g)
 {
     if (bg.kind == 1)
         pn_geometry(bg, x, g);
     else
-        ks_geometry(bg.M, bg.a, x, g);
+        ks_geometry(T(bg.M), T(bg.a), x, g);
 }
 
 __device__ __forceinline__ void cell_full(const Background &bg,
@@ -612,9 +653,10 @@ __device__ __forceinline__ double log_sqrt_det(const Geo &g)
 }
 
 /* ---------------- Valencia (Valencia.hpp) ---------------- */
-__device__ __forceinline__ double vsq(const double v[3], const double g[3][3])
+template <class T>
+__device__ __forceinline__ T vsq(const T v[3], const T g[3][3])
 {
-    double s = 0.0;
+    T s = T(0);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -624,10 +666,12 @@ __device__ __forceinline__ double vsq(const double v[3], const double g[3][3])
 }
 
 /* compute_conserved (Valencia.hpp:55-73) */
-__device__ __forceinline__ void prim2con(const Prim &p, double gamma,
-                                         const double g[3][3], Cons &c)
+template <class T>
+__device__ __forceinline__ void prim2con(const PrimT<T> &p, T gamma,
+                                         const T g[3][3], ConsT<T> &c)
 {
-    double vL[3];
+    (void)gamma;
+    T vL[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
@@ -636,14 +680,35 @@ __device__ __forceinline__ void prim2con(const Prim &p, double gamma,
         for (int j = 0; j < 3; ++j)
             vL[i] += g[i][j] * p.v[j];
     }
-    const double W = 1.0 / sqrt(1.0 - vsq(p.v, g));
-    const double h = 1.0 + p.eps + p.pressure / p.rho;
-    const double rhW2 = p.rho * h * W * W;
-    c.D = p.rho * W;
-    c.tau = rhW2 - p.pressure - c.D;
+    if constexpr (sizeof(T) < sizeof(double))
+    {
+        /* Reduced precision: tau = rho h W^2 - P - D cancels to
+         * rho (eps + O(v^2)) and would lose most of its float digits.  The
+         * same quantity without cancellation, from x = W^2 - 1 =
+         * v^2 / (1 - v^2):  tau = rho W x/(W + 1) + rho eps W^2 + P x. */
+        const T v2 = vsq(p.v, g);
+        const T x = v2 / (T(1) - v2);
+        const T W = T(1) / sqrt(T(1) - v2);
+        const T h = T(1) + p.eps + p.pressure / p.rho;
+        const T rhW2 = p.rho * h * W * W;
+        c.D = p.rho * W;
+        c.tau = p.rho * W * (x / (W + T(1))) + p.rho * p.eps * W * W +
+                p.pressure * x;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-        c.S[i] = rhW2 * vL[i];
+        for (int i = 0; i < 3; ++i)
+            c.S[i] = rhW2 * vL[i];
+    }
+    else
+    {
+        const T W = T(1) / sqrt(T(1) - vsq(p.v, g));
+        const T h = T(1) + p.eps + p.pressure / p.rho;
+        const T rhW2 = p.rho * h * W * W;
+        c.D = p.rho * W;
+        c.tau = rhW2 - p.pressure - c.D;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            c.S[i] = rhW2 * vL[i];
+    }
 }
 
 __device__ __forceinline__ double presid(double p, double D, double tau,
@@ -754,11 +819,12 @@ __device__ __forceinline__ bool con2prim(const Cons &c, const Options &o,
 }
 
 /* compute_flux (Valencia.hpp:230-249) */
-__device__ __forceinline__ void flux_of(const Prim &p, const Cons &c, int d,
-                                        double lapse, const double shift[3],
-                                        Cons &f)
+template <class T>
+__device__ __forceinline__ void flux_of(const PrimT<T> &p, const ConsT<T> &c,
+                                        int d, T lapse, const T shift[3],
+                                        ConsT<T> &f)
 {
-    const double tv = lapse * p.v[d] - shift[d];
+    const T tv = lapse * p.v[d] - shift[d];
     f.D = c.D * tv;
     f.tau = c.tau * tv + lapse * p.pressure * p.v[d];
 #pragma unroll
@@ -771,20 +837,21 @@ __device__ __forceinline__ void flux_of(const Prim &p, const Cons &c, int d,
 }
 
 /* ---------------- Atmosphere (Atmosphere.hpp:24-60) ---------------- */
-__device__ __forceinline__ void floors(Prim &p, const Options &o,
-                                       const double g[3][3])
+template <class T>
+__device__ __forceinline__ void floors(PrimT<T> &p, const Options &o,
+                                       const T g[3][3])
 {
-    p.rho = std_max(p.rho, o.rho_floor);
-    p.pressure = std_max(p.pressure, o.pressure_floor);
-    const double v2 = vsq(p.v, g);
-    if (v2 >= o.max_v2)
+    p.rho = std_max(p.rho, T(o.rho_floor));
+    p.pressure = std_max(p.pressure, T(o.pressure_floor));
+    const T v2 = vsq(p.v, g);
+    if (v2 >= T(o.max_v2))
     {
-        const double scale = sqrt(o.max_v2 / v2);
+        const T scale = sqrt(T(o.max_v2) / v2);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             p.v[i] *= scale;
     }
-    p.eps = p.pressure / ((o.gamma - 1.0) * p.rho);
+    p.eps = p.pressure / ((T(o.gamma) - T(1)) * p.rho);
 }
 
 __device__ __forceinline__ void atmosphere_prim(const Options &o, Prim &p)
@@ -812,68 +879,73 @@ __device__ __forceinline__ bool prim_differs(const Prim &a, const Prim &b)
 }
 
 /* ---------------- Reconstruction (Reconstruction.hpp:16-69) ---------------- */
-__device__ __forceinline__ double minmod2(double l, double r)
+template <class T> __device__ __forceinline__ T minmod2(T l, T r)
 {
-    if (l * r <= 0.0)
-        return 0.0;
-    return l > 0.0 ? std_min(l, r) : std_max(l, r);
+    if (l * r <= T(0))
+        return T(0);
+    return l > T(0) ? std_min(l, r) : std_max(l, r);
 }
 
-__device__ __forceinline__ double limited_slope(double l, double c, double r,
-                                                double theta)
+template <class T>
+__device__ __forceinline__ T limited_slope(T l, T c, T r, T theta)
 {
-    return minmod2(theta * (c - l), minmod2(0.5 * (r - l), theta * (r - c)));
+    return minmod2(theta * (c - l), minmod2(T(0.5) * (r - l), theta * (r - c)));
 }
 
 /* ---------------- Riemann solver (RiemannSolvers.hpp:23-118) ---------------- */
-__device__ __forceinline__ void char_speeds(const Prim &p, double gamma,
-                                            const Geo &g, int d,
-                                            double &lower, double &upper)
+template <class T>
+__device__ __forceinline__ void char_speeds(const PrimT<T> &p, T gamma,
+                                            const GeoT<T> &g, int d, T &lower,
+                                            T &upper)
 {
-    const double v2 = vsq(p.v, g.g);
-    const double h = 1.0 + p.eps + p.pressure / p.rho;
-    const double cs2 = gamma * p.pressure / (p.rho * h);
-    const double cs = sqrt(cs2);
-    const double vel = p.v[d];
-    const double den = 1.0 - v2 * cs2;
-    const double rad =
-        (1.0 - v2) * (g.gu[d][d] * (1.0 - v2 * cs2) - vel * vel * (1.0 - cs2));
-    const double root = cs * sqrt(std_max(0.0, rad));
-    const double lo = (vel * (1.0 - cs2) - root) / den;
-    const double hi = (vel * (1.0 - cs2) + root) / den;
+    const T v2 = vsq(p.v, g.g);
+    const T h = T(1) + p.eps + p.pressure / p.rho;
+    const T cs2 = gamma * p.pressure / (p.rho * h);
+    const T cs = sqrt(cs2);
+    const T vel = p.v[d];
+    const T den = T(1) - v2 * cs2;
+    const T rad = (T(1) - v2) *
+                  (g.gu[d][d] * (T(1) - v2 * cs2) - vel * vel * (T(1) - cs2));
+    const T root = cs * sqrt(std_max(T(0), rad));
+    const T lo = (vel * (T(1) - cs2) - root) / den;
+    const T hi = (vel * (T(1) - cs2) + root) / den;
     lower = g.lapse * lo - g.shift[d];
     upper = g.lapse * hi - g.shift[d];
 }
 
 /* compute_muscl_hlle_flux (FiniteVolume.hpp:63-98) after the MUSCL step:
  * floors, prim2con and HLLE on the face geometry. */
-__device__ __forceinline__ void hlle_face(Prim lp, Prim rp, const Options &o,
-                                          const Geo &face, int d, Cons &f)
+template <class T>
+__device__ __forceinline__ void hlle_face(PrimT<T> lp, PrimT<T> rp,
+                                          const Options &o,
+                                          const GeoT<T> &face, int d,
+                                          ConsT<T> &f)
 {
+    const T gamma = T(o.gamma);
     floors(lp, o, face.g);
     floors(rp, o, face.g);
-    Cons lc, rc;
-    prim2con(lp, o.gamma, face.g, lc);
-    prim2con(rp, o.gamma, face.g, rc);
-    Cons fl, fr;
+    ConsT<T> lc, rc;
+    prim2con(lp, gamma, face.g, lc);
+    prim2con(rp, gamma, face.g, rc);
+    ConsT<T> fl, fr;
     flux_of(lp, lc, d, face.lapse, face.shift, fl);
     flux_of(rp, rc, d, face.lapse, face.shift, fr);
-    double ll, lu, rl, ru;
-    char_speeds(lp, o.gamma, face, d, ll, lu);
-    char_speeds(rp, o.gamma, face, d, rl, ru);
-    const double sl = std_min(0.0, std_min(ll, rl));
-    const double su = std_max(0.0, std_max(lu, ru));
-    if (sl >= 0.0)
+    T ll, lu, rl, ru;
+    char_speeds(lp, gamma, face, d, ll, lu);
+    char_speeds(rp, gamma, face, d, rl, ru);
+    const T sl = std_min(T(0), std_min(ll, rl));
+    const T su = std_max(T(0), std_max(lu, ru));
+    if (sl >= T(0))
     {
         f = fl;
         return;
     }
-    if (su <= 0.0)
+    if (su <= T(0))
     {
         f = fr;
         return;
     }
-    const double inv = 1.0 / (su - sl);
+    const T inv = T(1) / (su - sl);
     f.D = (su * fl.D - sl * fr.D + sl * su * (rc.D - lc.D)) * inv;
     f.tau = (su * fl.tau - sl * fr.tau + sl * su * (rc.tau - lc.tau)) * inv;
 #pragma unroll
@@ -883,21 +955,22 @@ __device__ __forceinline__ void hlle_face(Prim lp, Prim rp, const Options &o,
 }
 
 /* average_geometry (FArrayBoxFiniteVolume.hpp:194-216), K omitted */
-__device__ __forceinline__ void average_geo(const Geo &l, const Geo &r,
-                                            Geo &out)
+template <class T>
+__device__ __forceinline__ void average_geo(const GeoT<T> &l, const GeoT<T> &r,
+                                            GeoT<T> &out)
 {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
         {
-            out.g[i][j] = 0.5 * (l.g[i][j] + r.g[i][j]);
-            out.gu[i][j] = 0.5 * (l.gu[i][j] + r.gu[i][j]);
+            out.g[i][j] = T(0.5) * (l.g[i][j] + r.g[i][j]);
+            out.gu[i][j] = T(0.5) * (l.gu[i][j] + r.gu[i][j]);
         }
-    out.lapse = 0.5 * (l.lapse + r.lapse);
+    out.lapse = T(0.5) * (l.lapse + r.lapse);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        out.shift[i] = 0.5 * (l.shift[i] + r.shift[i]);
+        out.shift[i] = T(0.5) * (l.shift[i] + r.shift[i]);
 }
 
 /* ---------------- Static-metric sources (StaticMetric.hpp:193-275) ------- */

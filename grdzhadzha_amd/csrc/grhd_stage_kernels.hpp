@@ -505,32 +505,72 @@ __global__ void __launch_bounds__(kBlock)
     wbad[idx] = 0;
 }
 
+/* The face kernels compute in T: double on the bit-exact path, float on
+ * the mixed-precision path (reconstruction, face metric and HLLE in FP32;
+ * W comes from the FP64 recovery and is rounded once on load; fluxes are
+ * stored as T and summed in FP64 by k_cells). */
+template <class T>
+__device__ __forceinline__ PrimT<T> load_prim_as(const double *P, long long n,
+                                                 long long o)
+{
+    PrimT<T> p;
+    p.rho = T(P[o]);
+    p.eps = T(P[n + o]);
+    p.pressure = T(P[2 * n + o]);
+    p.v[0] = T(P[3 * n + o]);
+    p.v[1] = T(P[4 * n + o]);
+    p.v[2] = T(P[5 * n + o]);
+    return p;
+}
+
+template <class T>
+__device__ __forceinline__ void coords_as(int i, int j, int k, const BoxDev &b,
+                                          T x[3])
+{
+    double xd[3];
+    coords(i, j, k, b.dx, b.center, xd);
+    x[0] = T(xd[0]);
+    x[1] = T(xd[1]);
+    x[2] = T(xd[2]);
+}
+
+/* Face-flux pool of direction d for box b, stored as T. */
+template <class T>
+__device__ __forceinline__ T *face_pool(const Pools &p, const BoxDev &b, int d)
+{
+    return reinterpret_cast<T *>(p.F[d]) + 5 * b.off_f[d];
+}
+
 /* add_scaled(c, slope, -1/2) and add_scaled(c, slope, +1/2)
  * (Reconstruction.hpp:59-69) of the MC-theta slope of c */
-__device__ __forceinline__ void slope_step(const Prim &l, const Prim &c,
-                                           const Prim &r, double theta,
-                                           Prim &minus_half, Prim &plus_half)
+template <class T>
+__device__ __forceinline__ void slope_step(const PrimT<T> &l,
+                                           const PrimT<T> &c,
+                                           const PrimT<T> &r, T theta,
+                                           PrimT<T> &minus_half,
+                                           PrimT<T> &plus_half)
 {
-    double s = limited_slope(l.rho, c.rho, r.rho, theta);
-    minus_half.rho = c.rho + -0.5 * s;
-    plus_half.rho = c.rho + 0.5 * s;
+    T s = limited_slope(l.rho, c.rho, r.rho, theta);
+    minus_half.rho = c.rho + T(-0.5) * s;
+    plus_half.rho = c.rho + T(0.5) * s;
     s = limited_slope(l.eps, c.eps, r.eps, theta);
-    minus_half.eps = c.eps + -0.5 * s;
-    plus_half.eps = c.eps + 0.5 * s;
+    minus_half.eps = c.eps + T(-0.5) * s;
+    plus_half.eps = c.eps + T(0.5) * s;
     s = limited_slope(l.pressure, c.pressure, r.pressure, theta);
-    minus_half.pressure = c.pressure + -0.5 * s;
-    plus_half.pressure = c.pressure + 0.5 * s;
+    minus_half.pressure = c.pressure + T(-0.5) * s;
+    plus_half.pressure = c.pressure + T(0.5) * s;
 #pragma unroll
     for (int m = 0; m < 3; ++m)
     {
         s = limited_slope(l.v[m], c.v[m], r.v[m], theta);
-        minus_half.v[m] = c.v[m] + -0.5 * s;
-        plus_half.v[m] = c.v[m] + 0.5 * s;
+        minus_half.v[m] = c.v[m] + T(-0.5) * s;
+        plus_half.v[m] = c.v[m] + T(0.5) * s;
     }
 }
 
-__device__ __forceinline__ void store_flux(double *F, long long nf,
-                                           long long fo, const Cons &f)
+template <class T>
+__device__ __forceinline__ void store_flux(T *F, long long nf, long long fo,
+                                           const ConsT<T> &f)
 {
     F[fo] = f.D;
     F[nf + fo] = f.S[0];
@@ -547,11 +587,12 @@ __device__ __forceinline__ void store_flux(double *F, long long nf,
 /* the face between a lane and the next is formed with __shfl_down,   */
 /* so loads are unit-stride along i and no metric is evaluated twice. */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ double shfl_next(double v)
+template <class T> __device__ __forceinline__ T shfl_next(T v)
 {
     return __shfl_down(v, 1, 64);
 }
 
+template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_faces_x(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
               Background bg, Pools p)
@@ -573,27 +614,27 @@ __global__ void __launch_bounds__(kBlock)
     const long long n = b.gnpts;
     const double *W = p.W + 6 * b.off_g;
     const long long oc = goff(b, i, j, k);
-    const Prim wc = load_prim(W, n, oc);
-    Prim mh = wc, ph = wc;
+    const PrimT<T> wc = load_prim_as<T>(W, n, oc);
+    PrimT<T> mh = wc, ph = wc;
     if (o.use_reconstruction)
     {
-        const Prim wl = load_prim(W, n, oc - 1);
-        const Prim wr = load_prim(W, n, oc + 1);
-        slope_step(wl, wc, wr, o.theta, mh, ph);
+        const PrimT<T> wl = load_prim_as<T>(W, n, oc - 1);
+        const PrimT<T> wr = load_prim_as<T>(W, n, oc + 1);
+        slope_step(wl, wc, wr, T(o.theta), mh, ph);
     }
-    double xc[3];
-    coords(i, j, k, b.dx, b.center, xc);
-    Geo gc;
+    T xc[3];
+    coords_as(i, j, k, b, xc);
+    GeoT<T> gc;
     cell_geometry(bg, xc, gc);
 
-    Prim rp;
+    PrimT<T> rp;
     rp.rho = shfl_next(mh.rho);
     rp.eps = shfl_next(mh.eps);
     rp.pressure = shfl_next(mh.pressure);
 #pragma unroll
     for (int m = 0; m < 3; ++m)
         rp.v[m] = shfl_next(mh.v[m]);
-    Geo gn;
+    GeoT<T> gn;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -609,15 +650,15 @@ __global__ void __launch_bounds__(kBlock)
 
     if (!live || lane == 63 || x == E - 1 || e + 1 >= next)
         return;
-    Geo face;
+    GeoT<T> face;
     average_geo(gc, gn, face);
-    Cons f;
+    ConsT<T> f;
     hlle_face(ph, rp, o, face, 0, f);
     const long long nf = b.nface[0];
     const long long fo =
         ((long long)(k - b.lo[2]) * b.in[1] + (j - b.lo[1])) * (b.in[0] + 1) +
         x;
-    store_flux(p.F[0] + 5 * b.off_f[0], nf, fo, f);
+    store_flux(face_pool<T>(p, b, 0), nf, fo, f);
 }
 
 /* ------------------------------------------------------------------ */
@@ -626,7 +667,7 @@ __global__ void __launch_bounds__(kBlock)
 /* state and the right cell's metric, so a cell's slope and metric are */
 /* computed once per walk; lanes are consecutive along i.              */
 /* ------------------------------------------------------------------ */
-template <int D>
+template <int D, class T>
 __global__ void __launch_bounds__(kBlock)
     k_faces(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o, Background bg,
             Pools p)
@@ -659,44 +700,46 @@ __global__ void __launch_bounds__(kBlock)
     long long fo = ((long long)(c[2] - b.lo[2]) * fn1 + (c[1] - b.lo[1])) *
                        fn0 + (c[0] - b.lo[0]);
     const long long nf = b.nface[D];
-    double *F = p.F[D] + 5 * b.off_f[D];
+    T *F = face_pool<T>(p, b, D);
+    const T theta = T(o.theta);
 
-    Prim wL = load_prim(W, n, oR - stride);
-    Prim wR = load_prim(W, n, oR);
-    Prim lp = wL;
+    PrimT<T> wL = load_prim_as<T>(W, n, oR - stride);
+    PrimT<T> wR = load_prim_as<T>(W, n, oR);
+    PrimT<T> lp = wL;
     if (o.use_reconstruction)
     {
-        const Prim wLL = load_prim(W, n, oR - 2 * stride);
-        Prim unused;
-        slope_step(wLL, wL, wR, o.theta, unused, lp);
+        const PrimT<T> wLL = load_prim_as<T>(W, n, oR - 2 * stride);
+        PrimT<T> unused;
+        slope_step(wLL, wL, wR, theta, unused, lp);
     }
-    double x[3];
+    T x[3];
     c[D] -= 1;
-    coords(c[0], c[1], c[2], b.dx, b.center, x);
-    Geo gL;
+    coords_as(c[0], c[1], c[2], b, x);
+    GeoT<T> gL;
     cell_geometry(bg, x, gL);
     for (int q = p0; q < p1; ++q)
     {
         c[D] += 1;
-        Prim rp = wR, lp_next = wR;
-        Prim wRR;
+        PrimT<T> rp = wR, lp_next = wR;
+        PrimT<T> wRR;
         if (o.use_reconstruction)
         {
-            wRR = load_prim(W, n, oR + stride);
-            slope_step(wL, wR, wRR, o.theta, rp, lp_next);
+            wRR = load_prim_as<T>(W, n, oR + stride);
+            slope_step(wL, wR, wRR, theta, rp, lp_next);
         }
-        coords(c[0], c[1], c[2], b.dx, b.center, x);
-        Geo gR, face;
+        coords_as(c[0], c[1], c[2], b, x);
+        GeoT<T> gR, face;
         cell_geometry(bg, x, gR);
         average_geo(gL, gR, face);
-        Cons f;
+        ConsT<T> f;
         hlle_face(lp, rp, o, face, D, f);
         store_flux(F, nf, fo, f);
         if (q + 1 < p1)
         {
             lp = lp_next;
             wL = wR;
-            wR = o.use_reconstruction ? wRR : load_prim(W, n, oR + stride);
+            wR = o.use_reconstruction ? wRR
+                                      : load_prim_as<T>(W, n, oR + stride);
             gL = gR;
             oR += stride;
             fo += f_stride;
@@ -733,7 +776,7 @@ struct CellArgs
 /* ------------------------------------------------------------------ */
 /* k_cells<MODE>: one thread per interior cell.                        */
 /* ------------------------------------------------------------------ */
-template <int MODE>
+template <int MODE, class FT>
 __global__ void __launch_bounds__(kBlock)
     k_cells(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
             Background bg_s, Background bg_next, Background bg_plus,
@@ -763,7 +806,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int d = 0; d < 3; ++d)
         {
-            const double *F = p.F[d] + 5 * b.off_f[d];
+            const FT *F = face_pool<const FT>(p, b, d);
             const long long nf = b.nface[d];
             const int fn0 = b.in[0] + (d == 0), fn1 = b.in[1] + (d == 1);
             const long long flo = ((long long)kk * fn1 + jj) * fn0 + ii;
@@ -771,11 +814,12 @@ __global__ void __launch_bounds__(kBlock)
                 flo + ((d == 0) ? 1
                                 : ((d == 1) ? (long long)fn0
                                             : (long long)fn0 * fn1));
-            rhs.D += mi * (F[fhi] - F[flo]);
-            rhs.tau += mi * (F[4 * nf + fhi] - F[4 * nf + flo]);
+            rhs.D += mi * ((double)F[fhi] - (double)F[flo]);
+            rhs.tau += mi * ((double)F[4 * nf + fhi] - (double)F[4 * nf + flo]);
 #pragma unroll
             for (int m = 0; m < 3; ++m)
-                rhs.S[m] += mi * (F[(1 + m) * nf + fhi] - F[(1 + m) * nf + flo]);
+                rhs.S[m] += mi * ((double)F[(1 + m) * nf + fhi] -
+                                  (double)F[(1 + m) * nf + flo]);
         }
 
         const Cons us = load_cons(pool_cons(p, a.us, b), n, so);

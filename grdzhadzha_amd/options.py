@@ -79,6 +79,9 @@ class OperatorOptions:
     use_static_metric_sources: bool = True
     use_metric_volume_time_sources: bool = False
     metric_time_derivative_step: float = 1.0e-3
+    # no counterpart in the reference: FP32 reconstruction, face metric and
+    # HLLE, everything else FP64 (False: the bit-exact FP64 path)
+    mixed_precision: bool = False
 
     def to_c(self):
         p = abi.GrhdParams()
@@ -98,6 +101,7 @@ class OperatorOptions:
         p.use_metric_volume_time_sources = int(
             self.use_metric_volume_time_sources)
         p.metric_time_derivative_step = self.metric_time_derivative_step
+        p.mixed_precision = int(self.mixed_precision)
         return p
 
 

@@ -317,3 +317,26 @@ def torus_state(box, M=1.0, spin=0.9, R0=12.0, sigma=3.0, seed=2):
     D, S, tau = prim_to_cons(rho, eps, p, v, g, gamma)
     state = pack_state(box, rho, eps, p, v, D, S, tau)
     return opts, state
+
+
+def mass_accretion_rate(box, state, M, spin, radius):
+    """Rest-mass accretion rate through the coordinate sphere |x| = radius:
+    minus the outward D flux sqrt(gamma) D (alpha v^i - beta^i) x_i/|x|
+    (the Valencia flux of D, Valencia.hpp:230-249) summed over the interior
+    cells with ||x| - radius| < dx/2, times dx^3, divided by the shell width
+    dx."""
+    X, Y, Z = cell_centres(box)
+    sl = box.interior_slices()
+    X, Y, Z = X[sl], Y[sl], Z[sl]
+    rr = np.sqrt(X * X + Y * Y + Z * Z)
+    sel = np.abs(rr - radius) < 0.5 * box.dx
+    x = np.stack([X[sel], Y[sel], Z[sel]])
+    g, lapse, shift = kerr_schild_metric(M, spin, x[0], x[1], x[2])
+    det = (g[0, 0] * (g[1, 1] * g[2, 2] - g[1, 2] * g[2, 1]) -
+           g[0, 1] * (g[1, 0] * g[2, 2] - g[1, 2] * g[2, 0]) +
+           g[0, 2] * (g[1, 0] * g[2, 1] - g[1, 1] * g[2, 0]))
+    D = state[0][sl][sel]
+    v = np.stack([state[8 + i][sl][sel] for i in range(3)])
+    flux = D * (lapse * v - shift)
+    radial = np.einsum("i...,i...->...", flux, x) / rr[sel]
+    return -float(np.sum(np.sqrt(det) * radial)) * box.dx ** 2

@@ -124,6 +124,11 @@ typedef struct grhd_params
     int use_static_metric_sources;
     int use_metric_volume_time_sources;
     double metric_time_derivative_step;
+    /* No counterpart in the reference: nonzero puts reconstruction, the
+     * face metric and HLLE in FP32 (face fluxes stored as float); con2prim,
+     * W, the flux-divergence sum, sources and the RK update stay FP64.
+     * Zero (the default) is the bit-exact FP64 path. */
+    int mixed_precision;
 } grhd_params;
 
 /* Analytic background.  For the binary, the orbital time at a call is

@@ -341,6 +341,10 @@ class LevelOperatorBase
     const grhd_background &background() const { return m_background; }
     grhd_ctx *context() const { return m_ctx->get(); }
 
+    /* FP32 reconstruction, face metric and HLLE (see grhd_params); off by
+     * default, which is the bit-exact FP64 path. */
+    void set_mixed_precision(bool on) { m_params.mixed_precision = on ? 1 : 0; }
+
   protected:
     std::vector<grhd_level_box> table(const DeviceLevelData &l,
                                       const ProblemDomain &domain) const
