@@ -217,3 +217,93 @@ def diagnostics(box, state, geo):
     v2 = np.einsum("ij...,i...,j...->...", G, v, v)
     return {"mass": mass, "max_rho": float(rho.max()),
             "max_pressure": float(p.max()), "max_v2": float(v2.max())}
+
+
+# ------------------------------------------------------------ convergence
+def solve_density_many(c, cfg, radii, anchors=64):
+    """Densities of the subsonic branch (radii > sonic radius) for many
+    radii at once: the reference's scalar solve on `anchors` radii spanning
+    the range gives brackets [0.8, 1.25] x the log-log interpolant, then a
+    vectorised bisection on bernoulli_residual to the same tolerance."""
+    radii = np.asarray(radii, dtype=float)
+    assert radii.min() > cfg.sonic_radius
+    ra = np.geomspace(radii.min() * 0.999, radii.max() * 1.001, anchors)
+    rho_a = np.array([solve_density(c, cfg, r) for r in ra])
+    guess = np.exp(np.interp(np.log(radii), np.log(ra), np.log(rho_a)))
+
+    def res(rho):
+        f = 1.0 - 2.0 * cfg.mass / radii
+        ur = c["mass_flux"] / (radii * radii * rho)
+        p = c["K"] * rho ** cfg.gamma
+        h = 1.0 + cfg.gamma * p / ((cfg.gamma - 1.0) * rho)
+        return h * h * (f + ur * ur) - c["bernoulli"] * c["bernoulli"]
+
+    lo, hi = guess * 0.8, guess * 1.25
+    flo, fhi = res(lo), res(hi)
+    assert np.all(flo * fhi < 0.0), "bracket lost the root"
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        fm = res(mid)
+        left = flo * fm <= 0.0
+        hi = np.where(left, mid, hi)
+        lo = np.where(left, lo, mid)
+        flo = np.where(left, flo, fm)
+        if np.all(hi - lo <= cfg.root_tolerance * np.abs(mid)):
+            break
+    return 0.5 * (lo + hi)
+
+
+def michel_box_state(cfg, box, opts):
+    """Exact Michel state (every storage cell, ghosts included) on a box of
+    Schwarzschild in Kerr-Schild coordinates, all cells outside the sonic
+    radius: rho from the Bernoulli constant, P = K rho^Gamma, u^r from the
+    mass flux, v^i = v^r x^i / r with v^r from the Kerr-Schild point
+    relations of make_point, conserved from prim2con on each cell's
+    metric."""
+    from grdzhadzha_amd import workloads
+    c = make_constants(cfg)
+    X, Y, Z = workloads.cell_centres(box)
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    rho = solve_density_many(c, cfg, r.ravel()).reshape(r.shape)
+    p = c["K"] * rho ** cfg.gamma
+    eps = p / ((cfg.gamma - 1.0) * rho)
+    ur = c["mass_flux"] / (r * r * rho)
+    f = 1.0 - 2.0 * cfg.mass / r
+    H = cfg.mass / r  # Kerr-Schild a = 0 on the radial axis
+    g_rr = 1.0 + 2.0 * H
+    lapse = 1.0 / np.sqrt(g_rr)
+    shift_r = 2.0 * H / g_rr
+    shift_L = g_rr * shift_r
+    g_tt = -lapse * lapse + g_rr * shift_r * shift_r
+    u_t = -np.sqrt(f + ur * ur)
+    W = lapse * (u_t - shift_L * ur) / g_tt
+    vr = ur / W + shift_r / lapse
+    v = np.stack([vr * X / r, vr * Y / r, vr * Z / r])
+    g, _, _ = workloads.kerr_schild_metric(cfg.mass, 0.0, X, Y, Z)
+    D, S, tau = workloads.prim_to_cons(rho, eps, p, v, g, cfg.gamma)
+    return workloads.pack_state(box, rho, eps, p, v, D, S, tau)
+
+
+def convergence_box(n, ghosts=2, lo_x=12.0, width=8.0):
+    """n^3 cells on [lo_x, lo_x + width] x [-width/2, width/2]^2 with a
+    ProblemDomain much larger than the box, so every ghost keeps the exact
+    state (no outflow fill)."""
+    dx = width / n
+    center = (-lo_x, 0.5 * width, 0.5 * width)
+    big = 1 << 20
+    return BoxSpec((0, 0, 0), (n - 1,) * 3, ghosts, dx, center,
+                   domain_lo=(-big,) * 3, domain_hi=(big,) * 3)
+
+
+def convergence_options(cfg, theta=2.0):
+    atm = AtmosphereOptions(rho_floor=1e-14, pressure_floor=1e-20,
+                            max_velocity_squared=0.95)
+    rec = RecoveryOptions(max_iterations=100, tolerance=1e-12,
+                          pressure_floor=1e-20, max_velocity_squared=0.99)
+    return OperatorOptions(adiabatic_index=cfg.gamma, atmosphere=atm,
+                           recovery=rec, limiter_theta=theta)
+
+
+def rhs_l1(box, rhs):
+    """Volume-weighted L1 norm of each conserved rhs component."""
+    return [float(np.abs(rhs[c]).sum()) * box.dx ** 3 for c in range(5)]
