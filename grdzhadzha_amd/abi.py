@@ -143,6 +143,10 @@ SIGNATURES = {
     "grhd_level_conserved_state_is_finite": (_i, [_vp, _P(GrhdLevelBox), _i,
                                                   _vp, _i, _i, _P(_i),
                                                   _vp]),
+    "grhd_rk4_adjoint_box": (_i, [_vp, _P(GrhdBox), _P(GrhdParams),
+                                  _P(GrhdBackground), _d, _i, _vp, _i, _i,
+                                  _d, _i, _i, _P(_d), _vp, _P(_d),
+                                  _P(GrhdError), _vp]),
     "grhd_level_checkpoint_pack": (_i, [_vp, _P(GrhdLevelBox), _i, _vp, _i,
                                         _P(_i), _P(_i), _i, _vp, _vp]),
     "grhd_level_checkpoint_unpack": (_i, [_vp, _P(GrhdLevelBox), _i, _vp,

@@ -1,0 +1,1027 @@
+/*
+ * grhd_adjoint.hpp -- device side of the discrete adjoint of the RK4 step.
+ *
+ * Every kernel of the forward step is a function of a few cells' data (a
+ * face flux of its 4-cell stencil, a cell's sources and recovery of that
+ * cell).  The adjoint needs the transposed local Jacobians of these
+ * functions applied to adjoint vectors.  They are formed column by column
+ * in forward mode: the same templated physics as the forward kernels is
+ * evaluated on Dual numbers (value + one tangent), once per local input,
+ * and the tangent of the outputs is dotted with the adjoint of the outputs.
+ * Branches (limiter choice, floors, HLLE wave-speed signs, atmosphere
+ * skips and resets, the v^2 clamp) compare values only, so they are frozen
+ * at the forward pass's decisions.  con2prim is differentiated at its
+ * converged root by the implicit-function theorem (con2prim_ift).
+ *
+ * Scope: one box with outflow (physical) ghosts on a Kerr-Schild
+ * background, FP64 faces, no volume-time sources.  The parameters that
+ * carry adjoints are the adiabatic index Gamma and the spin a.
+ */
+#ifndef GRHD_ADJOINT_HPP_
+#define GRHD_ADJOINT_HPP_
+
+#include "grhd_stage_kernels.hpp"
+
+namespace grhd
+{
+
+/* ------------------------------------------------------------------ */
+/* Forward-mode dual number (one tangent).                             */
+/* ------------------------------------------------------------------ */
+struct Dual
+{
+    double v, d;
+    Dual() = default;
+    __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
+    __host__ __device__ constexpr Dual(double x, double t) : v(x), d(t) {}
+};
+
+__device__ __forceinline__ Dual operator+(Dual a, Dual b)
+{
+    return Dual(a.v + b.v, a.d + b.d);
+}
+__device__ __forceinline__ Dual operator-(Dual a, Dual b)
+{
+    return Dual(a.v - b.v, a.d - b.d);
+}
+__device__ __forceinline__ Dual operator*(Dual a, Dual b)
+{
+    return Dual(a.v * b.v, a.d * b.v + a.v * b.d);
+}
+__device__ __forceinline__ Dual operator/(Dual a, Dual b)
+{
+    const double q = a.v / b.v;
+    return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator+(Dual a, double b)
+{
+    return Dual(a.v + b, a.d);
+}
+__device__ __forceinline__ Dual operator+(double a, Dual b)
+{
+    return Dual(a + b.v, b.d);
+}
+__device__ __forceinline__ Dual operator-(Dual a, double b)
+{
+    return Dual(a.v - b, a.d);
+}
+__device__ __forceinline__ Dual operator-(double a, Dual b)
+{
+    return Dual(a - b.v, -b.d);
+}
+__device__ __forceinline__ Dual operator*(Dual a, double b)
+{
+    return Dual(a.v * b, a.d * b);
+}
+__device__ __forceinline__ Dual operator*(double a, Dual b)
+{
+    return Dual(a * b.v, a * b.d);
+}
+__device__ __forceinline__ Dual operator/(Dual a, double b)
+{
+    return Dual(a.v / b, a.d / b);
+}
+__device__ __forceinline__ Dual operator/(double a, Dual b)
+{
+    const double q = a / b.v;
+    return Dual(q, -q * b.d / b.v);
+}
+__device__ __forceinline__ Dual &operator+=(Dual &a, Dual b)
+{
+    a = a + b;
+    return a;
+}
+__device__ __forceinline__ Dual &operator-=(Dual &a, Dual b)
+{
+    a = a - b;
+    return a;
+}
+__device__ __forceinline__ Dual &operator*=(Dual &a, Dual b)
+{
+    a = a * b;
+    return a;
+}
+#define GRHD_DUAL_CMP(op)                                                     \
+    __device__ __forceinline__ bool operator op(Dual a, Dual b)               \
+    {                                                                         \
+        return a.v op b.v;                                                    \
+    }                                                                         \
+    __device__ __forceinline__ bool operator op(Dual a, double b)             \
+    {                                                                         \
+        return a.v op b;                                                      \
+    }                                                                         \
+    __device__ __forceinline__ bool operator op(double a, Dual b)             \
+    {                                                                         \
+        return a op b.v;                                                      \
+    }
+GRHD_DUAL_CMP(<)
+GRHD_DUAL_CMP(<=)
+GRHD_DUAL_CMP(>)
+GRHD_DUAL_CMP(>=)
+GRHD_DUAL_CMP(==)
+GRHD_DUAL_CMP(!=)
+#undef GRHD_DUAL_CMP
+
+__device__ __forceinline__ Dual sqrt(Dual x)
+{
+    const double r = ::sqrt(x.v);
+    return Dual(r, r > 0.0 ? x.d / (2.0 * r) : 0.0);
+}
+__device__ __forceinline__ Dual fabs(Dual x) { return x.v < 0.0 ? -x : x; }
+/* the double overloads, which the Dual ones would otherwise hide here */
+__device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+__device__ __forceinline__ double fabs(double x) { return ::fabs(x); }
+
+__device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double val(Dual x) { return x.v; }
+
+/* ------------------------------------------------------------------ */
+/* con2prim differentiated at its root.                                */
+/* ------------------------------------------------------------------ */
+
+/* presid with every argument a Dual and the v^2 clamp frozen. */
+__device__ __forceinline__ Dual presid_dual(Dual p, Dual D, Dual tau, Dual S2,
+                                            Dual gamma, double max_v2)
+{
+    const Dual epp = tau + D + p;
+    Dual v2 = S2 / (epp * epp);
+    if (v2 > max_v2)
+        v2 = Dual(max_v2);
+    const Dual W = 1.0 / sqrt(1.0 - v2);
+    return D * W + gamma * p * W * W / (gamma - 1.0) - epp;
+}
+
+/* Primitive recovery with tangents.  The forward solve (double con2prim,
+ * the reference's algorithm) fixes the root p* and the branch.  There
+ * f(p*; U, gamma^ij, Gamma) = 0, so dp = -(df with p held) / (df/dp); the
+ * primitives are then formed from (U, p) exactly as con2prim forms them.
+ * A root taken at the recovery pressure floor by the early exit is a
+ * constant.  Returns false where the forward solve fails. */
+__device__ bool con2prim_ift(const ConsT<Dual> &c, const Options &o,
+                             Dual gamma, const Dual gu[3][3],
+                             PrimT<Dual> &out)
+{
+    Cons cv;
+    cv.D = c.D.v;
+    cv.tau = c.tau.v;
+    double guv[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+    {
+        cv.S[m] = c.S[m].v;
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+            guv[m][n] = gu[m][n].v;
+    }
+    Options ov = o;
+    ov.gamma = gamma.v;
+    Prim pv;
+    double res;
+    if (!con2prim(cv, ov, guv, pv, res))
+        return false;
+    const double ps = pv.pressure;
+    Dual S2 = 0.;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+            S2 += gu[m][n] * c.S[m] * c.S[n];
+    Dual p(ps, 0.0);
+    if (ps != o.rec_pressure_floor)
+    {
+        const Dual f_u =
+            presid_dual(Dual(ps, 0.0), c.D, c.tau, S2, gamma, o.rec_max_v2);
+        const Dual f_p = presid_dual(Dual(ps, 1.0), Dual(c.D.v), Dual(c.tau.v),
+                                     Dual(S2.v), Dual(gamma.v), o.rec_max_v2);
+        p = Dual(ps, -f_u.d / f_p.d);
+    }
+    const Dual epp = c.tau + c.D + p;
+    const Dual v2 = S2 / (epp * epp);
+    const Dual W = 1.0 / sqrt(1.0 - v2);
+    out.rho = c.D / W;
+    out.pressure = p;
+    out.eps = p / ((gamma - 1.0) * out.rho);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        Dual vi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            vi += gu[i][j] * c.S[j] / epp;
+        out.v[i] = vi;
+    }
+    return true;
+}
+
+/* ------------------------------------------------------------------ */
+/* Local functions, templated on double (values) / Dual (tangents).    */
+/* ------------------------------------------------------------------ */
+
+/* Parameters with their tangent slots: which local input is seeded. */
+struct AdjParams
+{
+    double M, a, gamma;
+};
+
+enum
+{
+    SEED_GAMMA = -1,
+    SEED_SPIN = -2,
+    SEED_NONE = -3
+};
+
+/* Face flux of direction D from the W of cells L-1, L, R, R+1 along D and
+ * the geometry of cells L and R (k_faces_x / k_faces<D>). */
+template <class T>
+__device__ __forceinline__ void face_flux_local(const PrimT<T> w[4],
+                                                const T xl[3], const T xr[3],
+                                                T M, T a, T gamma,
+                                                const Options &o, int D,
+                                                ConsT<T> &f)
+{
+    PrimT<T> lp = w[1], rp = w[2];
+    if (o.use_reconstruction)
+    {
+        PrimT<T> unused;
+        slope_step(w[0], w[1], w[2], T(o.theta), unused, lp);
+        slope_step(w[1], w[2], w[3], T(o.theta), rp, unused);
+    }
+    GeoT<T> gl, gr, face;
+    ks_geometry(M, a, xl, gl);
+    ks_geometry(M, a, xr, gr);
+    average_geo(gl, gr, face);
+    hlle_face_g(lp, rp, o, gamma, face, D, f);
+}
+
+/* Static-metric sources of an interior cell (k_cells): floors on W, the
+ * atmosphere skip, then the sources.  Returns false where skipped. */
+template <class T>
+__device__ __forceinline__ bool sources_local(const PrimT<T> &w,
+                                              const ConsT<T> &u, const T x[3],
+                                              T M, T a, T gamma,
+                                              const Options &o, ConsT<T> &src)
+{
+    KSCoreT<T> core;
+    GeoT<T> g;
+    ks_geometry_core(M, a, x, g, core);
+    PrimT<T> pw = w;
+    floors_g(pw, o, gamma, g.g);
+    Prim pv;
+    pv.rho = val(pw.rho);
+    pv.pressure = val(pw.pressure);
+    if (is_atmosphere(pv, o, o.source_atmosphere_factor))
+        return false;
+    FullGeoT<T> fg;
+    ks_derivs(M, a, x, core, g, fg);
+    static_sources(pw, u, gamma, fg, src);
+    return true;
+}
+
+/* ------------------------------------------------------------------ */
+/* Seeding helpers.                                                    */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ void seed_params(const AdjParams &pa, int seed,
+                                            Dual &M, Dual &a, Dual &gamma)
+{
+    M = Dual(pa.M);
+    a = Dual(pa.a, seed == SEED_SPIN ? 1.0 : 0.0);
+    gamma = Dual(pa.gamma, seed == SEED_GAMMA ? 1.0 : 0.0);
+}
+
+__device__ __forceinline__ PrimT<Dual> dual_prim(const Prim &p, int slot)
+{
+    PrimT<Dual> q;
+    q.rho = Dual(p.rho, slot == 0 ? 1.0 : 0.0);
+    q.eps = Dual(p.eps, slot == 1 ? 1.0 : 0.0);
+    q.pressure = Dual(p.pressure, slot == 2 ? 1.0 : 0.0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        q.v[m] = Dual(p.v[m], slot == 3 + m ? 1.0 : 0.0);
+    return q;
+}
+
+__device__ __forceinline__ ConsT<Dual> dual_cons(const Cons &c, int slot)
+{
+    ConsT<Dual> q;
+    q.D = Dual(c.D, slot == 0 ? 1.0 : 0.0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        q.S[m] = Dual(c.S[m], slot == 1 + m ? 1.0 : 0.0);
+    q.tau = Dual(c.tau, slot == 4 ? 1.0 : 0.0);
+    return q;
+}
+
+/* conserved component m of ConsT (D, S1, S2, S3, tau) */
+template <class T> __device__ __forceinline__ T cons_comp(const ConsT<T> &c, int m)
+{
+    return m == 0 ? c.D : (m == 4 ? c.tau : c.S[m - 1]);
+}
+
+__device__ __forceinline__ double dot_tangent(const ConsT<Dual> &f,
+                                              const double lam[5])
+{
+    return lam[0] * f.D.d + lam[1] * f.S[0].d + lam[2] * f.S[1].d +
+           lam[3] * f.S[2].d + lam[4] * f.tau.d;
+}
+
+/* prim component k (rho, eps, P, v1, v2, v3) of a Dual primitive's
+ * tangent */
+__device__ __forceinline__ double prim_tangent(const PrimT<Dual> &w, int k)
+{
+    switch (k)
+    {
+    case 0:
+        return w.rho.d;
+    case 1:
+        return w.eps.d;
+    case 2:
+        return w.pressure.d;
+    default:
+        return w.v[k - 3].d;
+    }
+}
+
+/* block sum of two doubles, one atomicAdd each by thread 0 */
+__device__ inline void block_add_params(double *out, double g0, double g1)
+{
+    __shared__ double part[2][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        g0 += __shfl_xor(g0, off, 64);
+        g1 += __shfl_xor(g1, off, 64);
+    }
+    if (lane == 0)
+    {
+        part[0][wave] = g0;
+        part[1][wave] = g1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w)
+        {
+            s0 += part[0][w];
+            s1 += part[1][w];
+        }
+        if (s0 != 0.0)
+            atomicAdd(&out[0], s0);
+        if (s1 != 0.0)
+            atomicAdd(&out[1], s1);
+    }
+}
+
+/* Interior arrays of the adjoint: 5 components over the interior box,
+ * component-major (the rhs FAB layout). */
+__device__ __forceinline__ long long iidx(const BoxDev &b, int i, int j, int k)
+{
+    return ((long long)(k - b.lo[2]) * b.in[1] + (j - b.lo[1])) * b.in[0] +
+           (i - b.lo[0]);
+}
+
+__device__ __forceinline__ bool interior(const BoxDev &b, int i, int j, int k)
+{
+    return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] &&
+           k >= b.lo[2] && k <= b.hi[2];
+}
+
+/* ------------------------------------------------------------------ */
+/* k_adj_faces<D>: lambda_F of each face of direction D from the rhs    */
+/* adjoint of its two cells, then J_F^T lambda_F into lambda_W of the   */
+/* four stencil cells (atomics) and into the parameter adjoints.        */
+/* One thread per face of the interior face box.                       */
+/* ------------------------------------------------------------------ */
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+    k_adj_faces(BoxDev b, Options o, AdjParams pa, const double *W,
+                const double *lam_k, double *lam_W, double *lam_par)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = b.in[0] + (D == 0), n1 = b.in[1] + (D == 1);
+    const int n2 = b.in[2] + (D == 2);
+    const long long nfaces = (long long)n0 * n1 * n2;
+    double g0 = 0.0, g1 = 0.0;
+    if (t < nfaces)
+    {
+        int c[3];
+        c[0] = b.lo[0] + (int)(t % n0);
+        c[1] = b.lo[1] + (int)((t / n0) % n1);
+        c[2] = b.lo[2] + (int)(t / ((long long)n0 * n1));
+        /* face between L = c - e_D and R = c */
+        int cl[3] = {c[0], c[1], c[2]};
+        cl[D] -= 1;
+        const long long ni = b.inpts;
+        double lam[5];
+        const bool Lin = interior(b, cl[0], cl[1], cl[2]);
+        const bool Rin = interior(b, c[0], c[1], c[2]);
+        const long long il = Lin ? iidx(b, cl[0], cl[1], cl[2]) : 0;
+        const long long ir = Rin ? iidx(b, c[0], c[1], c[2]) : 0;
+        bool any = false;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            const double lr = Rin ? lam_k[m * ni + ir] : 0.0;
+            const double ll = Lin ? lam_k[m * ni + il] : 0.0;
+            lam[m] = (lr - ll) * b.inv_dx;
+            any = any || lam[m] != 0.0;
+        }
+        if (any)
+        {
+            const long long n = b.gnpts;
+            long long so[4];
+            Prim wv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                int cc[3] = {c[0], c[1], c[2]};
+                cc[D] += q - 2;
+                so[q] = goff(b, cc[0], cc[1], cc[2]);
+                wv[q] = load_prim(W, n, so[q]);
+            }
+            double xl[3], xr[3];
+            coords(cl[0], cl[1], cl[2], b.dx, b.center, xl);
+            coords(c[0], c[1], c[2], b.dx, b.center, xr);
+            const Dual xld[3] = {Dual(xl[0]), Dual(xl[1]), Dual(xl[2])};
+            const Dual xrd[3] = {Dual(xr[0]), Dual(xr[1]), Dual(xr[2])};
+            /* 24 W inputs, then Gamma and a */
+            for (int s = 0; s < 26; ++s)
+            {
+                const int seed = s < 24 ? 0 : (s == 24 ? SEED_GAMMA : SEED_SPIN);
+                Dual M, a, gamma;
+                seed_params(pa, seed, M, a, gamma);
+                PrimT<Dual> wd[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    wd[q] = dual_prim(wv[q], (s < 24 && s / 6 == q) ? s % 6 : -1);
+                ConsT<Dual> f;
+                face_flux_local(wd, xld, xrd, M, a, gamma, o, D, f);
+                const double g = dot_tangent(f, lam);
+                if (s < 24)
+                {
+                    if (g != 0.0)
+                        atomicAdd(&lam_W[(long long)(s % 6) * n + so[s / 6]], g);
+                }
+                else if (s == 24)
+                    g0 += g;
+                else
+                    g1 += g;
+            }
+        }
+    }
+    block_add_params(lam_par, g0, g1);
+}
+
+/* ------------------------------------------------------------------ */
+/* k_adj_sources: sources of each interior cell: lambda_k -> lambda_W   */
+/* (6), lambda_U (5, added), parameter adjoints.                        */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(kBlock)
+    k_adj_sources(BoxDev b, Options o, AdjParams pa, const double *U,
+                  const double *W, const double *lam_k, double *lam_W,
+                  double *lam_U, double *lam_par)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double g0 = 0.0, g1 = 0.0;
+    if (it < b.inpts && o.use_static_sources)
+    {
+        const int i = b.lo[0] + (int)(it % b.in[0]);
+        const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+        const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+        const long long n = b.gnpts, ni = b.inpts;
+        const long long so = goff(b, i, j, k);
+        double lam[5];
+        bool any = false;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            lam[m] = lam_k[m * ni + it];
+            any = any || lam[m] != 0.0;
+        }
+        if (any)
+        {
+            const Prim wv = load_prim(W, n, so);
+            const Cons uv = load_cons(U, n, so);
+            double x[3];
+            coords(i, j, k, b.dx, b.center, x);
+            const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
+            double gW[6] = {0, 0, 0, 0, 0, 0};
+            /* 6 W inputs, 5 U inputs, Gamma, a */
+            for (int s = 0; s < 13; ++s)
+            {
+                const int seed = s < 11 ? 0 : (s == 11 ? SEED_GAMMA : SEED_SPIN);
+                Dual M, a, gamma;
+                seed_params(pa, seed, M, a, gamma);
+                const PrimT<Dual> wd = dual_prim(wv, s < 6 ? s : -1);
+                const ConsT<Dual> ud = dual_cons(uv, (s >= 6 && s < 11) ? s - 6 : -1);
+                ConsT<Dual> src;
+                if (!sources_local(wd, ud, xd, M, a, gamma, o, src))
+                    break; /* skipped cell: no dependence */
+                const double g = dot_tangent(src, lam);
+                if (s < 6)
+                    gW[s] = g;
+                else if (s < 11)
+                    lam_U[(long long)(s - 6) * ni + it] += g;
+                else if (s == 11)
+                    g0 += g;
+                else
+                    g1 += g;
+            }
+#pragma unroll
+            for (int m = 0; m < 6; ++m)
+                if (gW[m] != 0.0)
+                    atomicAdd(&lam_W[m * n + so], gW[m]);
+        }
+    }
+    block_add_params(lam_par, g0, g1);
+}
+
+/* The W of a storage cell as a function of the state (k_prims):
+ * interior, con2prim on the cell's metric; outflow ghost, the chain
+ * con2prim(source) -> floors(ghost metric) -> prim2con -> con2prim. */
+__device__ __forceinline__ bool w_of_state(const BoxDev &b, const Options &o,
+                                           Dual M, Dual a, Dual gamma,
+                                           int i, int j, int k,
+                                           const int src[3], bool physical,
+                                           const ConsT<Dual> &us,
+                                           PrimT<Dual> &w)
+{
+    double xt[3];
+    coords(i, j, k, b.dx, b.center, xt);
+    const Dual xtd[3] = {Dual(xt[0]), Dual(xt[1]), Dual(xt[2])};
+    GeoT<Dual> gt;
+    ks_geometry(M, a, xtd, gt);
+    if (!physical)
+        return con2prim_ift(us, o, gamma, gt.gu, w);
+    double xs[3];
+    coords(src[0], src[1], src[2], b.dx, b.center, xs);
+    const Dual xsd[3] = {Dual(xs[0]), Dual(xs[1]), Dual(xs[2])};
+    GeoT<Dual> gs;
+    ks_geometry(M, a, xsd, gs);
+    PrimT<Dual> pr;
+    if (!con2prim_ift(us, o, gamma, gs.gu, pr))
+        return false;
+    floors_g(pr, o, gamma, gt.g);
+    ConsT<Dual> cg;
+    prim2con(pr, gamma, gt.g, cg);
+    return con2prim_ift(cg, o, gamma, gt.gu, w);
+}
+
+/* ------------------------------------------------------------------ */
+/* k_adj_prims: lambda_W of every storage cell back to the interior     */
+/* state it was recovered from (interior: itself; outflow ghost: its    */
+/* clamped source, atomics), plus the parameter adjoints.               */
+/* ------------------------------------------------------------------ */
+__global__ void __launch_bounds__(kBlock)
+    k_adj_prims(BoxDev b, Options o, AdjParams pa, const double *U,
+                const double *lam_W, double *lam_U, double *lam_par)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double g0 = 0.0, g1 = 0.0;
+    if (it < b.gnpts)
+    {
+        const int i = b.glo[0] + (int)(it % b.gn[0]);
+        const int j = b.glo[1] + (int)((it / b.gn[0]) % b.gn[1]);
+        const int k = b.glo[2] + (int)(it / ((long long)b.gn[0] * b.gn[1]));
+        const long long n = b.gnpts;
+        double lw[6];
+        bool any = false;
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+        {
+            lw[m] = lam_W[m * n + it];
+            any = any || lw[m] != 0.0;
+        }
+        const int iv[3] = {i, j, k};
+        int src[3] = {i, j, k};
+        bool physical = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            if (iv[d] < b.lo[d])
+            {
+                physical = true;
+                src[d] = b.lo[d];
+            }
+            else if (iv[d] > b.hi[d])
+            {
+                physical = true;
+                src[d] = b.hi[d];
+            }
+        }
+        if (any)
+        {
+            const long long ss = goff(b, src[0], src[1], src[2]);
+            const Cons uv = load_cons(U, n, ss);
+            const long long is = iidx(b, src[0], src[1], src[2]);
+            for (int s = 0; s < 7; ++s)
+            {
+                const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
+                Dual M, a, gamma;
+                seed_params(pa, seed, M, a, gamma);
+                const ConsT<Dual> ud = dual_cons(uv, s < 5 ? s : -1);
+                PrimT<Dual> w;
+                if (!w_of_state(b, o, M, a, gamma, i, j, k, src, physical, ud, w))
+                    break;
+                double g = 0.0;
+#pragma unroll
+                for (int m = 0; m < 6; ++m)
+                    g += lw[m] * prim_tangent(w, m);
+                if (s < 5)
+                {
+                    if (g != 0.0)
+                        atomicAdd(&lam_U[(long long)s * b.inpts + is], g);
+                }
+                else if (s == 5)
+                    g0 += g;
+                else
+                    g1 += g;
+            }
+        }
+    }
+    block_add_params(lam_par, g0, g1);
+}
+
+/* ------------------------------------------------------------------ */
+/* Post-update recovery with atmosphere resets (k_cells' stage tail):   */
+/* U_pre -> U_post.                                                     */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ void recover_reset_v(const Cons &un,
+                                                const Options &o,
+                                                const Geo &g, Cons &out,
+                                                Prim &prim, bool &ok,
+                                                bool &reset)
+{
+    Prim rec;
+    double res;
+    ok = con2prim(un, o, g.gu, rec, res);
+    reset = !ok;
+    if (ok)
+    {
+        prim = rec;
+        floors(prim, o, g.g);
+        reset = prim_differs(rec, prim);
+    }
+    else
+    {
+        atmosphere_prim(o, prim);
+    }
+    out = un;
+    if (reset)
+        prim2con(prim, o.gamma, g.g, out);
+}
+
+/* Dual version with the branch frozen at the value path's decision. */
+__device__ __forceinline__ void recover_reset_d(const ConsT<Dual> &un,
+                                                const Options &o, Dual gamma,
+                                                const GeoT<Dual> &g,
+                                                bool ok, bool reset,
+                                                ConsT<Dual> &out)
+{
+    out = un;
+    if (!reset)
+        return;
+    PrimT<Dual> prim;
+    if (ok)
+    {
+        con2prim_ift(un, o, gamma, g.gu, prim);
+        floors_g(prim, o, gamma, g.g);
+    }
+    else
+    {
+        prim.rho = Dual(o.rho_floor);
+        prim.pressure = Dual(o.pressure_floor);
+        prim.eps = prim.pressure / ((gamma - 1.0) * prim.rho);
+        prim.v[0] = prim.v[1] = prim.v[2] = Dual(0.0);
+    }
+    prim2con(prim, gamma, g.g, out);
+}
+
+/* RK4 stage update of the adjoint's forward sweep: the same combination
+ * and order as k_cells<MODE_RK4_s> (U_s = U0 + c_stage k_s; the
+ * accumulator U0 + dt/6 k1 + dt/3 k2 + dt/3 k3; U_4 = acc + dt/6 k4), then
+ * the post-update recovery with resets.  Writes the stage state into the
+ * interior of `X` (5 components over the storage box). */
+__global__ void __launch_bounds__(kBlock)
+    k_fwd_stage(BoxDev b, Options o, AdjParams pa, int stage, double dt,
+                const double *U0, const double *k, double *acc, double *X)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= b.inpts)
+        return;
+    const int i = b.lo[0] + (int)(it % b.in[0]);
+    const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+    const int kk = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+    const long long n = b.gnpts, ni = b.inpts;
+    const long long so = goff(b, i, j, kk);
+    const double c2 = dt / 2.0, c3 = dt / 3.0, c6 = dt / 6.0;
+    const Cons u0 = load_cons(U0, n, so);
+    const Cons r = load_cons(k, ni, it);
+    Cons un;
+    if (stage < 4)
+    {
+        const double cst = stage == 3 ? dt : c2;
+        const double cac = stage == 1 ? c6 : c3;
+        Cons ac = stage == 1 ? u0 : load_cons(acc, ni, it);
+        ac.D += cac * r.D;
+        ac.tau += cac * r.tau;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            ac.S[m] += cac * r.S[m];
+        store_cons(acc, ni, it, ac);
+        un.D = u0.D + cst * r.D;
+        un.tau = u0.tau + cst * r.tau;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            un.S[m] = u0.S[m] + cst * r.S[m];
+    }
+    else
+    {
+        const Cons ac = load_cons(acc, ni, it);
+        un.D = ac.D + c6 * r.D;
+        un.tau = ac.tau + c6 * r.tau;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            un.S[m] = ac.S[m] + c6 * r.S[m];
+    }
+    double x[3];
+    coords(i, j, kk, b.dx, b.center, x);
+    Geo g;
+    ks_geometry(pa.M, pa.a, x, g);
+    Cons out;
+    Prim prim;
+    bool ok, reset;
+    recover_reset_v(un, o, g, out, prim, ok, reset);
+    store_cons(X, n, so, out);
+}
+
+/* k_adj_recover: lambda of the post-update state back through the
+ * recovery with resets to the pre-update state (lambda_pre = R'^T
+ * lambda_post), plus the parameter adjoints.  U_pre is rebuilt from the
+ * stored U0, k and accumulator exactly as k_fwd_stage formed it. */
+__global__ void __launch_bounds__(kBlock)
+    k_adj_recover(BoxDev b, Options o, AdjParams pa, int stage, double dt,
+                  const double *U0, const double *k, const double *acc_prev,
+                  const double *lam_post, double *lam_pre, double *lam_par)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double g0 = 0.0, g1 = 0.0;
+    if (it < b.inpts)
+    {
+        const int i = b.lo[0] + (int)(it % b.in[0]);
+        const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+        const int kk = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+        const long long n = b.gnpts, ni = b.inpts;
+        const long long so = goff(b, i, j, kk);
+        const double c2 = dt / 2.0, c6 = dt / 6.0;
+        const Cons r = load_cons(k, ni, it);
+        Cons un;
+        if (stage < 4)
+        {
+            const Cons u0 = load_cons(U0, n, so);
+            const double cst = stage == 3 ? dt : c2;
+            un.D = u0.D + cst * r.D;
+            un.tau = u0.tau + cst * r.tau;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                un.S[m] = u0.S[m] + cst * r.S[m];
+        }
+        else
+        {
+            const Cons ac = load_cons(acc_prev, ni, it);
+            un.D = ac.D + c6 * r.D;
+            un.tau = ac.tau + c6 * r.tau;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                un.S[m] = ac.S[m] + c6 * r.S[m];
+        }
+        double lam[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            lam[m] = lam_post[m * ni + it];
+        double x[3];
+        coords(i, j, kk, b.dx, b.center, x);
+        Geo g;
+        ks_geometry(pa.M, pa.a, x, g);
+        Cons out;
+        Prim prim;
+        bool ok, reset;
+        recover_reset_v(un, o, g, out, prim, ok, reset);
+        if (!reset)
+        {
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                lam_pre[m * ni + it] = lam[m];
+        }
+        else
+        {
+            const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
+            for (int s = 0; s < 7; ++s)
+            {
+                const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
+                Dual M, a, gamma;
+                seed_params(pa, seed, M, a, gamma);
+                GeoT<Dual> gd;
+                ks_geometry(M, a, xd, gd);
+                const ConsT<Dual> ud = dual_cons(un, s < 5 ? s : -1);
+                ConsT<Dual> od;
+                recover_reset_d(ud, o, gamma, gd, ok, reset, od);
+                const double gg = dot_tangent(od, lam);
+                if (s < 5)
+                    lam_pre[(long long)s * ni + it] = gg;
+                else if (s == 5)
+                    g0 += gg;
+                else
+                    g1 += gg;
+            }
+        }
+    }
+    block_add_params(lam_par, g0, g1);
+}
+
+/* Accretion rate through the coordinate sphere |x| = r0 (interior cells
+ * with ||x| - r0| < dx/2): -dx^2 sqrt(gamma) D (alpha v^i - beta^i)
+ * x_i / |x|, v from con2prim of the state. */
+template <class T>
+__device__ __forceinline__ T mdot_local(const ConsT<T> &u, const T x[3],
+                                        T M, T a, T gamma, const Options &o,
+                                        double dx, bool &ok)
+{
+    GeoT<T> g;
+    ks_geometry(M, a, x, g);
+    PrimT<T> w;
+    if constexpr (sizeof(T) == sizeof(double))
+    {
+        double res;
+        ok = con2prim(u, o, g.gu, w, res);
+    }
+    else
+    {
+        ok = con2prim_ift(u, o, gamma, g.gu, w);
+    }
+    if (!ok)
+        return T(0.0);
+    const T r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    T radial = T(0.0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        radial += (g.lapse * w.v[i] - g.shift[i]) * x[i];
+    const T sqrtg = sqrt(det_sym(g.g));
+    return -(dx * dx) * sqrtg * u.D * radial / r;
+}
+
+__device__ __forceinline__ bool in_shell(const double x[3], double r0,
+                                         double dx)
+{
+    const double r = ::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    return ::fabs(r - r0) < 0.5 * dx;
+}
+
+/* Per-cell accretion-rate terms (0 outside the shell) for a deterministic
+ * sum. */
+__global__ void __launch_bounds__(kBlock)
+    k_mdot_terms(BoxDev b, Options o, AdjParams pa, double r0,
+                 const double *U, double *terms)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= b.inpts)
+        return;
+    const int i = b.lo[0] + (int)(it % b.in[0]);
+    const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+    const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+    double x[3];
+    coords(i, j, k, b.dx, b.center, x);
+    double v = 0.0;
+    if (in_shell(x, r0, b.dx))
+    {
+        const Cons u = load_cons(U, b.gnpts, goff(b, i, j, k));
+        bool ok;
+        v = mdot_local(u, x, pa.M, pa.a, pa.gamma, o, b.dx, ok);
+    }
+    terms[it] = v;
+}
+
+/* lambda_U += w * d(mdot)/dU over the shell; parameter adjoints += w *
+ * d(mdot)/d(Gamma, a). */
+__global__ void __launch_bounds__(kBlock)
+    k_adj_mdot(BoxDev b, Options o, AdjParams pa, double r0, double w,
+               const double *U, double *lam_U, double *lam_par)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double g0 = 0.0, g1 = 0.0;
+    if (it < b.inpts)
+    {
+        const int i = b.lo[0] + (int)(it % b.in[0]);
+        const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+        const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+        double x[3];
+        coords(i, j, k, b.dx, b.center, x);
+        if (in_shell(x, r0, b.dx))
+        {
+            const Cons u = load_cons(U, b.gnpts, goff(b, i, j, k));
+            const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
+            for (int s = 0; s < 7; ++s)
+            {
+                const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
+                Dual M, a, gamma;
+                seed_params(pa, seed, M, a, gamma);
+                bool ok;
+                const Dual m = mdot_local(dual_cons(u, s < 5 ? s : -1), xd, M,
+                                          a, gamma, o, b.dx, ok);
+                if (!ok)
+                    break;
+                if (s < 5)
+                    lam_U[(long long)s * b.inpts + it] += w * m.d;
+                else if (s == 5)
+                    g0 += w * m.d;
+                else
+                    g1 += w * m.d;
+            }
+        }
+    }
+    block_add_params(lam_par, g0, g1);
+}
+
+/* y += a x over n entries */
+__global__ void __launch_bounds__(kBlock)
+    k_axpy(long long n, double a, const double *x, double *y)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        y[t] += a * x[t];
+}
+
+/* Interior 5-component array <-> the interior of a 5-component storage
+ * array (copy in either direction). */
+__global__ void __launch_bounds__(kBlock)
+    k_interior_copy(BoxDev b, const double *src, double *dst, int to_storage)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= b.inpts)
+        return;
+    const int i = b.lo[0] + (int)(it % b.in[0]);
+    const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+    const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+    const long long so = goff(b, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+    {
+        if (to_storage)
+            dst[m * b.gnpts + so] = src[m * b.inpts + it];
+        else
+            dst[m * b.inpts + it] = src[m * b.gnpts + so];
+    }
+}
+
+/* Deterministic double-double sum of n terms: per-block partials, then
+ * one block over the partials in order (k_diag_* use the same scheme). */
+__global__ void __launch_bounds__(kBlock)
+    k_sum_partial(const double *x, long long n, double *part)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    DD acc[5];
+    double plain[5];
+    double bad = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+    {
+        acc[m] = DD{0.0, 0.0};
+        plain[m] = 0.0;
+    }
+    if (t < n)
+    {
+        acc[0].hi = x[t];
+        plain[0] = x[t];
+    }
+    block_reduce_diag(acc, plain, bad);
+    if (threadIdx.x == 0)
+    {
+        part[2 * blockIdx.x] = acc[0].hi;
+        part[2 * blockIdx.x + 1] = acc[0].lo;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_sum_final(const double *part, long long nblocks, double *out)
+{
+    DD acc[5];
+    double plain[5];
+    double bad = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+    {
+        acc[m] = DD{0.0, 0.0};
+        plain[m] = 0.0;
+    }
+    for (long long q = threadIdx.x; q < nblocks; q += kBlock)
+        acc[0] = dd_add(acc[0], DD{part[2 * q], part[2 * q + 1]});
+    block_reduce_diag(acc, plain, bad);
+    if (threadIdx.x == 0)
+        *out = acc[0].hi;
+}
+
+} // namespace grhd
+
+#endif /* GRHD_ADJOINT_HPP_ */

@@ -16,6 +16,7 @@
 #include "../../include/grhd_abi.h"
 #include "grhd_physics.hpp"
 #include "grhd_stage_kernels.hpp"
+#include "grhd_adjoint.hpp"
 
 using namespace grhd;
 
@@ -1009,6 +1010,329 @@ int level_rk4(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
     return finish(c, s, err, diag, false);
 }
 
+/* ---------------- discrete adjoint of RK4 ---------------- */
+
+/* Device buffers of one adjoint call, freed on every return path. */
+struct DevBufs
+{
+    std::vector<void *> ptrs;
+    ~DevBufs()
+    {
+        for (void *q : ptrs)
+            (void)hipFree(q);
+    }
+    double *get(long long n)
+    {
+        void *q = nullptr;
+        if (n <= 0 || hipMalloc(&q, sizeof(double) * (size_t)n) != hipSuccess)
+            return nullptr;
+        ptrs.push_back(q);
+        return (double *)q;
+    }
+};
+
+struct AdjRun
+{
+    grhd_ctx *c;
+    BoxDev b;
+    Options o;
+    AdjParams pa;
+    Background bgd;
+    double dt;
+    hipStream_t s;
+    double *work; /* 11-component FAB (comp_base 0) the forward kernels use */
+    Pools p;
+};
+
+void fill0(double *x, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3(grid_of(n)), dim3(kBlock), 0, s, x, n,
+                       0.0);
+}
+
+/* k = L(X): X (5 components over the storage box) into the work FAB, ghost
+ * fill + recovery of every storage cell, faces, rhs into k (5 x interior).
+ * Leaves W of X in the context's scratch. */
+void adj_eval_rhs(AdjRun &r, const double *X, double *k)
+{
+    const BoxDev &b = r.b;
+    (void)hipMemcpyAsync(r.work, X, sizeof(double) * 5 * b.gnpts,
+                         hipMemcpyDeviceToDevice, r.s);
+    launch_prims(r.c, r.o, r.bgd, r.p, POOL_STATE, false, false, 0, r.s);
+    launch_faces(r.c, r.o, r.bgd, r.p, r.s);
+    CellArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.us = POOL_STATE;
+    Pools pp = r.p;
+    pp.rhs = k;
+    pp.rhs_ncomp = 5;
+    launch_cells(r.c, MODE_RHS, r.o, r.bgd, r.bgd, r.bgd, r.bgd, pp, a, r.s);
+}
+
+/* One RK4 step from X0 (storage, 5 components): stage states X[1..4]
+ * (X[4] = the new state), rhs k[0..3], accumulator after stage 3 in acc3
+ * (acc is scratch). */
+void adj_fwd_step(AdjRun &r, const double *X0, double *const X[5],
+                  double *const k[4], double *acc, double *acc3)
+{
+    const BoxDev &b = r.b;
+    const unsigned gi = grid_of(b.inpts);
+    for (int sg = 0; sg < 4; ++sg)
+    {
+        adj_eval_rhs(r, sg == 0 ? X0 : X[sg], k[sg]);
+        /* ghosts of the next stage state are refilled by its own eval */
+        if (X[sg + 1] != X0)
+            (void)hipMemcpyAsync(X[sg + 1], sg == 0 ? X0 : X[sg],
+                                 sizeof(double) * 5 * b.gnpts,
+                                 hipMemcpyDeviceToDevice, r.s);
+        hipLaunchKernelGGL(k_fwd_stage, dim3(gi), dim3(kBlock), 0, r.s, b, r.o,
+                           r.pa, sg + 1, r.dt, X0, k[sg], acc, X[sg + 1]);
+        if (sg == 2)
+            (void)hipMemcpyAsync(acc3, acc, sizeof(double) * 5 * b.inpts,
+                                 hipMemcpyDeviceToDevice, r.s);
+    }
+}
+
+/* lam_X = L'(X)^T lam_k (5 x interior), parameter adjoints into lam_par. */
+void adj_rhs_transpose(AdjRun &r, const double *X, const double *lam_k,
+                       double *lam_X, double *lam_W, double *lam_par)
+{
+    const BoxDev &b = r.b;
+    (void)hipMemcpyAsync(r.work, X, sizeof(double) * 5 * b.gnpts,
+                         hipMemcpyDeviceToDevice, r.s);
+    launch_prims(r.c, r.o, r.bgd, r.p, POOL_STATE, false, false, 0, r.s);
+    fill0(lam_W, 6 * b.gnpts, r.s);
+    fill0(lam_X, 5 * b.inpts, r.s);
+    const long long nf[3] = {(long long)(b.in[0] + 1) * b.in[1] * b.in[2],
+                             (long long)b.in[0] * (b.in[1] + 1) * b.in[2],
+                             (long long)b.in[0] * b.in[1] * (b.in[2] + 1)};
+    hipLaunchKernelGGL(k_adj_faces<0>, dim3(grid_of(nf[0])), dim3(kBlock), 0,
+                       r.s, b, r.o, r.pa, r.c->W, lam_k, lam_W, lam_par);
+    hipLaunchKernelGGL(k_adj_faces<1>, dim3(grid_of(nf[1])), dim3(kBlock), 0,
+                       r.s, b, r.o, r.pa, r.c->W, lam_k, lam_W, lam_par);
+    hipLaunchKernelGGL(k_adj_faces<2>, dim3(grid_of(nf[2])), dim3(kBlock), 0,
+                       r.s, b, r.o, r.pa, r.c->W, lam_k, lam_W, lam_par);
+    hipLaunchKernelGGL(k_adj_sources, dim3(grid_of(b.inpts)), dim3(kBlock), 0,
+                       r.s, b, r.o, r.pa, r.work, r.c->W, lam_k, lam_W, lam_X,
+                       lam_par);
+    hipLaunchKernelGGL(k_adj_prims, dim3(grid_of(b.gnpts)), dim3(kBlock), 0,
+                       r.s, b, r.o, r.pa, r.work, lam_W, lam_X, lam_par);
+}
+
+int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
+                    const grhd_params *params, const grhd_background *bg,
+                    double time, double dt, int nsteps, const double *state,
+                    int ncomp, int cb, double r0, int first_step,
+                    int ckpt_every, double *J, double *dJ_dU0,
+                    double *dJ_dparams, grhd_error *err, void *st)
+{
+    (void)time;
+    if (!box || !state || !J || !dJ_dU0 || !dJ_dparams || nsteps < 1 ||
+        first_step < 1 || first_step > nsteps || ckpt_every < 1 || dt <= 0.0)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    int rc = check_common(params, bg, ncomp, cb);
+    if (rc)
+        return rc;
+    if (bg->kind != GRHD_BG_KERR_SCHILD ||
+        params->use_metric_volume_time_sources || params->mixed_precision)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    for (int d = 0; d < 3; ++d)
+        if (box->periodic[d] || box->lo[d] != box->dlo[d] ||
+            box->hi[d] != box->dhi[d])
+            return GRHD_ERR_INVALID_ARGUMENT;
+    grhd_level_box lb;
+    std::memset(&lb, 0, sizeof(lb));
+    lb.box = *box;
+    if ((rc = prepare(c, &lb, 1, GRHD_NUM_COMPS)) ||
+        (rc = ensure_scratch(c, false)))
+        return rc;
+    AdjRun r;
+    r.c = c;
+    r.b = c->plan.hb[0];
+    r.o = make_options(params);
+    r.pa.M = bg->mass;
+    r.pa.a = bg->spin;
+    r.pa.gamma = params->adiabatic_index;
+    r.bgd = make_background(bg, 0.0);
+    r.dt = dt;
+    r.s = (hipStream_t)st;
+    const BoxDev &b = r.b;
+    const long long G = b.gnpts, I = b.inpts;
+    const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+
+    DevBufs mem;
+    r.work = mem.get(GRHD_NUM_COMPS * G);
+    double *ckpt = mem.get((long long)nseg * 5 * G);
+    /* per step of a segment: X[1..4] (5 x G) and k[0..3], acc3 (5 x I) */
+    const long long per_step = 4 * 5 * G + 5 * 5 * I;
+    double *seg = mem.get((long long)ckpt_every * per_step);
+    double *acc = mem.get(5 * I);
+    double *lam = mem.get(5 * I);
+    double *lam_k[4] = {mem.get(5 * I), mem.get(5 * I), mem.get(5 * I),
+                        mem.get(5 * I)};
+    double *lam_X = mem.get(5 * I);
+    double *lam_t = mem.get(5 * I);
+    double *lam_W = mem.get(6 * G);
+    double *lam_par = mem.get(2);
+    double *terms = mem.get(I);
+    const long long nb = (I + kBlock - 1) / kBlock;
+    double *part = mem.get(2 * nb);
+    double *mdot = mem.get(nsteps + 1);
+    double *X0 = mem.get(5 * G);
+    double *fx[5];
+    double *fk[4];
+    double *facc3 = mem.get(5 * I);
+    for (int q = 1; q < 5; ++q)
+        fx[q] = mem.get(5 * G);
+    for (int q = 0; q < 4; ++q)
+        fk[q] = mem.get(5 * I);
+    if (!r.work || !ckpt || !seg || !acc || !lam || !lam_k[3] || !lam_X ||
+        !lam_t || !lam_W || !lam_par || !terms || !part || !mdot || !X0 ||
+        !fx[4] || !fk[3] || !facc3)
+        return GRHD_ERR_DEVICE;
+    r.p = make_pools(c, r.work, 0);
+    hipStream_t s = r.s;
+    launch_status_init(c, s);
+    fill0(r.work, GRHD_NUM_COMPS * G, s);
+    fill0(mdot, nsteps + 1, s);
+    fill0(lam_par, 2, s);
+    fill0(lam, 5 * I, s);
+    (void)hipMemcpyAsync(X0, state + (long long)cb * G,
+                         sizeof(double) * 5 * G, hipMemcpyDeviceToDevice, s);
+
+    auto mdot_of = [&](const double *X, int n) {
+        hipLaunchKernelGGL(k_mdot_terms, dim3(grid_of(I)), dim3(kBlock), 0, s,
+                           b, r.o, r.pa, r0, X, terms);
+        hipLaunchKernelGGL(k_sum_partial, dim3((unsigned)nb), dim3(kBlock), 0,
+                           s, terms, I, part);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, s, part, nb,
+                           mdot + n);
+    };
+
+    /* forward sweep: checkpoints and J */
+    fx[0] = X0;
+    for (int n = 0; n < nsteps; ++n)
+    {
+        if (n % ckpt_every == 0)
+            (void)hipMemcpyAsync(ckpt + (long long)(n / ckpt_every) * 5 * G,
+                                 X0, sizeof(double) * 5 * G,
+                                 hipMemcpyDeviceToDevice, s);
+        adj_fwd_step(r, X0, fx, fk, acc, facc3);
+        (void)hipMemcpyAsync(X0, fx[4], sizeof(double) * 5 * G,
+                             hipMemcpyDeviceToDevice, s);
+        if (n + 1 >= first_step)
+            mdot_of(X0, n + 1);
+    }
+    if ((rc = finish(c, s, err, nullptr, false)))
+        return rc;
+    std::vector<double> hm(nsteps + 1);
+    if (hipMemcpy(hm.data(), mdot, sizeof(double) * (nsteps + 1),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return GRHD_ERR_DEVICE;
+    double Jv = 0.0;
+    for (int n = first_step; n <= nsteps; ++n)
+        Jv += dt * hm[n];
+    *J = Jv;
+
+    /* reverse sweep, one checkpoint segment at a time */
+    const unsigned gi = grid_of(I);
+    for (int sgm = nseg - 1; sgm >= 0; --sgm)
+    {
+        const int n0 = sgm * ckpt_every;
+        const int n1 = std::min(nsteps, n0 + ckpt_every);
+        /* recompute the segment, keeping each step's stage data */
+        (void)hipMemcpyAsync(X0, ckpt + (long long)sgm * 5 * G,
+                             sizeof(double) * 5 * G, hipMemcpyDeviceToDevice,
+                             s);
+        for (int n = n0; n < n1; ++n)
+        {
+            double *base = seg + (long long)(n - n0) * per_step;
+            double *X[5];
+            double *K[4];
+            X[0] = n == n0 ? ckpt + (long long)sgm * 5 * G
+                           : seg + (long long)(n - n0 - 1) * per_step +
+                                 3 * 5 * G;
+            for (int q = 1; q < 5; ++q)
+                X[q] = base + (q - 1) * 5 * G;
+            for (int q = 0; q < 4; ++q)
+                K[q] = base + 4 * 5 * G + q * 5 * I;
+            double *a3 = base + 4 * 5 * G + 4 * 5 * I;
+            adj_fwd_step(r, X[0], X, K, acc, a3);
+        }
+        for (int n = n1 - 1; n >= n0; --n)
+        {
+            double *base = seg + (long long)(n - n0) * per_step;
+            const double *X[5];
+            const double *K[4];
+            X[0] = n == n0 ? ckpt + (long long)sgm * 5 * G
+                           : seg + (long long)(n - n0 - 1) * per_step +
+                                 3 * 5 * G;
+            for (int q = 1; q < 5; ++q)
+                X[q] = base + (q - 1) * 5 * G;
+            for (int q = 0; q < 4; ++q)
+                K[q] = base + 4 * 5 * G + q * 5 * I;
+            const double *a3 = base + 4 * 5 * G + 4 * 5 * I;
+            /* the objective's seed at U_{n+1} */
+            if (n + 1 >= first_step)
+                hipLaunchKernelGGL(k_adj_mdot, dim3(gi), dim3(kBlock), 0, s,
+                                   b, r.o, r.pa, r0, dt, X[4], lam, lam_par);
+            const double c2 = dt / 2.0, c3 = dt / 3.0, c6 = dt / 6.0;
+            /* U_4 = R(acc3 + dt/6 k4) */
+            hipLaunchKernelGGL(k_adj_recover, dim3(gi), dim3(kBlock), 0, s, b,
+                               r.o, r.pa, 4, dt, X[0], K[3], a3, lam, lam_t,
+                               lam_par);
+            /* lam_U0 accumulates in lam_X's partner: keep it in `acc` */
+            double *lam_u0 = acc;
+            (void)hipMemcpyAsync(lam_u0, lam_t, sizeof(double) * 5 * I,
+                                 hipMemcpyDeviceToDevice, s);
+            fill0(lam_k[0], 5 * I, s);
+            fill0(lam_k[1], 5 * I, s);
+            fill0(lam_k[2], 5 * I, s);
+            fill0(lam_k[3], 5 * I, s);
+            hipLaunchKernelGGL(k_axpy, dim3(grid_of(5 * I)), dim3(kBlock), 0,
+                               s, 5 * I, c6, lam_t, lam_k[3]);
+            hipLaunchKernelGGL(k_axpy, dim3(grid_of(5 * I)), dim3(kBlock), 0,
+                               s, 5 * I, c6, lam_t, lam_k[0]);
+            hipLaunchKernelGGL(k_axpy, dim3(grid_of(5 * I)), dim3(kBlock), 0,
+                               s, 5 * I, c3, lam_t, lam_k[1]);
+            hipLaunchKernelGGL(k_axpy, dim3(grid_of(5 * I)), dim3(kBlock), 0,
+                               s, 5 * I, c3, lam_t, lam_k[2]);
+            /* stages 3, 2, 1: lam_{X_s} = L'(X_s)^T lam_k[s];
+             * X_s = R(U0 + c_s k_s) */
+            const double cst[3] = {c2, c2, dt};
+            for (int sg = 3; sg >= 1; --sg)
+            {
+                adj_rhs_transpose(r, X[sg], lam_k[sg], lam_X, lam_W, lam_par);
+                hipLaunchKernelGGL(k_adj_recover, dim3(gi), dim3(kBlock), 0, s,
+                                   b, r.o, r.pa, sg, dt, X[0], K[sg - 1],
+                                   a3, lam_X, lam_t, lam_par);
+                hipLaunchKernelGGL(k_axpy, dim3(grid_of(5 * I)), dim3(kBlock),
+                                   0, s, 5 * I, 1.0, lam_t, lam_u0);
+                hipLaunchKernelGGL(k_axpy, dim3(grid_of(5 * I)), dim3(kBlock),
+                                   0, s, 5 * I, cst[sg - 1], lam_t,
+                                   lam_k[sg - 1]);
+            }
+            /* k1 = L(U0) */
+            adj_rhs_transpose(r, X[0], lam_k[0], lam_X, lam_W, lam_par);
+            hipLaunchKernelGGL(k_axpy, dim3(grid_of(5 * I)), dim3(kBlock), 0,
+                               s, 5 * I, 1.0, lam_X, lam_u0);
+            (void)hipMemcpyAsync(lam, lam_u0, sizeof(double) * 5 * I,
+                                 hipMemcpyDeviceToDevice, s);
+        }
+    }
+    (void)hipMemcpyAsync(dJ_dU0, lam, sizeof(double) * 5 * I,
+                         hipMemcpyDeviceToDevice, s);
+    double hp[2];
+    if (hipMemcpyAsync(hp, lam_par, sizeof(hp), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+        return GRHD_ERR_DEVICE;
+    dJ_dparams[0] = hp[0];
+    dJ_dparams[1] = hp[1];
+    return GRHD_OK;
+}
+
 inline grhd_level_box one_box(const grhd_box *b)
 {
     grhd_level_box lb;
@@ -1281,6 +1605,21 @@ int grhd_level_update_conserved(grhd_ctx *ctx, const grhd_level_box *boxes,
     return level_update(ctx, boxes, nbox, params, background, state_pool,
                         ncomp, comp_base, rhs_pool, dt, recover,
                         recovery_time, mask, diagnostics, stream);
+}
+
+int grhd_rk4_adjoint_box(grhd_ctx *ctx, const grhd_box *box,
+                         const grhd_params *params,
+                         const grhd_background *background, double dt,
+                         int nsteps, const double *state, int ncomp,
+                         int comp_base, double shell_radius, int first_step,
+                         int checkpoint_every, double *objective,
+                         double *dJ_dconserved, double *dJ_dparams,
+                         grhd_error *err, void *stream)
+{
+    return rk4_adjoint_box(ctx, box, params, background, 0.0, dt, nsteps,
+                           state, ncomp, comp_base, shell_radius, first_step,
+                           checkpoint_every, objective, dJ_dconserved,
+                           dJ_dparams, err, stream);
 }
 
 int grhd_level_checkpoint_pack(grhd_ctx *ctx, const grhd_level_box *boxes,

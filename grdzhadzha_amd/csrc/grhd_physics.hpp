@@ -51,15 +51,16 @@ using Geo = GeoT<double>;
 
 /* Geometry plus everything the static-metric sources read
  * (StaticMetric.hpp:79-102, FixedBGMetric.hpp:17-55). */
-struct FullGeo
+template <class T> struct FullGeoT
 {
-    Geo geo;
-    double K[3][3];
-    double dlapse[3];
-    double dlogsqrtdet[3];
-    double dg[3][3][3];    /* [dir][i][j] */
-    double dshift[3][3];   /* [i][dir]    */
+    GeoT<T> geo;
+    T K[3][3];
+    T dlapse[3];
+    T dlogsqrtdet[3];
+    T dg[3][3][3];    /* [dir][i][j] */
+    T dshift[3][3];   /* [i][dir]    */
 };
+using FullGeo = FullGeoT<double>;
 
 /* Operator options passed by value to kernels. */
 struct Options
@@ -262,25 +263,25 @@ __device__ __forceinline__ void ks_geometry(T M, T a, const T x[3],
 /* The derivative half of the full ADM evaluation, given the geometry half
  * (KerrSchild.hpp:118-242 + FixedBGMetric.hpp:32-55): dgamma, dalpha,
  * dbeta, K_ij and d ln sqrt gamma. */
-__device__ __forceinline__ void ks_derivs(double M, double a,
-                                          const double xc[3],
-                                          const KSCore &c, const Geo &gin,
-                                          FullGeo &f)
+template <class T>
+__device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
+                                          const KSCoreT<T> &c,
+                                          const GeoT<T> &gin, FullGeoT<T> &f)
 {
     (void)M;
-    const double a2 = a * a;
-    const double z = xc[2];
-    const double el_t = 1.0;
-    const double H = c.H;
-    const double r = c.r, r2 = c.r2, rho = c.rho, rho2 = c.rho2;
+    const T a2 = a * a;
+    const T z = xc[2];
+    const T el_t = 1.0;
+    const T H = c.H;
+    const T r = c.r, r2 = c.r2, rho = c.rho, rho2 = c.rho2;
 
     /* get_KS_derivs (KerrSchild.hpp:171-242) */
-    const double cos_theta2 = c.cos_theta * c.cos_theta;
-    double drdx[3], dHdx[3], dldx[3][3];
+    const T cos_theta2 = c.cos_theta * c.cos_theta;
+    T drdx[3], dHdx[3], dldx[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        const double drhodx = xc[i] / rho;
+        const T drhodx = xc[i] / rho;
         drdx[i] = 0.5 / r *
                   (rho * drhodx +
                    0.5 / sqrt(0.25 * (rho2 - a2) * (rho2 - a2) + a2 * z * z) *
@@ -290,7 +291,7 @@ __device__ __forceinline__ void ks_derivs(double M, double a,
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        const double dcosthetadx = -z / r2 * drdx[i] + kdelta(i, 2) / r;
+        const T dcosthetadx = -z / r2 * drdx[i] + kdelta(i, 2) / r;
         dHdx[i] = H * (drdx[i] / r -
                        2.0 / (r2 + a2 * cos_theta2) *
                            (r * drdx[i] + a2 * c.cos_theta * dcosthetadx));
@@ -308,10 +309,10 @@ __device__ __forceinline__ void ks_derivs(double M, double a,
             (r2 + a2);
         dldx[2][i] = -xc[2] * drdx[i] / r2 + kdelta(i, 2) / r;
     }
-    const double dltdx = 0.0;
+    const T dltdx = 0.0;
 
     f.geo = gin;
-    const Geo &g = f.geo;
+    const GeoT<T> &g = f.geo;
 
     /* d1_gamma[i][j][k] stored as dg[k][i][j] */
 #pragma unroll
@@ -324,8 +325,8 @@ __device__ __forceinline__ void ks_derivs(double M, double a,
                     2.0 * (c.el[i] * c.el[j] * dHdx[k] +
                            H * c.el[i] * dldx[j][k] + H * c.el[j] * dldx[i][k]);
 
-    const double lapse2 = g.lapse * g.lapse;
-    const double lapse3 = g.lapse * g.lapse * g.lapse;
+    const T lapse2 = g.lapse * g.lapse;
+    const T lapse3 = g.lapse * g.lapse * g.lapse;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         f.dlapse[i] = -lapse3 * el_t * (el_t * dHdx[i] + 2.0 * H * dltdx);
@@ -346,15 +347,15 @@ __device__ __forceinline__ void ks_derivs(double M, double a,
 #pragma unroll
         for (int j = 0; j < 3; ++j)
         {
-            double LLL[3];
+            T LLL[3];
 #pragma unroll
             for (int l = 0; l < 3; ++l)
                 LLL[l] = 0.5 * (f.dg[j][i][l] + f.dg[i][j][l] - f.dg[l][i][j]);
-            double Kij = 0.0;
+            T Kij = 0.0;
 #pragma unroll
             for (int k = 0; k < 3; ++k)
             {
-                double ULL = 0;
+                T ULL = 0;
 #pragma unroll
                 for (int l = 0; l < 3; ++l)
                     ULL += g.gu[k][l] * LLL[l];
@@ -372,7 +373,7 @@ __device__ __forceinline__ void ks_derivs(double M, double a,
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        double s = 0.0;
+        T s = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -838,8 +839,8 @@ __device__ __forceinline__ void flux_of(const PrimT<T> &p, const ConsT<T> &c,
 
 /* ---------------- Atmosphere (Atmosphere.hpp:24-60) ---------------- */
 template <class T>
-__device__ __forceinline__ void floors(PrimT<T> &p, const Options &o,
-                                       const T g[3][3])
+__device__ __forceinline__ void floors_g(PrimT<T> &p, const Options &o,
+                                         T gamma, const T g[3][3])
 {
     p.rho = std_max(p.rho, T(o.rho_floor));
     p.pressure = std_max(p.pressure, T(o.pressure_floor));
@@ -851,7 +852,14 @@ __device__ __forceinline__ void floors(PrimT<T> &p, const Options &o,
         for (int i = 0; i < 3; ++i)
             p.v[i] *= scale;
     }
-    p.eps = p.pressure / ((T(o.gamma) - T(1)) * p.rho);
+    p.eps = p.pressure / ((gamma - T(1)) * p.rho);
+}
+
+template <class T>
+__device__ __forceinline__ void floors(PrimT<T> &p, const Options &o,
+                                       const T g[3][3])
+{
+    floors_g(p, o, T(o.gamma), g);
 }
 
 __device__ __forceinline__ void atmosphere_prim(const Options &o, Prim &p)
@@ -916,14 +924,13 @@ __device__ __forceinline__ void char_speeds(const PrimT<T> &p, T gamma,
 /* compute_muscl_hlle_flux (FiniteVolume.hpp:63-98) after the MUSCL step:
  * floors, prim2con and HLLE on the face geometry. */
 template <class T>
-__device__ __forceinline__ void hlle_face(PrimT<T> lp, PrimT<T> rp,
-                                          const Options &o,
-                                          const GeoT<T> &face, int d,
-                                          ConsT<T> &f)
+__device__ __forceinline__ void hlle_face_g(PrimT<T> lp, PrimT<T> rp,
+                                            const Options &o, T gamma,
+                                            const GeoT<T> &face, int d,
+                                            ConsT<T> &f)
 {
-    const T gamma = T(o.gamma);
-    floors(lp, o, face.g);
-    floors(rp, o, face.g);
+    floors_g(lp, o, gamma, face.g);
+    floors_g(rp, o, gamma, face.g);
     ConsT<T> lc, rc;
     prim2con(lp, gamma, face.g, lc);
     prim2con(rp, gamma, face.g, rc);
@@ -954,6 +961,16 @@ __device__ __forceinline__ void hlle_face(PrimT<T> lp, PrimT<T> rp,
                  inv;
 }
 
+template <class T>
+__device__ __forceinline__ void hlle_face(const PrimT<T> &lp,
+                                          const PrimT<T> &rp,
+                                          const Options &o,
+                                          const GeoT<T> &face, int d,
+                                          ConsT<T> &f)
+{
+    hlle_face_g(lp, rp, o, T(o.gamma), face, d, f);
+}
+
 /* average_geometry (FArrayBoxFiniteVolume.hpp:194-216), K omitted */
 template <class T>
 __device__ __forceinline__ void average_geo(const GeoT<T> &l, const GeoT<T> &r,
@@ -974,22 +991,24 @@ __device__ __forceinline__ void average_geo(const GeoT<T> &l, const GeoT<T> &r,
 }
 
 /* ---------------- Static-metric sources (StaticMetric.hpp:193-275) ------- */
-__device__ __forceinline__ void static_sources(const Prim &p, const Cons &c,
-                                               double gamma, const FullGeo &f,
-                                               Cons &src)
+template <class T>
+__device__ __forceinline__ void static_sources(const PrimT<T> &p,
+                                               const ConsT<T> &c, T gamma,
+                                               const FullGeoT<T> &f,
+                                               ConsT<T> &src)
 {
-    const Geo &g = f.geo;
-    const double W = 1.0 / sqrt(1.0 - vsq(p.v, g.g));
-    const double h = 1.0 + p.eps + p.pressure / p.rho;
-    const double rhW2 = p.rho * h * W * W;
-    double T[3][3];
+    const GeoT<T> &g = f.geo;
+    const T W = 1.0 / sqrt(1.0 - vsq(p.v, g.g));
+    const T h = 1.0 + p.eps + p.pressure / p.rho;
+    const T rhW2 = p.rho * h * W * W;
+    T Tuu[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            T[i][j] = rhW2 * p.v[i] * p.v[j] + p.pressure * g.gu[i][j];
-    const double E = c.D + c.tau;
-    double mU[3];
+            Tuu[i][j] = rhW2 * p.v[i] * p.v[j] + p.pressure * g.gu[i][j];
+    const T E = c.D + c.tau;
+    T mU[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         mU[i] = 0.0;
@@ -998,7 +1017,7 @@ __device__ __forceinline__ void static_sources(const Prim &p, const Cons &c,
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             mU[i] += g.gu[i][j] * c.S[j];
-    Cons cf[3];
+    ConsT<T> cf[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         flux_of(p, c, d, g.lapse, g.shift, cf[d]);
@@ -1016,16 +1035,16 @@ __device__ __forceinline__ void static_sources(const Prim &p, const Cons &c,
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            src.tau += g.lapse * T[i][j] * f.K[i][j];
+            src.tau += g.lapse * Tuu[i][j] * f.K[i][j];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
     {
-        double ms = -E * f.dlapse[m];
+        T ms = -E * f.dlapse[m];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                ms += 0.5 * g.lapse * T[i][j] * f.dg[m][i][j];
+                ms += 0.5 * g.lapse * Tuu[i][j] * f.dg[m][i][j];
 #pragma unroll
         for (int s = 0; s < 3; ++s)
             ms += c.S[s] * f.dshift[s][m];

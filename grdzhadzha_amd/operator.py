@@ -228,6 +228,29 @@ class FixedBGLevelDataFiniteVolumeOperator:
         _raise(rc, err, "advance_rk4")
         return diag.as_dict()
 
+    def rk4_adjoint(self, state, dt, nsteps, shell_radius, first_step,
+                    checkpoint_every=20, stream=None):
+        """Discrete adjoint of `nsteps` RK4 steps (grhd_rk4_adjoint_box):
+        J = sum over steps n >= first_step of dt * mdot(U_n), the rest-mass
+        flux through |x| = shell_radius.  Returns (J, dJ/dU0 as a (5,) +
+        interior-shape tensor, dJ/dGamma, dJ/da) at fixed U0."""
+        _check_fab(state, self.box.storage_shape, "state")
+        ncomp = state.shape[0]
+        grad = torch.zeros((5,) + self.box.interior_shape,
+                           dtype=torch.float64, device=state.device)
+        J = ctypes.c_double()
+        par = (ctypes.c_double * 2)()
+        err = abi.GrhdError()
+        rc = self.lib.grhd_rk4_adjoint_box(
+            self.ctx.handle, ctypes.byref(self._cbox),
+            ctypes.byref(self._cparams), ctypes.byref(self._cbg),
+            ctypes.c_double(dt), int(nsteps), _ptr(state), ncomp,
+            self.comp_base, ctypes.c_double(shell_radius), int(first_step),
+            int(checkpoint_every), ctypes.byref(J), _ptr(grad), par,
+            ctypes.byref(err), _stream(stream))
+        _raise(rc, err, "rk4_adjoint")
+        return J.value, grad, par[0], par[1]
+
     def cell_geometry(self, time=0.0, stream=None):
         out = torch.empty((16,) + self.box.storage_shape, dtype=torch.float64,
                           device="cuda")

@@ -365,6 +365,31 @@ int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
                                          int ncomp, int comp_base,
                                          int *is_finite, void *stream);
 
+/* Discrete adjoint of `nsteps` RK4 steps of size dt of one box (the steps
+ * of grhd_advance_rk4_box, starting at time 0) for the objective
+ *   J = sum_{n = first_step}^{nsteps} dt * mdot(U_n),
+ *   mdot(U) = -dx^2 sum over interior cells with ||x| - r0| < dx/2 of
+ *             sqrt(gamma) D (alpha v^i - beta^i) x_i / |x|,
+ * the rest-mass flux through the coordinate sphere |x| = r0 = shell_radius
+ * (v from con2prim of U_n).  No counterpart in the reference.  Returns J
+ * (host), dJ/dU_0 (device, 5 x interior cells: D, S1, S2, S3, tau,
+ * component-major over the interior box) and dJ/dGamma, dJ/da (host,
+ * 2 doubles) at fixed U_0.  Limiter, floor, HLLE wave-speed, atmosphere and
+ * reset branches are frozen at the forward pass; con2prim is differentiated
+ * at its converged root by the implicit-function theorem.  The forward
+ * state is checkpointed every `checkpoint_every` steps and each segment is
+ * recomputed in the reverse sweep.  `state` is not modified.  Scope:
+ * Kerr-Schild background, box == ProblemDomain (outflow ghosts), no
+ * volume-time sources, FP64 faces; otherwise GRHD_ERR_INVALID_ARGUMENT. */
+int grhd_rk4_adjoint_box(grhd_ctx *ctx, const grhd_box *box,
+                         const grhd_params *params,
+                         const grhd_background *background, double dt,
+                         int nsteps, const double *state, int ncomp,
+                         int comp_base, double shell_radius, int first_step,
+                         int checkpoint_every, double *objective,
+                         double *dJ_dconserved, double *dJ_dparams,
+                         grhd_error *err, void *stream);
+
 /* KITPCHK1 checkpoint payload <-> level pool, on the device.  The payload
  * covers the ProblemDomain [domain_lo, domain_hi] grown by `ghosts`, point
  * major with the ncomp components of a cell adjacent (ncomp = NUM_VARS):
