@@ -55,7 +55,10 @@ class GrhdParams(ctypes.Structure):
                 ("use_static_metric_sources", ctypes.c_int),
                 ("use_metric_volume_time_sources", ctypes.c_int),
                 ("metric_time_derivative_step", ctypes.c_double),
-                ("mixed_precision", ctypes.c_int)]
+                ("mixed_precision", ctypes.c_int),
+                ("excise_horizon", ctypes.c_int),
+                ("excision_inner_buffer", ctypes.c_double),
+                ("excision_outer_buffer", ctypes.c_double)]
 
 
 class GrhdBackground(ctypes.Structure):

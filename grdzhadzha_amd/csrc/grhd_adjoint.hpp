@@ -699,6 +699,28 @@ __device__ __forceinline__ void recover_reset_d(const ConsT<Dual> &un,
     prim2con(prim, gamma, g.g, out);
 }
 
+/* Horizon excision: an excised cell's rhs is the constant zero
+ * (ExcisionTest), so its lambda_k carries nothing back through its fluxes
+ * or sources.  Zeroes the 5 rows of lambda_k of every excised cell. */
+__global__ void __launch_bounds__(kBlock)
+    k_adj_excise(BoxDev b, Options o, AdjParams pa, double *lam_k)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= b.inpts)
+        return;
+    const int i = b.lo[0] + (int)(it % b.in[0]);
+    const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+    const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+    double x[3];
+    coords(i, j, k, b.dx, b.center, x);
+    if (!ks_excised(pa.M, pa.a, x, o.excise_inner, o.excise_outer))
+        return;
+    const long long ni = b.inpts;
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+        lam_k[m * ni + it] = 0.0;
+}
+
 /* RK4 stage update of the adjoint's forward sweep: the same combination
  * and order as k_cells<MODE_RK4_s> (U_s = U0 + c_stage k_s; the
  * accumulator U0 + dt/6 k1 + dt/3 k2 + dt/3 k3; U_4 = acc + dt/6 k4), then

@@ -45,6 +45,9 @@ Options make_options(const grhd_params *p)
     o.use_static_sources = p->use_static_metric_sources;
     o.use_volume_sources = p->use_metric_volume_time_sources;
     o.mixed = p->mixed_precision != 0;
+    o.excise = p->excise_horizon != 0;
+    o.excise_inner = p->excision_inner_buffer;
+    o.excise_outer = p->excision_outer_buffer;
     return o;
 }
 
@@ -113,6 +116,8 @@ int check_common(const grhd_params *o, const grhd_background *bg, int ncomp,
     if (cb < 0 || cb + GRHD_NUM_COMPS > ncomp)
         return GRHD_ERR_INVALID_ARGUMENT;
     if (o->max_iterations < 0)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (o->excise_horizon && bg->kind != GRHD_BG_KERR_SCHILD)
         return GRHD_ERR_INVALID_ARGUMENT;
     if (bg->kind == GRHD_BG_KERR_SCHILD)
     {
@@ -1093,11 +1098,16 @@ void adj_fwd_step(AdjRun &r, const double *X0, double *const X[5],
     }
 }
 
-/* lam_X = L'(X)^T lam_k (5 x interior), parameter adjoints into lam_par. */
-void adj_rhs_transpose(AdjRun &r, const double *X, const double *lam_k,
+/* lam_X = L'(X)^T lam_k (5 x interior), parameter adjoints into lam_par.
+ * With horizon excision the rows of excised cells in lam_k are zeroed in
+ * place first (their rhs is the constant zero). */
+void adj_rhs_transpose(AdjRun &r, const double *X, double *lam_k,
                        double *lam_X, double *lam_W, double *lam_par)
 {
     const BoxDev &b = r.b;
+    if (r.o.excise)
+        hipLaunchKernelGGL(k_adj_excise, dim3(grid_of(b.inpts)), dim3(kBlock),
+                           0, r.s, b, r.o, r.pa, lam_k);
     (void)hipMemcpyAsync(r.work, X, sizeof(double) * 5 * b.gnpts,
                          hipMemcpyDeviceToDevice, r.s);
     launch_prims(r.c, r.o, r.bgd, r.p, POOL_STATE, false, false, 0, r.s);
@@ -1127,7 +1137,7 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
                     double *dJ_dparams, grhd_error *err, void *st)
 {
     (void)time;
-    if (!box || !state || !J || !dJ_dU0 || !dJ_dparams || nsteps < 1 ||
+    if (!box || !state || !J || nsteps < 1 ||
         first_step < 1 || first_step > nsteps || ckpt_every < 1 || dt <= 0.0)
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
@@ -1158,21 +1168,26 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     r.s = (hipStream_t)st;
     const BoxDev &b = r.b;
     const long long G = b.gnpts, I = b.inpts;
+    /* the objective alone needs neither checkpoints nor adjoint buffers */
+    const bool grad = dJ_dU0 && dJ_dparams;
+    if (!grad)
+        ckpt_every = nsteps;
     const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
 
     DevBufs mem;
     r.work = mem.get(GRHD_NUM_COMPS * G);
-    double *ckpt = mem.get((long long)nseg * 5 * G);
     /* per step of a segment: X[1..4] (5 x G) and k[0..3], acc3 (5 x I) */
     const long long per_step = 4 * 5 * G + 5 * 5 * I;
-    double *seg = mem.get((long long)ckpt_every * per_step);
+    const long long nadj = grad ? 1 : 0;
+    double *ckpt = mem.get(nadj * nseg * 5 * G + 1);
+    double *seg = mem.get(nadj * ckpt_every * per_step + 1);
     double *acc = mem.get(5 * I);
-    double *lam = mem.get(5 * I);
-    double *lam_k[4] = {mem.get(5 * I), mem.get(5 * I), mem.get(5 * I),
-                        mem.get(5 * I)};
-    double *lam_X = mem.get(5 * I);
-    double *lam_t = mem.get(5 * I);
-    double *lam_W = mem.get(6 * G);
+    double *lam = mem.get(nadj * 5 * I + 1);
+    double *lam_k[4] = {mem.get(nadj * 5 * I + 1), mem.get(nadj * 5 * I + 1),
+                        mem.get(nadj * 5 * I + 1), mem.get(nadj * 5 * I + 1)};
+    double *lam_X = mem.get(nadj * 5 * I + 1);
+    double *lam_t = mem.get(nadj * 5 * I + 1);
+    double *lam_W = mem.get(nadj * 6 * G + 1);
     double *lam_par = mem.get(2);
     double *terms = mem.get(I);
     const long long nb = (I + kBlock - 1) / kBlock;
@@ -1196,7 +1211,8 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     fill0(r.work, GRHD_NUM_COMPS * G, s);
     fill0(mdot, nsteps + 1, s);
     fill0(lam_par, 2, s);
-    fill0(lam, 5 * I, s);
+    if (grad)
+        fill0(lam, 5 * I, s);
     (void)hipMemcpyAsync(X0, state + (long long)cb * G,
                          sizeof(double) * 5 * G, hipMemcpyDeviceToDevice, s);
 
@@ -1213,7 +1229,7 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     fx[0] = X0;
     for (int n = 0; n < nsteps; ++n)
     {
-        if (n % ckpt_every == 0)
+        if (grad && n % ckpt_every == 0)
             (void)hipMemcpyAsync(ckpt + (long long)(n / ckpt_every) * 5 * G,
                                  X0, sizeof(double) * 5 * G,
                                  hipMemcpyDeviceToDevice, s);
@@ -1233,6 +1249,8 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     for (int n = first_step; n <= nsteps; ++n)
         Jv += dt * hm[n];
     *J = Jv;
+    if (!grad)
+        return GRHD_OK;
 
     /* reverse sweep, one checkpoint segment at a time */
     const unsigned gi = grid_of(I);

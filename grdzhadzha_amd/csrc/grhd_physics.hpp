@@ -72,6 +72,8 @@ struct Options
     double theta;
     int use_reconstruction, use_static_sources, use_volume_sources;
     int mixed; /* FP32 reconstruction, face metric and HLLE */
+    int excise; /* zero rhs inside the buffered Kerr-Schild horizons */
+    double excise_inner, excise_outer;
 };
 
 /* A background evaluated at one time (background_factory(time)); the
@@ -95,6 +97,24 @@ template <class T> __device__ __forceinline__ T std_min(T a, T b)
 template <class T = double> __device__ __forceinline__ T kdelta(int i, int j)
 {
     return (i == j) ? T(1) : T(0);
+}
+
+/* KerrSchild::check_if_excised (KerrSchild.hpp:247-280): x lies inside the
+ * outer horizon's oblate ellipsoid (x^2 + y^2) / (2 M r_+) + z^2 / r_+^2 = 1
+ * scaled by outer_buffer, or inside the inner horizon's scaled by
+ * inner_buffer.  With a = 0 the inner ratio is inf or NaN: never inside. */
+__device__ __forceinline__ bool ks_excised(double M, double a,
+                                           const double x[3],
+                                           double inner_buffer,
+                                           double outer_buffer)
+{
+    const double root = sqrt(M * M - a * a);
+    const double r_out = M + root, r_in = M - root;
+    const double cyl2 = x[0] * x[0] + x[1] * x[1];
+    const double z2 = x[2] * x[2];
+    const double outer = cyl2 / (2.0 * M * r_out) + z2 / r_out / r_out;
+    const double inner = cyl2 / (2.0 * M * r_in) + z2 / r_in / r_in;
+    return outer < outer_buffer || inner < inner_buffer;
 }
 
 /* ---------------- GRChombo TensorAlgebra (restated) ---------------- */

@@ -867,6 +867,14 @@ __global__ void __launch_bounds__(kBlock)
             for (int m = 0; m < 3; ++m)
                 rhs.S[m] += 0.0 - us.S[m] * dlog;
         }
+        /* ExcisionTest (ExcisionTest.hpp:40-63): after the matter rhs */
+        if (o.excise &&
+            ks_excised(bg_s.M, bg_s.a, x, o.excise_inner, o.excise_outer))
+        {
+            rhs.D = 0.0;
+            rhs.tau = 0.0;
+            rhs.S[0] = rhs.S[1] = rhs.S[2] = 0.0;
+        }
 
         if (MODE == MODE_RHS)
         {

@@ -251,6 +251,22 @@ class FixedBGLevelDataFiniteVolumeOperator:
         _raise(rc, err, "rk4_adjoint")
         return J.value, grad, par[0], par[1]
 
+    def rk4_objective(self, state, dt, nsteps, shell_radius, first_step,
+                      stream=None):
+        """The objective J of rk4_adjoint alone (forward sweep only)."""
+        _check_fab(state, self.box.storage_shape, "state")
+        J = ctypes.c_double()
+        err = abi.GrhdError()
+        rc = self.lib.grhd_rk4_adjoint_box(
+            self.ctx.handle, ctypes.byref(self._cbox),
+            ctypes.byref(self._cparams), ctypes.byref(self._cbg),
+            ctypes.c_double(dt), int(nsteps), _ptr(state), state.shape[0],
+            self.comp_base, ctypes.c_double(shell_radius), int(first_step),
+            int(nsteps), ctypes.byref(J), None, None, ctypes.byref(err),
+            _stream(stream))
+        _raise(rc, err, "rk4_objective")
+        return J.value
+
     def cell_geometry(self, time=0.0, stream=None):
         out = torch.empty((16,) + self.box.storage_shape, dtype=torch.float64,
                           device="cuda")

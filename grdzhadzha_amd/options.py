@@ -82,6 +82,12 @@ class OperatorOptions:
     # no counterpart in the reference: FP32 reconstruction, face metric and
     # HLLE, everything else FP64 (False: the bit-exact FP64 path)
     mixed_precision: bool = False
+    # KerrSchild::check_if_excised (KerrSchild.hpp:247-280) with
+    # ExcisionTest's zero right-hand side (ExcisionTest.hpp:40-63) inside
+    # the buffered horizons; Kerr-Schild backgrounds only
+    excise_horizon: bool = False
+    excision_inner_buffer: float = 1.05
+    excision_outer_buffer: float = 0.9
 
     def to_c(self):
         p = abi.GrhdParams()
@@ -102,6 +108,9 @@ class OperatorOptions:
             self.use_metric_volume_time_sources)
         p.metric_time_derivative_step = self.metric_time_derivative_step
         p.mixed_precision = int(self.mixed_precision)
+        p.excise_horizon = int(self.excise_horizon)
+        p.excision_inner_buffer = self.excision_inner_buffer
+        p.excision_outer_buffer = self.excision_outer_buffer
         return p
 
 

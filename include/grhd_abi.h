@@ -129,6 +129,18 @@ typedef struct grhd_params
      * W, the flux-divergence sum, sources and the RK update stay FP64.
      * Zero (the default) is the bit-exact FP64 path. */
     int mixed_precision;
+    /* KerrSchild::check_if_excised (KerrSchild.hpp:247-280) applied the way
+     * ExcisionTest applies it to the matter right-hand side
+     * (Tests/KerrBHScalarTest/ExcisionTest.hpp:40-63): nonzero sets the
+     * conserved right-hand side of every interior cell with
+     * (x^2 + y^2) / (2 M r_+) + z^2 / r_+^2 < excision_outer_buffer or
+     * (x^2 + y^2) / (2 M r_-) + z^2 / r_-^2 < excision_inner_buffer to zero,
+     * after the fluxes and sources; those cells then keep their state.
+     * Kerr-Schild backgrounds only.  Zero (the default) leaves the
+     * right-hand side untouched. */
+    int excise_horizon;
+    double excision_inner_buffer; /* the reference's default: 1.05 */
+    double excision_outer_buffer; /* the reference's default: 0.9 */
 } grhd_params;
 
 /* Analytic background.  For the binary, the orbital time at a call is
@@ -378,7 +390,9 @@ int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
  * reset branches are frozen at the forward pass; con2prim is differentiated
  * at its converged root by the implicit-function theorem.  The forward
  * state is checkpointed every `checkpoint_every` steps and each segment is
- * recomputed in the reverse sweep.  `state` is not modified.  Scope:
+ * recomputed in the reverse sweep.  With dJ_dconserved or dJ_dparams NULL
+ * only the forward sweep runs and J is returned.  `state` is not
+ * modified.  Scope:
  * Kerr-Schild background, box == ProblemDomain (outflow ghosts), no
  * volume-time sources, FP64 faces; otherwise GRHD_ERR_INVALID_ARGUMENT. */
 int grhd_rk4_adjoint_box(grhd_ctx *ctx, const grhd_box *box,

@@ -336,6 +336,8 @@ class LevelOperatorBase
         p.use_metric_volume_time_sources =
             use_metric_volume_time_sources ? 1 : 0;
         p.metric_time_derivative_step = metric_time_derivative_step;
+        p.excision_inner_buffer = 1.05;
+        p.excision_outer_buffer = 0.9;
     }
 
     const grhd_background &background() const { return m_background; }
@@ -344,6 +346,16 @@ class LevelOperatorBase
     /* FP32 reconstruction, face metric and HLLE (see grhd_params); off by
      * default, which is the bit-exact FP64 path. */
     void set_mixed_precision(bool on) { m_params.mixed_precision = on ? 1 : 0; }
+
+    /* KerrSchild::check_if_excised with ExcisionTest's zero right-hand side
+     * (see grhd_params); Kerr-Schild backgrounds only. */
+    void set_horizon_excision(bool on, double inner_buffer = 1.05,
+                              double outer_buffer = 0.9)
+    {
+        m_params.excise_horizon = on ? 1 : 0;
+        m_params.excision_inner_buffer = inner_buffer;
+        m_params.excision_outer_buffer = outer_buffer;
+    }
 
   protected:
     std::vector<grhd_level_box> table(const DeviceLevelData &l,

@@ -62,6 +62,7 @@ def lib():
             "grhd_oracle_log_sqrt_det_dt": (d, [Bg, d, vp, d, vp, d]),
             "grhd_oracle_log": (d, [d]),
             "grhd_oracle_num_threads": (i, []),
+            "grhd_oracle_ks_excised": (i, [d, d, vp, d, d]),
             "grhd_oracle_leveldata_conserved_sum": (None, [B, i, P(vp), i, i,
                                                            P(d)]),
             "grhd_oracle_leveldata_conserved_state_is_finite": (
@@ -301,6 +302,13 @@ def minmod(a, b):
 
 def num_threads():
     return lib().grhd_oracle_num_threads()
+
+
+def ks_excised(M, a, x, inner_buffer=1.05, outer_buffer=0.9):
+    """KerrSchild::check_if_excised (KerrSchild.hpp:247-280)."""
+    xs = (ctypes.c_double * 3)(*[float(c) for c in x])
+    return bool(lib().grhd_oracle_ks_excised(M, a, xs, inner_buffer,
+                                             outer_buffer))
 
 
 def _level_args(boxes, fabs):

@@ -1516,6 +1516,44 @@ static void add_volume_time_sources(double *rhs, const fab_shape_t *rs,
             }
 }
 
+/* KerrSchild::check_if_excised (KerrSchild.hpp:247-280): inside the outer
+ * horizon's ellipsoid (x^2 + y^2) / (2 M r_+) + z^2 / r_+^2 scaled by
+ * outer_buffer, or the inner horizon's scaled by inner_buffer. */
+int grhd_oracle_ks_excised(double M, double a, const double x[3],
+                           double inner_buffer, double outer_buffer)
+{
+    const double root = sqrt(M * M - a * a);
+    const double r_out = M + root, r_in = M - root;
+    const double cyl2 = x[0] * x[0] + x[1] * x[1];
+    const double z2 = x[2] * x[2];
+    const double outer = cyl2 / (2.0 * M * r_out) + z2 / r_out / r_out;
+    const double inner = cyl2 / (2.0 * M * r_in) + z2 / r_in / r_in;
+    return outer < outer_buffer || inner < inner_buffer;
+}
+
+/* ExcisionTest::compute (Tests/KerrBHScalarTest/ExcisionTest.hpp:40-63):
+ * the matter right-hand side of an excised cell is set to zero. */
+static void excise_rhs(double *rhs, const fab_shape_t *rs, int cb,
+                       const grhd_box *b, const grhd_background *bgp,
+                       const grhd_params *o)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b->lo[2]; k <= b->hi[2]; ++k)
+        for (int j = b->lo[1]; j <= b->hi[1]; ++j)
+            for (int i = b->lo[0]; i <= b->hi[0]; ++i)
+            {
+                double x[3];
+                cell_coords(i, j, k, b->dx, b->center, x);
+                if (!grhd_oracle_ks_excised(bgp->mass, bgp->spin, x,
+                                            o->excision_inner_buffer,
+                                            o->excision_outer_buffer))
+                    continue;
+                const long long roff = fab_off(rs, i, j, k);
+                for (int c = 0; c < GRHD_NUM_COMPS; ++c)
+                    FAB(rhs, rs, cb + c, roff) = 0.0;
+            }
+}
+
 /* ------------------------------------------------------------------ */
 /* Argument checks                                                    */
 /* ------------------------------------------------------------------ */
@@ -1525,6 +1563,8 @@ static int check_args(const grhd_box *b, const grhd_params *o,
     if (!b || !o || !bg)
         return GRHD_ERR_INVALID_ARGUMENT;
     if (cb < 0 || cb + GRHD_NUM_COMPS > ncomp)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (o->excise_horizon && bg->kind != GRHD_BG_KERR_SCHILD)
         return GRHD_ERR_INVALID_ARGUMENT;
     if (!(b->dx > 0.0))
         return GRHD_ERR_INVALID_ARGUMENT;
@@ -1617,6 +1657,8 @@ int grhd_oracle_compute_rhs(const grhd_box *b, const grhd_params *o,
     if (o->use_metric_volume_time_sources)
         add_volume_time_sources(rhs, &rs, cb, state, &ss, b, bgp, time,
                                 o->metric_time_derivative_step);
+    if (o->excise_horizon)
+        excise_rhs(rhs, &rs, cb, b, bgp, o);
     return GRHD_OK;
 }
 

@@ -5,6 +5,8 @@ Bar: conserved variables within 1e-12 relative (max |gpu - oracle| / max
 evaluate the reference's formulas in the same order without contraction),
 recovery masks and reset counts bit-exact, dt bit-exact.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -220,6 +222,45 @@ def test_rk4_torus_two_steps(hip_context):
 
 def test_ssprk2_michel(hip_context):
     _step_case(workloads.michel_box_32(), hip_context, "ssprk2")
+
+
+# ------------------------------------------------------- horizon excision
+
+
+def excised(w):
+    return workloads.Workload(
+        w.name + "_excised", w.box, w.background,
+        dataclasses.replace(w.options, excise_horizon=True), w.state, w.dt,
+        w.nsteps)
+
+
+def test_rhs_michel_32_excised(hip_context):
+    _rhs_case(excised(workloads.michel_box_32()), hip_context)
+
+
+def test_rhs_torus_kerr_excised(hip_context):
+    _rhs_case(excised(small_torus()), hip_context)
+
+
+def test_rk4_michel_excised_three_steps(hip_context):
+    _step_case(excised(workloads.michel_box_32()), hip_context, "rk4", 3)
+
+
+def test_ssprk2_torus_excised(hip_context):
+    _step_case(excised(small_torus()), hip_context, "ssprk2")
+
+
+def test_excision_needs_a_kerr_schild_background(hip_context):
+    w = workloads.michel_box_32(n=16, half_width=4.0)
+    op = FixedBGLevelDataFiniteVolumeOperator(
+        CircularBinaryPNParams(0.5, 0.5, 10.0, 0.0, 0.0, 0.1, 0.4), w.box,
+        dataclasses.replace(w.options, excise_horizon=True),
+        context=hip_context)
+    st = to_dev(w.state)
+    rhs = torch.zeros((st.shape[0],) + w.box.interior_shape,
+                      dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        op.compute_rhs(rhs, st)
 
 
 # ---------------------------------------------------------------- errors
