@@ -152,6 +152,51 @@ def bench_batched(args):
     tb = run_batched(steps)
     batched = cells * steps / tb
 
+    # batched mixed precision on the same box set, from the same state:
+    # throughput and relative L1 / Linf error per conserved variable over
+    # every interior cell of every level against batched FP64 (which is
+    # bit-exact with the oracle)
+    import dataclasses
+    o32 = dataclasses.replace(opts, mixed_precision=True)
+    ops32 = [LevelOperator(bg, boxes, o32, time_dependent=True, context=ctx)
+             for boxes, _, _ in levels]
+
+    def run_levels(lops, k):
+        pools = [p.clone() for p in init]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for lev, pool, (_, _, dt) in zip(lops, pools, levels):
+            lev.advance_rk4(pool, dt, k, time=0.0)
+        torch.cuda.synchronize()
+        return pools, time.perf_counter() - t0
+
+    run_levels(ops32, args.warmup or 1)
+    p32, tm = run_levels(ops32, steps)
+    p64, _ = run_levels(ops, steps)
+    mixed = cells * steps / tm
+    names = ("D", "S1", "S2", "S3", "tau")
+    num = {k: 0.0 for k in names}
+    den = {k: 0.0 for k in names}
+    mx = {k: 0.0 for k in names}
+    mref = {k: 0.0 for k in names}
+    for lev, a64, a32, (boxes, _, _) in zip(ops, p64, p32, levels):
+        for b, f64, f32 in zip(boxes, lev.unpack(a64), lev.unpack(a32)):
+            sl = b.interior_slices()
+            for c, name in enumerate(names):
+                ref = f64[c][sl]
+                diff = abs(f32[c][sl] - ref)
+                num[name] += float(diff.sum())
+                den[name] += float(abs(ref).sum())
+                mx[name] = max(mx[name], float(diff.max()))
+                mref[name] = max(mref[name], float(abs(ref).max()))
+    err_m = {k: {"l1_rel": num[k] / den[k], "linf_rel": mx[k] / mref[k]}
+             for k in names}
+    del p32, p64
+    print("bench: batched mixed %.3f s (%.3e cell-updates/s), max L1 rel "
+          "error %.3e" % (tm, mixed, max(e["l1_rel"] for e in
+                                         err_m.values())),
+          file=sys.stderr, flush=True)
+
     # plain path: one call per box, the same boxes and steps; each box has
     # its own context, so its launch plan is built once, not per call
     plain_steps = max(1, min(steps, args.plain_steps))
@@ -185,7 +230,12 @@ def bench_batched(args):
                        "interior_cells": cells, "parallelism": "single GPU"},
             "plain_per_box": {"value": plain, "steps": plain_steps,
                               "ms_per_step": tp / plain_steps * 1e3},
-            "speedup_vs_plain": batched / plain}
+            "speedup_vs_plain": batched / plain,
+            "batched_mixed": {
+                "value": mixed, "ms_per_step": tm / steps * 1e3,
+                "speedup_vs_plain_per_box": mixed / plain,
+                "speedup_vs_batched_fp64": mixed / batched,
+                "error_vs_batched_fp64": err_m}}
 
 
 def bench_mixed(args):
@@ -225,15 +275,25 @@ def bench_mixed(args):
         torch.cuda.synchronize()
         return st, time.perf_counter() - t0
 
-    run(op64, 1)
-    run(op32, 1)
+    sl = (slice(None),) + w.box.interior_slices()
+    names = ("D", "S1", "S2", "S3", "tau")
+
+    def l1(b64, b32):
+        return {name: float(np.abs(b32[c][sl[1:]] - b64[c][sl[1:]]).sum() /
+                            np.abs(b64[c][sl[1:]]).sum())
+                for c, name in enumerate(names)}
+
+    # one step from the same state: the FP32 rounding of a single step,
+    # before branch decisions (resets, floors, atmosphere) can diverge
+    one64, _ = run(op64, 1)
+    one32, _ = run(op32, 1)
+    err_one = l1(one64.cpu().numpy(), one32.cpu().numpy())
+    del one64, one32
     s64, t64 = run(op64, steps)
     s32, t32 = run(op32, steps)
     a64 = s64.cpu().numpy()
     a32 = s32.cpu().numpy()
     del s64, s32
-    sl = (slice(None),) + w.box.interior_slices()
-    names = ("D", "S1", "S2", "S3", "tau")
     err = {}
     for c, name in enumerate(names):
         ref = a64[c][sl[1:]]
@@ -259,6 +319,7 @@ def bench_mixed(args):
             "ms_per_step": t32 / steps * 1e3,
             "fp64_ms_per_step": t64 / steps * 1e3,
             "error_vs_fp64": err,
+            "l1_rel_after_one_step": err_one,
             "reference": "the FP64 GPU path, bit-exact with the oracle"}
 
 
