@@ -560,7 +560,8 @@ def main():
                           "steps": b["steps"], "value": b["value"],
                           "ms_per_step": b["ms_per_step"],
                           "plain_per_box": b["plain_per_box"],
-                          "speedup_vs_plain": b["speedup_vs_plain"]}
+                          "speedup_vs_plain": b["speedup_vs_plain"],
+                          "batched_mixed": b["batched_mixed"]}
     if not args.no_mixed:
         torch.cuda.empty_cache()
         out["mixed"] = bench_mixed(args)
