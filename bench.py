@@ -396,6 +396,8 @@ def bench_adjoint(args):
         rng = np.random.default_rng(16)
         dirs = [ah.smooth_field(box, rng) * rho0 for _ in range(ndir)]
         fd_out = {}
+        if not hs:
+            fd_out["skipped"] = True
         for h in hs:
             rels = []
             for d in dirs:
@@ -427,6 +429,9 @@ def bench_adjoint(args):
                        "last %d steps, horizon excised" % (n, spin0, steps,
                                                           window),
            "checkpoint_every": every, "recovery_tolerance": 1e-16}
+    if args.no_adjoint_checks:
+        rec.update(run(True, 0, ()))
+        return rec
     rec.update(run(True, 16, (1e-6,)))
     rec["without_excision"] = run(False, 4, (1e-4, 1e-6))
     return rec
@@ -442,6 +447,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint record of the default run")
+    ap.add_argument("--no-adjoint-checks", action="store_true",
+                    help="adjoint record without the finite-difference "
+                         "checks and the run without excision (profiling)")
     ap.add_argument("--workload", choices=["torus", "batched", "mixed",
                                            "adjoint"],
                     default="torus",

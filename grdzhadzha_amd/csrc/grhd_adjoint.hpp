@@ -158,6 +158,28 @@ __device__ __forceinline__ Dual presid_dual(Dual p, Dual D, Dual tau, Dual S2,
  * primitives are then formed from (U, p) exactly as con2prim forms them.
  * A root taken at the recovery pressure floor by the early exit is a
  * constant.  Returns false where the forward solve fails. */
+/* The forward solve alone: the root pressure of con2prim (values). */
+__device__ __forceinline__ bool con2prim_root(const Cons &cv,
+                                              const Options &o, double gamma,
+                                              const double guv[3][3],
+                                              double &ps)
+{
+    Options ov = o;
+    ov.gamma = gamma;
+    Prim pv;
+    double res;
+    if (!con2prim(cv, ov, guv, pv, res))
+        return false;
+    ps = pv.pressure;
+    return true;
+}
+
+/* The primitives with tangents at a given root ps of the forward solve. */
+__device__ __forceinline__ void con2prim_at_root(const ConsT<Dual> &c,
+                                                 const Options &o, Dual gamma,
+                                                 const Dual gu[3][3],
+                                                 double ps, PrimT<Dual> &out);
+
 __device__ bool con2prim_ift(const ConsT<Dual> &c, const Options &o,
                              Dual gamma, const Dual gu[3][3],
                              PrimT<Dual> &out)
@@ -174,13 +196,18 @@ __device__ bool con2prim_ift(const ConsT<Dual> &c, const Options &o,
         for (int n = 0; n < 3; ++n)
             guv[m][n] = gu[m][n].v;
     }
-    Options ov = o;
-    ov.gamma = gamma.v;
-    Prim pv;
-    double res;
-    if (!con2prim(cv, ov, guv, pv, res))
+    double ps;
+    if (!con2prim_root(cv, o, gamma.v, guv, ps))
         return false;
-    const double ps = pv.pressure;
+    con2prim_at_root(c, o, gamma, gu, ps, out);
+    return true;
+}
+
+__device__ __forceinline__ void con2prim_at_root(const ConsT<Dual> &c,
+                                                 const Options &o, Dual gamma,
+                                                 const Dual gu[3][3],
+                                                 double ps, PrimT<Dual> &out)
+{
     Dual S2 = 0.;
 #pragma unroll
     for (int m = 0; m < 3; ++m)
@@ -211,7 +238,6 @@ __device__ bool con2prim_ift(const ConsT<Dual> &c, const Options &o,
             vi += gu[i][j] * c.S[j] / epp;
         out.v[i] = vi;
     }
-    return true;
 }
 
 /* ------------------------------------------------------------------ */
@@ -231,14 +257,53 @@ enum
     SEED_NONE = -3
 };
 
-/* Face flux of direction D from the W of cells L-1, L, R, R+1 along D and
- * the geometry of cells L and R (k_faces_x / k_faces<D>). */
+/* A double geometry as duals with zero tangents: the metric of every pass
+ * whose seed is not the spin (the metric depends on a and x only), so the
+ * Kerr-Schild evaluation runs once in double instead of once per pass. */
+__device__ __forceinline__ void dual_geo(const Geo &g, GeoT<Dual> &d)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        d.shift[i] = Dual(g.shift[i]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            d.g[i][j] = Dual(g.g[i][j]);
+            d.gu[i][j] = Dual(g.gu[i][j]);
+        }
+    }
+    d.lapse = Dual(g.lapse);
+}
+
+__device__ __forceinline__ void dual_full_geo(const FullGeo &g,
+                                              FullGeoT<Dual> &d)
+{
+    dual_geo(g.geo, d.geo);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        d.dlapse[i] = Dual(g.dlapse[i]);
+        d.dlogsqrtdet[i] = Dual(g.dlogsqrtdet[i]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            d.K[i][j] = Dual(g.K[i][j]);
+            d.dshift[i][j] = Dual(g.dshift[i][j]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                d.dg[i][j][k] = Dual(g.dg[i][j][k]);
+        }
+    }
+}
+
+/* Face flux of direction D from the W of cells L-1, L, R, R+1 along D on a
+ * given face geometry (k_faces_x / k_faces<D> after the metric). */
 template <class T>
-__device__ __forceinline__ void face_flux_local(const PrimT<T> w[4],
-                                                const T xl[3], const T xr[3],
-                                                T M, T a, T gamma,
-                                                const Options &o, int D,
-                                                ConsT<T> &f)
+__device__ __forceinline__ void face_flux_geo(const PrimT<T> w[4],
+                                              const GeoT<T> &face, T gamma,
+                                              const Options &o, int D,
+                                              ConsT<T> &f)
 {
     PrimT<T> lp = w[1], rp = w[2];
     if (o.use_reconstruction)
@@ -247,33 +312,47 @@ __device__ __forceinline__ void face_flux_local(const PrimT<T> w[4],
         slope_step(w[0], w[1], w[2], T(o.theta), unused, lp);
         slope_step(w[1], w[2], w[3], T(o.theta), rp, unused);
     }
-    GeoT<T> gl, gr, face;
-    ks_geometry(M, a, xl, gl);
-    ks_geometry(M, a, xr, gr);
-    average_geo(gl, gr, face);
     hlle_face_g(lp, rp, o, gamma, face, D, f);
 }
 
-/* Static-metric sources of an interior cell (k_cells): floors on W, the
- * atmosphere skip, then the sources.  Returns false where skipped. */
+/* The face geometry: mean of the geometries of cells L and R. */
 template <class T>
-__device__ __forceinline__ bool sources_local(const PrimT<T> &w,
-                                              const ConsT<T> &u, const T x[3],
-                                              T M, T a, T gamma,
-                                              const Options &o, ConsT<T> &src)
+__device__ __forceinline__ void face_geo(const T xl[3], const T xr[3], T M,
+                                         T a, GeoT<T> &face)
+{
+    GeoT<T> gl, gr;
+    ks_geometry(M, a, xl, gl);
+    ks_geometry(M, a, xr, gr);
+    average_geo(gl, gr, face);
+}
+
+/* The full geometry the static-metric sources read. */
+template <class T>
+__device__ __forceinline__ void source_geo(const T x[3], T M, T a,
+                                           FullGeoT<T> &fg)
 {
     KSCoreT<T> core;
     GeoT<T> g;
     ks_geometry_core(M, a, x, g, core);
+    ks_derivs(M, a, x, core, g, fg);
+}
+
+/* Static-metric sources of an interior cell (k_cells) on a given full
+ * geometry: floors on W, the atmosphere skip, then the sources.  Returns
+ * false where skipped. */
+template <class T>
+__device__ __forceinline__ bool sources_geo(const PrimT<T> &w,
+                                            const ConsT<T> &u,
+                                            const FullGeoT<T> &fg, T gamma,
+                                            const Options &o, ConsT<T> &src)
+{
     PrimT<T> pw = w;
-    floors_g(pw, o, gamma, g.g);
+    floors_g(pw, o, gamma, fg.geo.g);
     Prim pv;
     pv.rho = val(pw.rho);
     pv.pressure = val(pw.pressure);
     if (is_atmosphere(pv, o, o.source_atmosphere_factor))
         return false;
-    FullGeoT<T> fg;
-    ks_derivs(M, a, x, core, g, fg);
     static_sources(pw, u, gamma, fg, src);
     return true;
 }
@@ -446,6 +525,11 @@ __global__ void __launch_bounds__(kBlock)
             coords(c[0], c[1], c[2], b.dx, b.center, xr);
             const Dual xld[3] = {Dual(xl[0]), Dual(xl[1]), Dual(xl[2])};
             const Dual xrd[3] = {Dual(xr[0]), Dual(xr[1]), Dual(xr[2])};
+            /* the face metric for every pass but the spin's */
+            Geo facev;
+            face_geo(xl, xr, pa.M, pa.a, facev);
+            GeoT<Dual> face0;
+            dual_geo(facev, face0);
             /* 24 W inputs, then Gamma and a */
             for (int s = 0; s < 26; ++s)
             {
@@ -457,7 +541,14 @@ __global__ void __launch_bounds__(kBlock)
                 for (int q = 0; q < 4; ++q)
                     wd[q] = dual_prim(wv[q], (s < 24 && s / 6 == q) ? s % 6 : -1);
                 ConsT<Dual> f;
-                face_flux_local(wd, xld, xrd, M, a, gamma, o, D, f);
+                if (seed == SEED_SPIN)
+                {
+                    GeoT<Dual> face;
+                    face_geo(xld, xrd, M, a, face);
+                    face_flux_geo(wd, face, gamma, o, D, f);
+                }
+                else
+                    face_flux_geo(wd, face0, gamma, o, D, f);
                 const double g = dot_tangent(f, lam);
                 if (s < 24)
                 {
@@ -507,6 +598,11 @@ __global__ void __launch_bounds__(kBlock)
             double x[3];
             coords(i, j, k, b.dx, b.center, x);
             const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
+            /* the full geometry for every pass but the spin's */
+            FullGeo fgv;
+            source_geo(x, pa.M, pa.a, fgv);
+            FullGeoT<Dual> fg0;
+            dual_full_geo(fgv, fg0);
             double gW[6] = {0, 0, 0, 0, 0, 0};
             /* 6 W inputs, 5 U inputs, Gamma, a */
             for (int s = 0; s < 13; ++s)
@@ -517,7 +613,16 @@ __global__ void __launch_bounds__(kBlock)
                 const PrimT<Dual> wd = dual_prim(wv, s < 6 ? s : -1);
                 const ConsT<Dual> ud = dual_cons(uv, (s >= 6 && s < 11) ? s - 6 : -1);
                 ConsT<Dual> src;
-                if (!sources_local(wd, ud, xd, M, a, gamma, o, src))
+                bool live;
+                if (seed == SEED_SPIN)
+                {
+                    FullGeoT<Dual> fg;
+                    source_geo(xd, M, a, fg);
+                    live = sources_geo(wd, ud, fg, gamma, o, src);
+                }
+                else
+                    live = sources_geo(wd, ud, fg0, gamma, o, src);
+                if (!live)
                     break; /* skipped cell: no dependence */
                 const double g = dot_tangent(src, lam);
                 if (s < 6)
@@ -541,32 +646,84 @@ __global__ void __launch_bounds__(kBlock)
 /* The W of a storage cell as a function of the state (k_prims):
  * interior, con2prim on the cell's metric; outflow ghost, the chain
  * con2prim(source) -> floors(ghost metric) -> prim2con -> con2prim. */
-__device__ __forceinline__ bool w_of_state(const BoxDev &b, const Options &o,
-                                           Dual M, Dual a, Dual gamma,
-                                           int i, int j, int k,
-                                           const int src[3], bool physical,
-                                           const ConsT<Dual> &us,
-                                           PrimT<Dual> &w)
+/* The values of that chain, computed once per cell: the geometries of the
+ * cell and (ghost) of its source, and the roots of the one or two
+ * recoveries.  Returns false where a recovery fails. */
+struct WChain
+{
+    Geo gt, gs;
+    double p1, p2;
+};
+
+__device__ __forceinline__ bool w_chain_values(const BoxDev &b,
+                                               const Options &o,
+                                               const AdjParams &pa, int i,
+                                               int j, int k, const int src[3],
+                                               bool physical, const Cons &us,
+                                               WChain &v)
 {
     double xt[3];
     coords(i, j, k, b.dx, b.center, xt);
-    const Dual xtd[3] = {Dual(xt[0]), Dual(xt[1]), Dual(xt[2])};
-    GeoT<Dual> gt;
-    ks_geometry(M, a, xtd, gt);
+    ks_geometry(pa.M, pa.a, xt, v.gt);
     if (!physical)
-        return con2prim_ift(us, o, gamma, gt.gu, w);
+        return con2prim_root(us, o, pa.gamma, v.gt.gu, v.p1);
     double xs[3];
     coords(src[0], src[1], src[2], b.dx, b.center, xs);
-    const Dual xsd[3] = {Dual(xs[0]), Dual(xs[1]), Dual(xs[2])};
-    GeoT<Dual> gs;
-    ks_geometry(M, a, xsd, gs);
-    PrimT<Dual> pr;
-    if (!con2prim_ift(us, o, gamma, gs.gu, pr))
+    ks_geometry(pa.M, pa.a, xs, v.gs);
+    Options og = o;
+    og.gamma = pa.gamma;
+    Prim pr;
+    double res;
+    if (!con2prim(us, og, v.gs.gu, pr, res))
         return false;
+    v.p1 = pr.pressure;
+    floors_g(pr, o, pa.gamma, v.gt.g);
+    Cons cg;
+    prim2con(pr, pa.gamma, v.gt.g, cg);
+    return con2prim_root(cg, o, pa.gamma, v.gt.gu, v.p2);
+}
+
+/* One tangent pass of the chain at the roots of `v`; the geometry is
+ * re-evaluated with duals only when the seed is the spin. */
+__device__ __forceinline__ void w_of_state(const BoxDev &b, const Options &o,
+                                           Dual M, Dual a, Dual gamma,
+                                           int seed, int i, int j, int k,
+                                           const int src[3], bool physical,
+                                           const WChain &v,
+                                           const ConsT<Dual> &us,
+                                           PrimT<Dual> &w)
+{
+    GeoT<Dual> gt;
+    if (seed == SEED_SPIN)
+    {
+        double xt[3];
+        coords(i, j, k, b.dx, b.center, xt);
+        const Dual xtd[3] = {Dual(xt[0]), Dual(xt[1]), Dual(xt[2])};
+        ks_geometry(M, a, xtd, gt);
+    }
+    else
+        dual_geo(v.gt, gt);
+    if (!physical)
+    {
+        con2prim_at_root(us, o, gamma, gt.gu, v.p1, w);
+        return;
+    }
+    GeoT<Dual> gs;
+    if (seed == SEED_SPIN)
+    {
+        double xs[3];
+        coords(src[0], src[1], src[2], b.dx, b.center, xs);
+        const Dual xsd[3] = {Dual(xs[0]), Dual(xs[1]), Dual(xs[2])};
+        ks_geometry(M, a, xsd, gs);
+    }
+    else
+        dual_geo(v.gs, gs);
+    PrimT<Dual> pr;
+    con2prim_at_root(us, o, gamma, gs.gu, v.p1, pr);
     floors_g(pr, o, gamma, gt.g);
     ConsT<Dual> cg;
     prim2con(pr, gamma, gt.g, cg);
-    return con2prim_ift(cg, o, gamma, gt.gu, w);
+    con2prim_at_root(cg, o, gamma, gt.gu, v.p2, w);
 }
 
 /* ------------------------------------------------------------------ */
@@ -616,15 +773,18 @@ __global__ void __launch_bounds__(kBlock)
             const long long ss = goff(b, src[0], src[1], src[2]);
             const Cons uv = load_cons(U, n, ss);
             const long long is = iidx(b, src[0], src[1], src[2]);
-            for (int s = 0; s < 7; ++s)
+            WChain v;
+            const bool ok = w_chain_values(b, o, pa, i, j, k, src, physical,
+                                           uv, v);
+            for (int s = 0; ok && s < 7; ++s)
             {
                 const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
                 Dual M, a, gamma;
                 seed_params(pa, seed, M, a, gamma);
                 const ConsT<Dual> ud = dual_cons(uv, s < 5 ? s : -1);
                 PrimT<Dual> w;
-                if (!w_of_state(b, o, M, a, gamma, i, j, k, src, physical, ud, w))
-                    break;
+                w_of_state(b, o, M, a, gamma, seed, i, j, k, src, physical, v,
+                           ud, w);
                 double g = 0.0;
 #pragma unroll
                 for (int m = 0; m < 6; ++m)
