@@ -174,11 +174,13 @@ __device__ __forceinline__ bool con2prim_root(const Cons &cv,
     return true;
 }
 
-/* The primitives with tangents at a given root ps of the forward solve. */
+/* The primitives with tangents at a given root ps of the forward solve,
+ * on a metric of type G (double, or Dual for the spin's tangent). */
+template <class G>
 __device__ __forceinline__ void con2prim_at_root(const ConsT<Dual> &c,
                                                  const Options &o, Dual gamma,
-                                                 const Dual gu[3][3],
-                                                 double ps, PrimT<Dual> &out);
+                                                 const G gu[3][3], double ps,
+                                                 PrimT<Dual> &out);
 
 __device__ bool con2prim_ift(const ConsT<Dual> &c, const Options &o,
                              Dual gamma, const Dual gu[3][3],
@@ -203,10 +205,11 @@ __device__ bool con2prim_ift(const ConsT<Dual> &c, const Options &o,
     return true;
 }
 
+template <class G>
 __device__ __forceinline__ void con2prim_at_root(const ConsT<Dual> &c,
                                                  const Options &o, Dual gamma,
-                                                 const Dual gu[3][3],
-                                                 double ps, PrimT<Dual> &out)
+                                                 const G gu[3][3], double ps,
+                                                 PrimT<Dual> &out)
 {
     Dual S2 = 0.;
 #pragma unroll
@@ -257,51 +260,13 @@ enum
     SEED_NONE = -3
 };
 
-/* A double geometry as duals with zero tangents: the metric of every pass
- * whose seed is not the spin (the metric depends on a and x only), so the
- * Kerr-Schild evaluation runs once in double instead of once per pass. */
-__device__ __forceinline__ void dual_geo(const Geo &g, GeoT<Dual> &d)
-{
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-    {
-        d.shift[i] = Dual(g.shift[i]);
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-        {
-            d.g[i][j] = Dual(g.g[i][j]);
-            d.gu[i][j] = Dual(g.gu[i][j]);
-        }
-    }
-    d.lapse = Dual(g.lapse);
-}
-
-__device__ __forceinline__ void dual_full_geo(const FullGeo &g,
-                                              FullGeoT<Dual> &d)
-{
-    dual_geo(g.geo, d.geo);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-    {
-        d.dlapse[i] = Dual(g.dlapse[i]);
-        d.dlogsqrtdet[i] = Dual(g.dlogsqrtdet[i]);
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-        {
-            d.K[i][j] = Dual(g.K[i][j]);
-            d.dshift[i][j] = Dual(g.dshift[i][j]);
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                d.dg[i][j][k] = Dual(g.dg[i][j][k]);
-        }
-    }
-}
-
 /* Face flux of direction D from the W of cells L-1, L, R, R+1 along D on a
- * given face geometry (k_faces_x / k_faces<D> after the metric). */
-template <class T>
+ * given face geometry (k_faces_x / k_faces<D> after the metric).  The
+ * metric is double in every pass but the spin's: it depends on a and x
+ * only, so its tangent is zero there and it is evaluated once. */
+template <class T, class G>
 __device__ __forceinline__ void face_flux_geo(const PrimT<T> w[4],
-                                              const GeoT<T> &face, T gamma,
+                                              const GeoT<G> &face, T gamma,
                                               const Options &o, int D,
                                               ConsT<T> &f)
 {
@@ -313,6 +278,29 @@ __device__ __forceinline__ void face_flux_geo(const PrimT<T> w[4],
         slope_step(w[1], w[2], w[3], T(o.theta), rp, unused);
     }
     hlle_face_g(lp, rp, o, gamma, face, D, f);
+}
+
+/* Component m of a primitive state, in W's order (rho, eps, P, v). */
+__device__ __forceinline__ double prim_comp(const Prim &w, int m)
+{
+    return m == 0 ? w.rho : (m == 1 ? w.eps : (m == 2 ? w.pressure : w.v[m - 3]));
+}
+
+/* d(c + h * limited_slope(l, c, r, theta)) / d(l, c, r): the reconstruction
+ * of one component (slope_step, h = +1/2 for the plus half, -1/2 for the
+ * minus half), with the limiter's branch frozen. */
+__device__ __forceinline__ void half_step_jacobian(double l, double c,
+                                                   double r, double theta,
+                                                   double h, double j[3])
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+    {
+        const Dual L(l, q == 0 ? 1.0 : 0.0), C(c, q == 1 ? 1.0 : 0.0),
+            R(r, q == 2 ? 1.0 : 0.0);
+        const Dual out = C + Dual(h) * limited_slope(L, C, R, Dual(theta));
+        j[q] = out.d;
+    }
 }
 
 /* The face geometry: mean of the geometries of cells L and R. */
@@ -340,10 +328,10 @@ __device__ __forceinline__ void source_geo(const T x[3], T M, T a,
 /* Static-metric sources of an interior cell (k_cells) on a given full
  * geometry: floors on W, the atmosphere skip, then the sources.  Returns
  * false where skipped. */
-template <class T>
+template <class T, class G>
 __device__ __forceinline__ bool sources_geo(const PrimT<T> &w,
                                             const ConsT<T> &u,
-                                            const FullGeoT<T> &fg, T gamma,
+                                            const FullGeoT<G> &fg, T gamma,
                                             const Options &o, ConsT<T> &src)
 {
     PrimT<T> pw = w;
@@ -473,7 +461,7 @@ __device__ __forceinline__ bool interior(const BoxDev &b, int i, int j, int k)
 /* four stencil cells (atomics) and into the parameter adjoints.        */
 /* One thread per face of the interior face box.                       */
 /* ------------------------------------------------------------------ */
-template <int D>
+template <int D, bool SPIN>
 __global__ void __launch_bounds__(kBlock)
     k_adj_faces(BoxDev b, Options o, AdjParams pa, const double *W,
                 const double *lam_k, double *lam_W, double *lam_par)
@@ -525,40 +513,81 @@ __global__ void __launch_bounds__(kBlock)
             coords(c[0], c[1], c[2], b.dx, b.center, xr);
             const Dual xld[3] = {Dual(xl[0]), Dual(xl[1]), Dual(xl[2])};
             const Dual xrd[3] = {Dual(xr[0]), Dual(xr[1]), Dual(xr[2])};
-            /* the face metric for every pass but the spin's */
-            Geo facev;
-            face_geo(xl, xr, pa.M, pa.a, facev);
-            GeoT<Dual> face0;
-            dual_geo(facev, face0);
-            /* 24 W inputs, then Gamma and a */
-            for (int s = 0; s < 26; ++s)
+            if constexpr (SPIN)
             {
-                const int seed = s < 24 ? 0 : (s == 24 ? SEED_GAMMA : SEED_SPIN);
+                /* the spin's tangent: the face metric with duals */
                 Dual M, a, gamma;
-                seed_params(pa, seed, M, a, gamma);
+                seed_params(pa, SEED_SPIN, M, a, gamma);
                 PrimT<Dual> wd[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    wd[q] = dual_prim(wv[q], (s < 24 && s / 6 == q) ? s % 6 : -1);
+                    wd[q] = dual_prim(wv[q], -1);
+                GeoT<Dual> face;
+                face_geo(xld, xrd, M, a, face);
                 ConsT<Dual> f;
-                if (seed == SEED_SPIN)
+                face_flux_geo(wd, face, gamma, o, D, f);
+                g1 = dot_tangent(f, lam);
+            }
+            else
+            {
+                /* the face metric in double: no pass here seeds it */
+                Geo facev;
+                face_geo(xl, xr, pa.M, pa.a, facev);
+                /* the face states (values), as k_faces forms them */
+                Prim lpv = wv[1], rpv = wv[2];
+                if (o.use_reconstruction)
                 {
-                    GeoT<Dual> face;
-                    face_geo(xld, xrd, M, a, face);
-                    face_flux_geo(wd, face, gamma, o, D, f);
+                    Prim unused;
+                    slope_step(wv[0], wv[1], wv[2], o.theta, unused, lpv);
+                    slope_step(wv[1], wv[2], wv[3], o.theta, rpv, unused);
                 }
-                else
-                    face_flux_geo(wd, face0, gamma, o, D, f);
-                const double g = dot_tangent(f, lam);
-                if (s < 24)
+                /* lambda . dF/d(face states): 6 + 6 passes, then Gamma */
+                double lam_lp[6], lam_rp[6];
+                for (int s = 0; s < 13; ++s)
                 {
-                    if (g != 0.0)
-                        atomicAdd(&lam_W[(long long)(s % 6) * n + so[s / 6]], g);
+                    Dual M, a, gamma;
+                    seed_params(pa, s < 12 ? 0 : SEED_GAMMA, M, a, gamma);
+                    const PrimT<Dual> ld = dual_prim(lpv, s < 6 ? s : -1);
+                    const PrimT<Dual> rd =
+                        dual_prim(rpv, (s >= 6 && s < 12) ? s - 6 : -1);
+                    ConsT<Dual> f;
+                    hlle_face_g(ld, rd, o, gamma, facev, D, f);
+                    const double g = dot_tangent(f, lam);
+                    if (s < 6)
+                        lam_lp[s] = g;
+                    else if (s < 12)
+                        lam_rp[s - 6] = g;
+                    else
+                        g0 += g;
                 }
-                else if (s == 24)
-                    g0 += g;
-                else
-                    g1 += g;
+                /* back through the reconstruction, component by component:
+                 * lp = W_L + s_L/2 from (W_{L-1}, W_L, W_R), rp = W_R - s_R/2
+                 * from (W_L, W_R, W_{R+1}) */
+#pragma unroll
+                for (int m = 0; m < 6; ++m)
+                {
+                    double gq[4] = {0.0, lam_lp[m], lam_rp[m], 0.0};
+                    if (o.use_reconstruction)
+                    {
+                        double jl[3], jr[3];
+                        half_step_jacobian(prim_comp(wv[0], m),
+                                           prim_comp(wv[1], m),
+                                           prim_comp(wv[2], m), o.theta, 0.5,
+                                           jl);
+                        half_step_jacobian(prim_comp(wv[1], m),
+                                           prim_comp(wv[2], m),
+                                           prim_comp(wv[3], m), o.theta, -0.5,
+                                           jr);
+                        gq[0] = lam_lp[m] * jl[0];
+                        gq[1] = lam_lp[m] * jl[1] + lam_rp[m] * jr[0];
+                        gq[2] = lam_lp[m] * jl[2] + lam_rp[m] * jr[1];
+                        gq[3] = lam_rp[m] * jr[2];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (gq[q] != 0.0)
+                            atomicAdd(&lam_W[(long long)m * n + so[q]], gq[q]);
+                }
             }
         }
     }
@@ -569,6 +598,7 @@ __global__ void __launch_bounds__(kBlock)
 /* k_adj_sources: sources of each interior cell: lambda_k -> lambda_W   */
 /* (6), lambda_U (5, added), parameter adjoints.                        */
 /* ------------------------------------------------------------------ */
+template <bool SPIN>
 __global__ void __launch_bounds__(kBlock)
     k_adj_sources(BoxDev b, Options o, AdjParams pa, const double *U,
                   const double *W, const double *lam_k, double *lam_W,
@@ -598,46 +628,49 @@ __global__ void __launch_bounds__(kBlock)
             double x[3];
             coords(i, j, k, b.dx, b.center, x);
             const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
-            /* the full geometry for every pass but the spin's */
-            FullGeo fgv;
-            source_geo(x, pa.M, pa.a, fgv);
-            FullGeoT<Dual> fg0;
-            dual_full_geo(fgv, fg0);
-            double gW[6] = {0, 0, 0, 0, 0, 0};
-            /* 6 W inputs, 5 U inputs, Gamma, a */
-            for (int s = 0; s < 13; ++s)
+            if constexpr (SPIN)
             {
-                const int seed = s < 11 ? 0 : (s == 11 ? SEED_GAMMA : SEED_SPIN);
+                /* the spin's tangent: the full geometry with duals */
                 Dual M, a, gamma;
-                seed_params(pa, seed, M, a, gamma);
-                const PrimT<Dual> wd = dual_prim(wv, s < 6 ? s : -1);
-                const ConsT<Dual> ud = dual_cons(uv, (s >= 6 && s < 11) ? s - 6 : -1);
+                seed_params(pa, SEED_SPIN, M, a, gamma);
+                FullGeoT<Dual> fg;
+                source_geo(xd, M, a, fg);
                 ConsT<Dual> src;
-                bool live;
-                if (seed == SEED_SPIN)
-                {
-                    FullGeoT<Dual> fg;
-                    source_geo(xd, M, a, fg);
-                    live = sources_geo(wd, ud, fg, gamma, o, src);
-                }
-                else
-                    live = sources_geo(wd, ud, fg0, gamma, o, src);
-                if (!live)
-                    break; /* skipped cell: no dependence */
-                const double g = dot_tangent(src, lam);
-                if (s < 6)
-                    gW[s] = g;
-                else if (s < 11)
-                    lam_U[(long long)(s - 6) * ni + it] += g;
-                else if (s == 11)
-                    g0 += g;
-                else
-                    g1 += g;
+                if (sources_geo(dual_prim(wv, -1), dual_cons(uv, -1), fg,
+                                gamma, o, src))
+                    g1 = dot_tangent(src, lam);
             }
+            else
+            {
+                /* the full geometry in double: no pass here seeds it */
+                FullGeo fgv;
+                source_geo(x, pa.M, pa.a, fgv);
+                double gW[6] = {0, 0, 0, 0, 0, 0};
+                /* 6 W inputs, 5 U inputs, Gamma */
+                for (int s = 0; s < 12; ++s)
+                {
+                    const int seed = s < 11 ? 0 : SEED_GAMMA;
+                    Dual M, a, gamma;
+                    seed_params(pa, seed, M, a, gamma);
+                    const PrimT<Dual> wd = dual_prim(wv, s < 6 ? s : -1);
+                    const ConsT<Dual> ud =
+                        dual_cons(uv, (s >= 6 && s < 11) ? s - 6 : -1);
+                    ConsT<Dual> src;
+                    if (!sources_geo(wd, ud, fgv, gamma, o, src))
+                        break; /* skipped cell: no dependence */
+                    const double g = dot_tangent(src, lam);
+                    if (s < 6)
+                        gW[s] = g;
+                    else if (s < 11)
+                        lam_U[(long long)(s - 6) * ni + it] += g;
+                    else
+                        g0 += g;
+                }
 #pragma unroll
-            for (int m = 0; m < 6; ++m)
-                if (gW[m] != 0.0)
-                    atomicAdd(&lam_W[m * n + so], gW[m]);
+                for (int m = 0; m < 6; ++m)
+                    if (gW[m] != 0.0)
+                        atomicAdd(&lam_W[m * n + so], gW[m]);
+            }
         }
     }
     block_add_params(lam_par, g0, g1);
@@ -683,41 +716,22 @@ __device__ __forceinline__ bool w_chain_values(const BoxDev &b,
     return con2prim_root(cg, o, pa.gamma, v.gt.gu, v.p2);
 }
 
-/* One tangent pass of the chain at the roots of `v`; the geometry is
- * re-evaluated with duals only when the seed is the spin. */
-__device__ __forceinline__ void w_of_state(const BoxDev &b, const Options &o,
-                                           Dual M, Dual a, Dual gamma,
-                                           int seed, int i, int j, int k,
-                                           const int src[3], bool physical,
-                                           const WChain &v,
+/* One tangent pass of the chain at the roots of `v` on the geometries
+ * gt (the cell) and gs (its source): the double ones of `v`, or dual ones
+ * for the spin's tangent. */
+template <class G>
+__device__ __forceinline__ void w_of_state(const GeoT<G> &gt,
+                                           const GeoT<G> &gs, Dual gamma,
+                                           bool physical, const WChain &v,
+                                           const Options &o,
                                            const ConsT<Dual> &us,
                                            PrimT<Dual> &w)
 {
-    GeoT<Dual> gt;
-    if (seed == SEED_SPIN)
-    {
-        double xt[3];
-        coords(i, j, k, b.dx, b.center, xt);
-        const Dual xtd[3] = {Dual(xt[0]), Dual(xt[1]), Dual(xt[2])};
-        ks_geometry(M, a, xtd, gt);
-    }
-    else
-        dual_geo(v.gt, gt);
     if (!physical)
     {
         con2prim_at_root(us, o, gamma, gt.gu, v.p1, w);
         return;
     }
-    GeoT<Dual> gs;
-    if (seed == SEED_SPIN)
-    {
-        double xs[3];
-        coords(src[0], src[1], src[2], b.dx, b.center, xs);
-        const Dual xsd[3] = {Dual(xs[0]), Dual(xs[1]), Dual(xs[2])};
-        ks_geometry(M, a, xsd, gs);
-    }
-    else
-        dual_geo(v.gs, gs);
     PrimT<Dual> pr;
     con2prim_at_root(us, o, gamma, gs.gu, v.p1, pr);
     floors_g(pr, o, gamma, gt.g);
@@ -731,6 +745,7 @@ __device__ __forceinline__ void w_of_state(const BoxDev &b, const Options &o,
 /* state it was recovered from (interior: itself; outflow ghost: its    */
 /* clamped source, atomics), plus the parameter adjoints.               */
 /* ------------------------------------------------------------------ */
+template <bool SPIN>
 __global__ void __launch_bounds__(kBlock)
     k_adj_prims(BoxDev b, Options o, AdjParams pa, const double *U,
                 const double *lam_W, double *lam_U, double *lam_par)
@@ -776,15 +791,32 @@ __global__ void __launch_bounds__(kBlock)
             WChain v;
             const bool ok = w_chain_values(b, o, pa, i, j, k, src, physical,
                                            uv, v);
-            for (int s = 0; ok && s < 7; ++s)
+            /* SPIN: the spin's pass alone; otherwise 5 U inputs and Gamma */
+            const int s0 = SPIN ? 6 : 0, s1 = SPIN ? 7 : 6;
+            for (int s = s0; ok && s < s1; ++s)
             {
                 const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
                 Dual M, a, gamma;
                 seed_params(pa, seed, M, a, gamma);
                 const ConsT<Dual> ud = dual_cons(uv, s < 5 ? s : -1);
                 PrimT<Dual> w;
-                w_of_state(b, o, M, a, gamma, seed, i, j, k, src, physical, v,
-                           ud, w);
+                if constexpr (SPIN)
+                {
+                    double xt[3], xs[3];
+                    coords(i, j, k, b.dx, b.center, xt);
+                    coords(src[0], src[1], src[2], b.dx, b.center, xs);
+                    const Dual xtd[3] = {Dual(xt[0]), Dual(xt[1]),
+                                         Dual(xt[2])};
+                    const Dual xsd[3] = {Dual(xs[0]), Dual(xs[1]),
+                                         Dual(xs[2])};
+                    GeoT<Dual> gt, gs;
+                    ks_geometry(M, a, xtd, gt);
+                    if (physical)
+                        ks_geometry(M, a, xsd, gs);
+                    w_of_state(gt, gs, gamma, physical, v, o, ud, w);
+                }
+                else
+                    w_of_state(v.gt, v.gs, gamma, physical, v, o, ud, w);
                 double g = 0.0;
 #pragma unroll
                 for (int m = 0; m < 6; ++m)

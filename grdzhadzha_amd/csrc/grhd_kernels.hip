@@ -1116,17 +1116,38 @@ void adj_rhs_transpose(AdjRun &r, const double *X, double *lam_k,
     const long long nf[3] = {(long long)(b.in[0] + 1) * b.in[1] * b.in[2],
                              (long long)b.in[0] * (b.in[1] + 1) * b.in[2],
                              (long long)b.in[0] * b.in[1] * (b.in[2] + 1)};
-    hipLaunchKernelGGL(k_adj_faces<0>, dim3(grid_of(nf[0])), dim3(kBlock), 0,
-                       r.s, b, r.o, r.pa, r.c->W, lam_k, lam_W, lam_par);
-    hipLaunchKernelGGL(k_adj_faces<1>, dim3(grid_of(nf[1])), dim3(kBlock), 0,
-                       r.s, b, r.o, r.pa, r.c->W, lam_k, lam_W, lam_par);
-    hipLaunchKernelGGL(k_adj_faces<2>, dim3(grid_of(nf[2])), dim3(kBlock), 0,
-                       r.s, b, r.o, r.pa, r.c->W, lam_k, lam_W, lam_par);
-    hipLaunchKernelGGL(k_adj_sources, dim3(grid_of(b.inpts)), dim3(kBlock), 0,
-                       r.s, b, r.o, r.pa, r.work, r.c->W, lam_k, lam_W, lam_X,
-                       lam_par);
-    hipLaunchKernelGGL(k_adj_prims, dim3(grid_of(b.gnpts)), dim3(kBlock), 0,
-                       r.s, b, r.o, r.pa, r.work, lam_W, lam_X, lam_par);
+    /* each local Jacobian in two launches: the passes on the double metric,
+     * and the spin's pass, whose dual metric needs far more registers */
+    hipLaunchKernelGGL((k_adj_faces<0, false>), dim3(grid_of(nf[0])),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       lam_W, lam_par);
+    hipLaunchKernelGGL((k_adj_faces<0, true>), dim3(grid_of(nf[0])),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       lam_W, lam_par);
+    hipLaunchKernelGGL((k_adj_faces<1, false>), dim3(grid_of(nf[1])),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       lam_W, lam_par);
+    hipLaunchKernelGGL((k_adj_faces<1, true>), dim3(grid_of(nf[1])),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       lam_W, lam_par);
+    hipLaunchKernelGGL((k_adj_faces<2, false>), dim3(grid_of(nf[2])),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       lam_W, lam_par);
+    hipLaunchKernelGGL((k_adj_faces<2, true>), dim3(grid_of(nf[2])),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       lam_W, lam_par);
+    hipLaunchKernelGGL(k_adj_sources<false>, dim3(grid_of(b.inpts)),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
+                       lam_k, lam_W, lam_X, lam_par);
+    hipLaunchKernelGGL(k_adj_sources<true>, dim3(grid_of(b.inpts)),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
+                       lam_k, lam_W, lam_X, lam_par);
+    hipLaunchKernelGGL(k_adj_prims<false>, dim3(grid_of(b.gnpts)),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, lam_W,
+                       lam_X, lam_par);
+    hipLaunchKernelGGL(k_adj_prims<true>, dim3(grid_of(b.gnpts)),
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, lam_W,
+                       lam_X, lam_par);
 }
 
 int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,

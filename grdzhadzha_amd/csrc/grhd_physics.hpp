@@ -674,8 +674,11 @@ __device__ __forceinline__ double log_sqrt_det(const Geo &g)
 }
 
 /* ---------------- Valencia (Valencia.hpp) ---------------- */
-template <class T>
-__device__ __forceinline__ T vsq(const T v[3], const T g[3][3])
+/* The geometry-taking functions below have a second type G for the metric
+ * (default T): the adjoint evaluates fluid tangents on a metric held in
+ * double. */
+template <class T, class G = T>
+__device__ __forceinline__ T vsq(const T v[3], const G g[3][3])
 {
     T s = T(0);
 #pragma unroll
@@ -687,9 +690,9 @@ __device__ __forceinline__ T vsq(const T v[3], const T g[3][3])
 }
 
 /* compute_conserved (Valencia.hpp:55-73) */
-template <class T>
+template <class T, class G = T>
 __device__ __forceinline__ void prim2con(const PrimT<T> &p, T gamma,
-                                         const T g[3][3], ConsT<T> &c)
+                                         const G g[3][3], ConsT<T> &c)
 {
     (void)gamma;
     T vL[3];
@@ -840,9 +843,9 @@ __device__ __forceinline__ bool con2prim(const Cons &c, const Options &o,
 }
 
 /* compute_flux (Valencia.hpp:230-249) */
-template <class T>
+template <class T, class G = T>
 __device__ __forceinline__ void flux_of(const PrimT<T> &p, const ConsT<T> &c,
-                                        int d, T lapse, const T shift[3],
+                                        int d, G lapse, const G shift[3],
                                         ConsT<T> &f)
 {
     const T tv = lapse * p.v[d] - shift[d];
@@ -858,9 +861,9 @@ __device__ __forceinline__ void flux_of(const PrimT<T> &p, const ConsT<T> &c,
 }
 
 /* ---------------- Atmosphere (Atmosphere.hpp:24-60) ---------------- */
-template <class T>
+template <class T, class G = T>
 __device__ __forceinline__ void floors_g(PrimT<T> &p, const Options &o,
-                                         T gamma, const T g[3][3])
+                                         T gamma, const G g[3][3])
 {
     p.rho = std_max(p.rho, T(o.rho_floor));
     p.pressure = std_max(p.pressure, T(o.pressure_floor));
@@ -921,9 +924,9 @@ __device__ __forceinline__ T limited_slope(T l, T c, T r, T theta)
 }
 
 /* ---------------- Riemann solver (RiemannSolvers.hpp:23-118) ---------------- */
-template <class T>
+template <class T, class G = T>
 __device__ __forceinline__ void char_speeds(const PrimT<T> &p, T gamma,
-                                            const GeoT<T> &g, int d, T &lower,
+                                            const GeoT<G> &g, int d, T &lower,
                                             T &upper)
 {
     const T v2 = vsq(p.v, g.g);
@@ -943,10 +946,10 @@ __device__ __forceinline__ void char_speeds(const PrimT<T> &p, T gamma,
 
 /* compute_muscl_hlle_flux (FiniteVolume.hpp:63-98) after the MUSCL step:
  * floors, prim2con and HLLE on the face geometry. */
-template <class T>
+template <class T, class G = T>
 __device__ __forceinline__ void hlle_face_g(PrimT<T> lp, PrimT<T> rp,
                                             const Options &o, T gamma,
-                                            const GeoT<T> &face, int d,
+                                            const GeoT<G> &face, int d,
                                             ConsT<T> &f)
 {
     floors_g(lp, o, gamma, face.g);
@@ -1011,13 +1014,13 @@ __device__ __forceinline__ void average_geo(const GeoT<T> &l, const GeoT<T> &r,
 }
 
 /* ---------------- Static-metric sources (StaticMetric.hpp:193-275) ------- */
-template <class T>
+template <class T, class G = T>
 __device__ __forceinline__ void static_sources(const PrimT<T> &p,
                                                const ConsT<T> &c, T gamma,
-                                               const FullGeoT<T> &f,
+                                               const FullGeoT<G> &f,
                                                ConsT<T> &src)
 {
-    const GeoT<T> &g = f.geo;
+    const GeoT<G> &g = f.geo;
     const T W = 1.0 / sqrt(1.0 - vsq(p.v, g.g));
     const T h = 1.0 + p.eps + p.pressure / p.rho;
     const T rhW2 = p.rho * h * W * W;
