@@ -44,6 +44,11 @@ FP64_PEAK_TFLOPS = 78.6
 #   prims:   per exterior cell: one state read, state + W written.
 ALG_BYTES = {"faces_x": 88, "faces_y": 88, "faces_z": 88,
              "cells": 120 + 48 + 120 + 40 + 40 + 48, "prims": 40 + 40 + 48}
+# Whole RK4 step, per cell-update: FP64 state traffic of a fused RK4 (stage
+# 1: U0 read, U1 and acc written, 120 B; stages 2 and 3: U_s, U0, acc read,
+# U_{s+1}, acc written, 200 B each; stage 4: U3, acc read, U written,
+# 120 B), plus the primitives (48 B) and the mask byte.
+STEP_ALG_BYTES = 120 + 200 + 200 + 120 + 48 + 1
 
 
 def units_per_launch(cls, n, g):
@@ -98,6 +103,58 @@ def cpu_baseline(w, steps=3):
                       "from the same initial state, %.1f s wall, OpenMP over "
                       "cells, reference call structure"
                       % (steps, w.box.interior_shape[0], cells, t1 - t0)}
+
+
+def bench_c1(args):
+    """The reference's own CPU-runnable case: the 32^3 Michel-like box on
+    [-8, 8]^3 (Schwarzschild, MC limiter, dt = 0.25 dx), plain FP64 RK4 on
+    the GPU, and the oracle on this host with all its threads and with one.
+    At 32,768 cells the GPU is launch-latency bound, not compute bound.
+    Returns the JSON record."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from grdzhadzha_amd import workloads
+    from grdzhadzha_amd.operator import FixedBGLevelDataFiniteVolumeOperator
+
+    w = workloads.michel_box_32()
+    cells = w.box.num_interior
+    op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, w.options)
+    init = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
+    steps = 200
+    op.advance_rk4(init.clone(), w.dt, 2)
+    st = init.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.advance_rk4(st, w.dt, steps)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    O = oracle.Oracle(w.background, w.box, w.options)
+    full = oracle.num_threads()
+
+    def cpu(k):
+        s = np.ascontiguousarray(w.state).copy()
+        t = time.perf_counter()
+        O.advance_rk4(s, w.dt, k)
+        return cells * k / (time.perf_counter() - t)
+
+    c_all = cpu(20)
+    oracle.set_num_threads(1)
+    try:
+        c_one = cpu(3)
+    finally:
+        oracle.set_num_threads(full)
+    print("bench: C1 32^3 Michel GPU %.3e cell-updates/s (%.3f ms/step); "
+          "oracle %.3e (%d threads), %.3e (1 thread)"
+          % (cells * steps / tg, tg / steps * 1e3, c_all, full, c_one),
+          file=sys.stderr, flush=True)
+    return {"workload": "C1 michel-like 32^3 on [-8,8]^3 Schwarzschild, MC, "
+                        "plain FP64 RK4",
+            "interior_cells": cells, "steps": steps,
+            "value": cells * steps / tg, "ms_per_step": tg / steps * 1e3,
+            "cpu_oracle": {"value": c_all, "cores": full, "steps": 20},
+            "cpu_oracle_1_thread": {"value": c_one, "cores": 1, "steps": 3}}
 
 
 def calibrate_copy():
@@ -170,27 +227,36 @@ def bench_batched(args):
         torch.cuda.synchronize()
         return pools, time.perf_counter() - t0
 
+    names = ("D", "S1", "S2", "S3", "tau")
+
+    def level_errors(p64, p32):
+        num = {k: 0.0 for k in names}
+        den = {k: 0.0 for k in names}
+        mx = {k: 0.0 for k in names}
+        mref = {k: 0.0 for k in names}
+        for lev, a64, a32, (boxes, _, _) in zip(ops, p64, p32, levels):
+            for b, f64, f32 in zip(boxes, lev.unpack(a64), lev.unpack(a32)):
+                sl = b.interior_slices()
+                for c, name in enumerate(names):
+                    ref = f64[c][sl]
+                    diff = abs(f32[c][sl] - ref)
+                    num[name] += float(diff.sum())
+                    den[name] += float(abs(ref).sum())
+                    mx[name] = max(mx[name], float(diff.max()))
+                    mref[name] = max(mref[name], float(abs(ref).max()))
+        return {k: {"l1_rel": num[k] / den[k], "linf_rel": mx[k] / mref[k]}
+                for k in names}
+
+    # one step: the FP32 rounding of a single step, before branch decisions
+    # (resets, floors, atmosphere) can come out differently
+    err_one = {k: v["l1_rel"] for k, v in
+               level_errors(run_levels(ops, 1)[0],
+                            run_levels(ops32, 1)[0]).items()}
     run_levels(ops32, args.warmup or 1)
     p32, tm = run_levels(ops32, steps)
     p64, _ = run_levels(ops, steps)
     mixed = cells * steps / tm
-    names = ("D", "S1", "S2", "S3", "tau")
-    num = {k: 0.0 for k in names}
-    den = {k: 0.0 for k in names}
-    mx = {k: 0.0 for k in names}
-    mref = {k: 0.0 for k in names}
-    for lev, a64, a32, (boxes, _, _) in zip(ops, p64, p32, levels):
-        for b, f64, f32 in zip(boxes, lev.unpack(a64), lev.unpack(a32)):
-            sl = b.interior_slices()
-            for c, name in enumerate(names):
-                ref = f64[c][sl]
-                diff = abs(f32[c][sl] - ref)
-                num[name] += float(diff.sum())
-                den[name] += float(abs(ref).sum())
-                mx[name] = max(mx[name], float(diff.max()))
-                mref[name] = max(mref[name], float(abs(ref).max()))
-    err_m = {k: {"l1_rel": num[k] / den[k], "linf_rel": mx[k] / mref[k]}
-             for k in names}
+    err_m = level_errors(p64, p32)
     del p32, p64
     print("bench: batched mixed %.3f s (%.3e cell-updates/s), max L1 rel "
           "error %.3e" % (tm, mixed, max(e["l1_rel"] for e in
@@ -235,7 +301,8 @@ def bench_batched(args):
                 "value": mixed, "ms_per_step": tm / steps * 1e3,
                 "speedup_vs_plain_per_box": mixed / plain,
                 "speedup_vs_batched_fp64": mixed / batched,
-                "error_vs_batched_fp64": err_m}}
+                "error_vs_batched_fp64": err_m,
+                "l1_rel_after_one_step": err_one}}
 
 
 def bench_mixed(args):
@@ -450,8 +517,10 @@ def main():
     ap.add_argument("--no-adjoint-checks", action="store_true",
                     help="adjoint record without the finite-difference "
                          "checks and the run without excision (profiling)")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 32^3 Michel (C1) record of the default run")
     ap.add_argument("--workload", choices=["torus", "batched", "mixed",
-                                           "adjoint"],
+                                           "adjoint", "c1"],
                     default="torus",
                     help="torus: the 128^3 plain FP64 box (default); "
                          "batched: the 3-level binary-PN box set; mixed: "
@@ -487,6 +556,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps(bench_adjoint(args)), flush=True)
+        return
+    if args.workload == "c1":
+        print(json.dumps(bench_c1(args)), flush=True)
         return
 
     import numpy as np
@@ -556,7 +628,10 @@ def main():
                       "3 ghosts, minmod, dt = 0.25 dx)",
                       "interior_cells": cells, "ghosts": w.box.ghosts,
                       "dt": w.dt, "parallelism": "single GPU"},
-           "roofline": roofline}
+           "roofline": roofline,
+           "step_hbm": {"alg_bytes_per_cell_update": STEP_ALG_BYTES,
+                        "achieved_gbs": value * STEP_ALG_BYTES / 1e9,
+                        "frac": value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS}}
     if not args.no_batched:
         del op, st, init
         torch.cuda.empty_cache()
@@ -576,6 +651,8 @@ def main():
     if not args.no_adjoint:
         torch.cuda.empty_cache()
         out["adjoint"] = bench_adjoint(args)
+    if not args.no_c1:
+        out["c1"] = bench_c1(args)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on 3 RK4 steps ...",
               file=sys.stderr, flush=True)

@@ -62,6 +62,7 @@ def lib():
             "grhd_oracle_log_sqrt_det_dt": (d, [Bg, d, vp, d, vp, d]),
             "grhd_oracle_log": (d, [d]),
             "grhd_oracle_num_threads": (i, []),
+            "grhd_oracle_set_num_threads": (None, [i]),
             "grhd_oracle_ks_excised": (i, [d, d, vp, d, d]),
             "grhd_oracle_leveldata_conserved_sum": (None, [B, i, P(vp), i, i,
                                                            P(d)]),
@@ -302,6 +303,11 @@ def minmod(a, b):
 
 def num_threads():
     return lib().grhd_oracle_num_threads()
+
+
+def set_num_threads(n):
+    """OpenMP threads of the oracle's parallel loops (omp_set_num_threads)."""
+    lib().grhd_oracle_set_num_threads(int(n))
 
 
 def ks_excised(M, a, x, inner_buffer=1.05, outer_buffer=0.9):

@@ -2244,6 +2244,8 @@ double grhd_oracle_log_sqrt_det_dt(const grhd_background *bgp, double time,
 
 int grhd_oracle_num_threads(void) { return omp_get_max_threads(); }
 
+void grhd_oracle_set_num_threads(int n) { omp_set_num_threads(n); }
+
 /* compute_leveldata_conserved_sum (LevelDataDriver.hpp:22-57): one
  * sequential accumulator per component over boxes, then BoxIterator order. */
 void grhd_oracle_leveldata_conserved_sum(const grhd_box *boxes, int nbox,

@@ -78,6 +78,7 @@ double grhd_oracle_log_sqrt_det_dt(const grhd_background *bg, double time,
                                    const double center[3], double step);
 double grhd_oracle_log(double x);
 int grhd_oracle_num_threads(void);
+void grhd_oracle_set_num_threads(int n);
 /* KerrSchild::check_if_excised (KerrSchild.hpp:247-280) */
 int grhd_oracle_ks_excised(double M, double a, const double x[3],
                            double inner_buffer, double outer_buffer);
