@@ -679,41 +679,40 @@ __global__ void __launch_bounds__(kBlock)
 /* The W of a storage cell as a function of the state (k_prims):
  * interior, con2prim on the cell's metric; outflow ghost, the chain
  * con2prim(source) -> floors(ghost metric) -> prim2con -> con2prim. */
-/* The values of that chain, computed once per cell: the geometries of the
- * cell and (ghost) of its source, and the roots of the one or two
- * recoveries.  Returns false where a recovery fails. */
+/* The values of that chain for one cell: the geometries of the cell and
+ * (ghost) of its source, and the roots of the one or two recoveries.  The
+ * roots are the pressures k_prims just wrote into W for this state: the
+ * cell's own (interior: con2prim of U; ghost: the chain's last recovery)
+ * and, for a ghost, its source cell's (the chain's first recovery, the
+ * same solve on the same inputs).  The adjoint runs only after a forward
+ * sweep in which every recovery succeeded. */
 struct WChain
 {
     Geo gt, gs;
     double p1, p2;
 };
 
-__device__ __forceinline__ bool w_chain_values(const BoxDev &b,
-                                               const Options &o,
-                                               const AdjParams &pa, int i,
-                                               int j, int k, const int src[3],
-                                               bool physical, const Cons &us,
-                                               WChain &v)
+__device__ __forceinline__ void w_chain_values(const BoxDev &b,
+                                               const AdjParams &pa,
+                                               const double *W, int i, int j,
+                                               int k, const int src[3],
+                                               bool physical, WChain &v)
 {
+    const long long n = b.gnpts;
     double xt[3];
     coords(i, j, k, b.dx, b.center, xt);
     ks_geometry(pa.M, pa.a, xt, v.gt);
+    const double p_own = W[2 * n + goff(b, i, j, k)];
     if (!physical)
-        return con2prim_root(us, o, pa.gamma, v.gt.gu, v.p1);
+    {
+        v.p1 = p_own;
+        return;
+    }
     double xs[3];
     coords(src[0], src[1], src[2], b.dx, b.center, xs);
     ks_geometry(pa.M, pa.a, xs, v.gs);
-    Options og = o;
-    og.gamma = pa.gamma;
-    Prim pr;
-    double res;
-    if (!con2prim(us, og, v.gs.gu, pr, res))
-        return false;
-    v.p1 = pr.pressure;
-    floors_g(pr, o, pa.gamma, v.gt.g);
-    Cons cg;
-    prim2con(pr, pa.gamma, v.gt.g, cg);
-    return con2prim_root(cg, o, pa.gamma, v.gt.gu, v.p2);
+    v.p1 = W[2 * n + goff(b, src[0], src[1], src[2])];
+    v.p2 = p_own;
 }
 
 /* One tangent pass of the chain at the roots of `v` on the geometries
@@ -748,7 +747,8 @@ __device__ __forceinline__ void w_of_state(const GeoT<G> &gt,
 template <bool SPIN>
 __global__ void __launch_bounds__(kBlock)
     k_adj_prims(BoxDev b, Options o, AdjParams pa, const double *U,
-                const double *lam_W, double *lam_U, double *lam_par)
+                const double *W, const double *lam_W, double *lam_U,
+                double *lam_par)
 {
     const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double g0 = 0.0, g1 = 0.0;
@@ -789,11 +789,10 @@ __global__ void __launch_bounds__(kBlock)
             const Cons uv = load_cons(U, n, ss);
             const long long is = iidx(b, src[0], src[1], src[2]);
             WChain v;
-            const bool ok = w_chain_values(b, o, pa, i, j, k, src, physical,
-                                           uv, v);
+            w_chain_values(b, pa, W, i, j, k, src, physical, v);
             /* SPIN: the spin's pass alone; otherwise 5 U inputs and Gamma */
             const int s0 = SPIN ? 6 : 0, s1 = SPIN ? 7 : 6;
-            for (int s = s0; ok && s < s1; ++s)
+            for (int s = s0; s < s1; ++s)
             {
                 const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
                 Dual M, a, gamma;

@@ -1143,11 +1143,11 @@ void adj_rhs_transpose(AdjRun &r, const double *X, double *lam_k,
                        dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
                        lam_k, lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_prims<false>, dim3(grid_of(b.gnpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, lam_W,
-                       lam_X, lam_par);
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
+                       lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_prims<true>, dim3(grid_of(b.gnpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, lam_W,
-                       lam_X, lam_par);
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
+                       lam_W, lam_X, lam_par);
 }
 
 int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
