@@ -332,3 +332,38 @@ def test_later_evaluation_ghost_fill_error(hip_context, scheme):
             op.advance_rk4(to_dev(st.copy()), 0.125, 1)
         else:
             op.advance_ssprk2(to_dev(st.copy()), 0.125)
+
+
+# ------------------------------------------------------------ tiny boxes
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (1, 2, 3), (3, 1, 2),
+                                   (2, 5, 1), (9, 1, 1)])
+def test_tiny_and_ragged_boxes_rhs_and_rk4(hip_context, shape):
+    """Interiors thinner than the ghost width along some axis: the x-face
+    wave packing, the y/z marching with fewer faces than a segment and the
+    exterior-cell enumeration against the oracle (rhs and one RK4 step)."""
+    bg = KerrSchildParams(1.0, 0.5)
+    box = BoxSpec((2, 3, 4), tuple(l + n - 1 for l, n in
+                                   zip((2, 3, 4), shape)), 3, 0.37,
+                  (-4.1, 1.3, -0.7))
+    opts = OperatorOptions(adiabatic_index=4.0 / 3.0)
+    st = reference_test_state(box, bg, 4.0 / 3.0)
+    op = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
+                                              context=hip_context)
+    O = oracle.Oracle(bg, box, opts)
+    ref = st.copy()
+    rhs_ref = O.compute_rhs(ref)
+    d = to_dev(st)
+    rhs = torch.zeros((11,) + box.interior_shape, dtype=torch.float64,
+                      device="cuda")
+    op.compute_rhs(rhs, d)
+    assert_conserved_close(rhs.cpu().numpy(), rhs_ref, TOL, what="rhs")
+    ref = st.copy()
+    diag_ref, mask_ref = O.advance_rk4(ref, 0.05, 1)
+    d = to_dev(st)
+    mask = torch.zeros(box.interior_shape, dtype=torch.uint8, device="cuda")
+    diag = op.advance_rk4(d, 0.05, 1, mask=mask)
+    assert_conserved_close(d.cpu().numpy(), ref, TOL, what="rk4")
+    assert diag == diag_ref
+    assert np.array_equal(mask.cpu().numpy(), mask_ref)
