@@ -280,6 +280,55 @@ __device__ __forceinline__ void face_flux_geo(const PrimT<T> w[4],
     hlle_face_g(lp, rp, o, gamma, face, D, f);
 }
 
+/* One state of the HLLE face (hlle_face_g's per-state part, in its
+ * order): floors on the face metric, conserved variables, flux and
+ * characteristic speeds. */
+template <class T> struct FaceSide
+{
+    ConsT<T> c, f;
+    T lo, hi;
+};
+
+template <class T, class G>
+__device__ __forceinline__ void face_side(PrimT<T> p, const Options &o,
+                                          T gamma, const GeoT<G> &face,
+                                          int d, FaceSide<T> &s)
+{
+    floors_g(p, o, gamma, face.g);
+    prim2con(p, gamma, face.g, s.c);
+    flux_of(p, s.c, d, face.lapse, face.shift, s.f);
+    char_speeds(p, gamma, face, d, s.lo, s.hi);
+}
+
+/* hlle_face_g's combination of the two states. */
+template <class T>
+__device__ __forceinline__ void hlle_combine(const FaceSide<T> &l,
+                                             const FaceSide<T> &r,
+                                             ConsT<T> &f)
+{
+    const T sl = std_min(T(0), std_min(l.lo, r.lo));
+    const T su = std_max(T(0), std_max(l.hi, r.hi));
+    if (sl >= T(0))
+    {
+        f = l.f;
+        return;
+    }
+    if (su <= T(0))
+    {
+        f = r.f;
+        return;
+    }
+    const T inv = T(1) / (su - sl);
+    f.D = (su * l.f.D - sl * r.f.D + sl * su * (r.c.D - l.c.D)) * inv;
+    f.tau = (su * l.f.tau - sl * r.f.tau + sl * su * (r.c.tau - l.c.tau)) *
+            inv;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        f.S[i] = (su * l.f.S[i] - sl * r.f.S[i] +
+                  sl * su * (r.c.S[i] - l.c.S[i])) *
+                 inv;
+}
+
 /* Component m of a primitive state, in W's order (rho, eps, P, v). */
 __device__ __forceinline__ double prim_comp(const Prim &w, int m)
 {
@@ -377,6 +426,17 @@ __device__ __forceinline__ ConsT<Dual> dual_cons(const Cons &c, int slot)
         q.S[m] = Dual(c.S[m], slot == 1 + m ? 1.0 : 0.0);
     q.tau = Dual(c.tau, slot == 4 ? 1.0 : 0.0);
     return q;
+}
+
+/* A double face state as duals with zero tangents: the state a pass does
+ * not seed. */
+__device__ __forceinline__ void dual_side(const FaceSide<double> &v,
+                                          FaceSide<Dual> &s)
+{
+    s.c = dual_cons(v.c, -1);
+    s.f = dual_cons(v.f, -1);
+    s.lo = Dual(v.lo);
+    s.hi = Dual(v.hi);
 }
 
 /* conserved component m of ConsT (D, S1, S2, S3, tau) */
@@ -541,24 +601,43 @@ __global__ void __launch_bounds__(kBlock)
                     slope_step(wv[0], wv[1], wv[2], o.theta, unused, lpv);
                     slope_step(wv[1], wv[2], wv[3], o.theta, rpv, unused);
                 }
-                /* lambda . dF/d(face states): 6 + 6 passes, then Gamma */
+                /* lambda . dF/d(face states): 6 + 6 passes, each with the
+                 * seeded state in duals and the other from its values,
+                 * then Gamma (both states) */
+                FaceSide<double> lv, rv;
+                face_side(lpv, o, pa.gamma, facev, D, lv);
+                face_side(rpv, o, pa.gamma, facev, D, rv);
+                FaceSide<Dual> l0, r0;
+                dual_side(lv, l0);
+                dual_side(rv, r0);
                 double lam_lp[6], lam_rp[6];
-                for (int s = 0; s < 13; ++s)
+                for (int s = 0; s < 12; ++s)
                 {
-                    Dual M, a, gamma;
-                    seed_params(pa, s < 12 ? 0 : SEED_GAMMA, M, a, gamma);
-                    const PrimT<Dual> ld = dual_prim(lpv, s < 6 ? s : -1);
-                    const PrimT<Dual> rd =
-                        dual_prim(rpv, (s >= 6 && s < 12) ? s - 6 : -1);
+                    const Dual gamma(pa.gamma);
+                    FaceSide<Dual> sd;
                     ConsT<Dual> f;
-                    hlle_face_g(ld, rd, o, gamma, facev, D, f);
-                    const double g = dot_tangent(f, lam);
                     if (s < 6)
-                        lam_lp[s] = g;
-                    else if (s < 12)
-                        lam_rp[s - 6] = g;
+                    {
+                        face_side(dual_prim(lpv, s), o, gamma, facev, D, sd);
+                        hlle_combine(sd, r0, f);
+                        lam_lp[s] = dot_tangent(f, lam);
+                    }
                     else
-                        g0 += g;
+                    {
+                        face_side(dual_prim(rpv, s - 6), o, gamma, facev, D,
+                                  sd);
+                        hlle_combine(l0, sd, f);
+                        lam_rp[s - 6] = dot_tangent(f, lam);
+                    }
+                }
+                {
+                    const Dual gamma(pa.gamma, 1.0);
+                    FaceSide<Dual> ls, rs;
+                    face_side(dual_prim(lpv, -1), o, gamma, facev, D, ls);
+                    face_side(dual_prim(rpv, -1), o, gamma, facev, D, rs);
+                    ConsT<Dual> f;
+                    hlle_combine(ls, rs, f);
+                    g0 += dot_tangent(f, lam);
                 }
                 /* back through the reconstruction, component by component:
                  * lp = W_L + s_L/2 from (W_{L-1}, W_L, W_R), rp = W_R - s_R/2
