@@ -1020,6 +1020,7 @@ __device__ __forceinline__ void static_sources(const PrimT<T> &p,
                                                const FullGeoT<G> &f,
                                                ConsT<T> &src)
 {
+    (void)gamma; /* the EOS argument of the reference's signature */
     const GeoT<G> &g = f.geo;
     const T W = 1.0 / sqrt(1.0 - vsq(p.v, g.g));
     const T h = 1.0 + p.eps + p.pressure / p.rho;
