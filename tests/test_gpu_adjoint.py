@@ -21,6 +21,7 @@ are therefore:
 Per-cell noise at h = 1e-6 moves neighbour differences by ~h dx/L relative
 and crosses limiter switches at near-ties, which no derivative resolves."""
 import dataclasses
+import os
 import numpy as np
 import pytest
 import torch
@@ -253,5 +254,49 @@ def test_adjoint_matches_fd_of_the_oracle_evolution(hip_context):
     print("oracle FD rho0:", errs, "Gamma:", fdg, float(gamma.grad),
           "spin:", fda, float(spin.grad))
     assert all(rel(a, b) < 1e-5 for a, b in errs), errs
+    assert rel(fdg, float(gamma.grad)) < 1e-5
+    assert rel(fda, float(spin.grad)) < 1e-5
+
+
+@pytest.mark.skipif(not os.environ.get("GRHD_SLOW_TESTS"),
+                    reason="full-size check, ~11 min of CPU oracle time; "
+                           "set GRHD_SLOW_TESTS=1 to run")
+def test_c5_adjoint_against_fd_of_the_oracle_full_size(hip_context):
+    """The differentiation configuration at full size -- 64^3 on
+    [-16, 16]^3, Kerr a = 0.5, 200 RK4 steps, J over the last 50 through
+    r = 4M, checkpoints every 20, horizon excised, recovery tolerance
+    1e-16 -- against central differences (h = 1e-6) of the CPU oracle's
+    own evolution along one smooth direction of rho0, in Gamma and in a."""
+    box = workloads.cube_box(64, 3, 16.0)
+    dt, nsteps, first, r0 = 0.25 * box.dx, 200, 151, 4.0
+    kw = dict(tolerance=1e-16, excise=True)
+    rho0 = torch.from_numpy(ah.seed_rho0(box, 5)).cuda()
+    rho = rho0.clone().requires_grad_(True)
+    gamma, spin = f64(GAMMA, True), f64(SPIN, True)
+    J = ah.objective(box, rho, gamma, spin, dt, nsteps, r0, first, 20,
+                     hip_context, **kw)
+    J.backward()
+    d = ah.smooth_field(box, np.random.default_rng(16)) * rho0
+
+    def obj(r, gm, sp, tag):
+        v = oracle_objective(box, r, gm, sp, dt, nsteps, r0, first, **kw)
+        print("oracle J (%s) = %.17g" % (tag, v), flush=True)
+        return v
+
+    J_cpu = obj(rho0, GAMMA, SPIN, "base")
+    assert rel(float(J.detach()), J_cpu) < 1e-12
+    fd = (obj(rho0 + H * d, GAMMA, SPIN, "+h d") -
+          obj(rho0 - H * d, GAMMA, SPIN, "-h d")) / (2 * H)
+    ad = float((rho.grad * d).sum())
+    fdg = (obj(rho0, GAMMA * (1 + H), SPIN, "+h Gamma") -
+           obj(rho0, GAMMA * (1 - H), SPIN, "-h Gamma")) / (2 * H * GAMMA)
+    fda = (obj(rho0, GAMMA, SPIN * (1 + H), "+h a") -
+           obj(rho0, GAMMA, SPIN * (1 - H), "-h a")) / (2 * H * SPIN)
+    print("full-size oracle FD: rho0 %r vs %r (rel %.2e); Gamma %r vs %r "
+          "(rel %.2e); a %r vs %r (rel %.2e)"
+          % (fd, ad, rel(fd, ad), fdg, float(gamma.grad),
+             rel(fdg, float(gamma.grad)), fda, float(spin.grad),
+             rel(fda, float(spin.grad))), flush=True)
+    assert rel(fd, ad) < 1e-5
     assert rel(fdg, float(gamma.grad)) < 1e-5
     assert rel(fda, float(spin.grad)) < 1e-5
