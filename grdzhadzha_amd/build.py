@@ -1,7 +1,9 @@
 """Build recipe of the HIP library (gfx950) -- no CMake, one hipcc line.
 
 Flags: -ffp-contract=off keeps every multiply and add separately rounded
-(the reference's CPU arithmetic); packed FP32 instructions are disabled.
+(the reference's CPU arithmetic); packed FP32 instructions are disabled;
+FP32 divide and square root (mixed-precision faces only) use the
+hardware's approximations.
 """
 import os
 import subprocess
@@ -14,6 +16,10 @@ ARCH = os.environ.get("GRHD_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",
+    # FP32 divide and square root by the hardware's approximations (a few
+    # ulp): only the mixed-precision face kernels do FP32 arithmetic; every
+    # FP64 kernel compiles to the same instructions with or without it
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
