@@ -157,6 +157,51 @@ def bench_c1(args):
             "cpu_oracle_1_thread": {"value": c_one, "cores": 1, "steps": 3}}
 
 
+def bench_sizes(args):
+    """Plain FP64 RK4 throughput against box size: the 128^3 torus recipe
+    (dx = 0.5, box n/4 M wide, Kerr a = 0.9) at n = 32 .. args.max_n, each
+    timed over enough steps for about a second of GPU time.  Shows where
+    the launch-latency-bound regime ends and how much of the device the
+    largest boxes occupy (device memory in use, the library's scratch and
+    the caching allocator included).  Returns the JSON record."""
+    import numpy as np
+    import torch
+
+    from grdzhadzha_amd import workloads
+    from grdzhadzha_amd.operator import FixedBGLevelDataFiniteVolumeOperator
+
+    rows = []
+    for n in (32, 64, 96, 128, 192, 256, 384, 512):
+        if n > args.max_n:
+            break
+        print("bench: building the %d^3 torus" % n, file=sys.stderr,
+              flush=True)
+        w = workloads.torus_box(n=n)
+        op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box,
+                                                  w.options)
+        st = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
+        del w.state
+        op.advance_rk4(st.clone(), w.dt, 1)
+        steps = max(2, min(200, int(2.0e8 / n ** 3)))
+        s = st.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.advance_rk4(s, w.dt, steps)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        free, total = torch.cuda.mem_get_info()
+        rows.append({"n": n, "steps": steps, "ms_per_step": t / steps * 1e3,
+                     "value": n ** 3 * steps / t,
+                     "device_used_gb": (total - free) / 1e9})
+        print("bench: %d^3 %.3e cell-updates/s (%.3f ms/step)"
+              % (n, rows[-1]["value"], rows[-1]["ms_per_step"]),
+              file=sys.stderr, flush=True)
+        del op, st, s
+        torch.cuda.empty_cache()
+    return {"workload": "torus recipe, plain FP64 RK4, box size sweep",
+            "unit": "cell-updates/s", "sizes": rows}
+
+
 def calibrate_copy():
     """Five dst.copy_(src) of 64 Mi float64 (512 MiB read + 512 MiB write
     per dispatch): the known-size kernel that converts rocprofv3's
@@ -529,8 +574,10 @@ def main():
                          "checks and the run without excision (profiling)")
     ap.add_argument("--no-c1", action="store_true",
                     help="skip the 32^3 Michel (C1) record of the default run")
+    ap.add_argument("--max-n", type=int, default=512,
+                    help="largest box of --workload sizes")
     ap.add_argument("--workload", choices=["torus", "batched", "mixed",
-                                           "adjoint", "c1"],
+                                           "adjoint", "c1", "sizes"],
                     default="torus",
                     help="torus: the 128^3 plain FP64 box (default); "
                          "batched: the 3-level binary-PN box set; mixed: "
@@ -569,6 +616,9 @@ def main():
         return
     if args.workload == "c1":
         print(json.dumps(bench_c1(args)), flush=True)
+        return
+    if args.workload == "sizes":
+        print(json.dumps(bench_sizes(args)), flush=True)
         return
 
     import numpy as np
