@@ -346,6 +346,12 @@ struct grhd_ctx
     long long cap_diag = 0;
     DevStatus *dstat = nullptr;
     DevStatus *hstat = nullptr;
+    /* side streams of the y and z face kernels, which run beside the x
+     * kernel (default priority), and the events that fork them off and
+     * join them back into the caller's stream */
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[2] = {nullptr, nullptr};
     /* instrumentation */
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -522,16 +528,39 @@ void launch_prims(grhd_ctx *c, const Options &o, const Background &bg,
     tick(c, GRHD_KERNEL_PRIMS, s);
 }
 
+/* Interior cells of a call from which the face kernels run concurrently:
+ * below it the fork/join events cost more than the overlap gains. */
+constexpr long long kConcurrentFacesCells = 1ll << 20;
+
+/* The three face directions are independent (W in, each its own flux
+ * pool out).  On calls of at least kConcurrentFacesCells interior cells, y
+ * and z run on the context's side streams beside x, forked after
+ * everything already queued on s (the ghost fill and recovery of this
+ * stage, the previous stage's cell kernel) and joined back into s before
+ * the cell kernel.  The faces timing class covers all three. */
 template <class T>
 void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
                     const Pools &p, hipStream_t s)
 {
+    if (c->plan.tot_i < kConcurrentFacesCells)
+    {
+        GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
+        GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), s, o, bg, p);
+        GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), s, o, bg, p);
+        tick(c, GRHD_KERNEL_FACES_X, s);
+        return;
+    }
+    (void)hipEventRecord(c->fork, s);
+    (void)hipStreamWaitEvent(c->side[0], c->fork, 0);
+    (void)hipStreamWaitEvent(c->side[1], c->fork, 0);
+    GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), c->side[0], o, bg, p);
+    (void)hipEventRecord(c->join[0], c->side[0]);
+    GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), c->side[1], o, bg, p);
+    (void)hipEventRecord(c->join[1], c->side[1]);
     GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
+    (void)hipStreamWaitEvent(s, c->join[0], 0);
+    (void)hipStreamWaitEvent(s, c->join[1], 0);
     tick(c, GRHD_KERNEL_FACES_X, s);
-    GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), s, o, bg, p);
-    tick(c, GRHD_KERNEL_FACES_Y, s);
-    GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), s, o, bg, p);
-    tick(c, GRHD_KERNEL_FACES_Z, s);
 }
 
 /* FP64 faces, or FP32 faces when the options ask for mixed precision. */
@@ -1393,9 +1422,19 @@ int grhd_ctx_create(grhd_ctx **out)
     grhd_ctx *c = new grhd_ctx();
     if (hipGetDevice(&c->device) != hipSuccess ||
         hipMalloc((void **)&c->dstat, sizeof(DevStatus)) != hipSuccess ||
-        hipHostMalloc((void **)&c->hstat, sizeof(DevStatus)) != hipSuccess)
+        hipHostMalloc((void **)&c->hstat, sizeof(DevStatus)) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side[0], hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipStreamCreateWithFlags(&c->side[1], hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&c->join[0], hipEventDisableTiming) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&c->join[1], hipEventDisableTiming) !=
+            hipSuccess)
     {
-        delete c;
+        grhd_ctx_destroy(c);
         return GRHD_ERR_DEVICE;
     }
     *out = c;
@@ -1418,6 +1457,15 @@ void grhd_ctx_destroy(grhd_ctx *c)
     (void)hipFree(c->diag);
     for (hipEvent_t e : c->events)
         (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i)
+    {
+        if (c->side[i])
+            (void)hipStreamDestroy(c->side[i]);
+        if (c->join[i])
+            (void)hipEventDestroy(c->join[i]);
+    }
+    if (c->fork)
+        (void)hipEventDestroy(c->fork);
     (void)hipFree(c->dstat);
     (void)hipHostFree(c->hstat);
     delete c;

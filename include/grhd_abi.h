@@ -457,9 +457,10 @@ int grhd_level_advance_rk4(grhd_ctx *ctx, const grhd_level_box *boxes,
 enum
 {
     GRHD_KERNEL_PRIMS = 0,   /* ghost fill + con2prim over the storage box */
-    GRHD_KERNEL_FACES_X = 1, /* MUSCL-HLLE faces, direction 0             */
-    GRHD_KERNEL_FACES_Y = 2,
-    GRHD_KERNEL_FACES_Z = 3,
+    GRHD_KERNEL_FACES_X = 1, /* MUSCL-HLLE faces: the three directions run
+                                concurrently and are timed together here  */
+    GRHD_KERNEL_FACES_Y = 2, /* (no longer used)                          */
+    GRHD_KERNEL_FACES_Z = 3, /* (no longer used)                          */
     GRHD_KERNEL_CELLS = 4,   /* divergence + sources + stage update       */
     GRHD_KERNEL_OTHER = 5,
     GRHD_NUM_KERNEL_CLASSES = 6
