@@ -172,9 +172,75 @@ bool same_key(const Plan &p, const grhd_level_box *boxes, int nbox,
                        sizeof(grhd_level_box) * (size_t)nbox) == 0;
 }
 
+/* A coarse bucket grid over the boxes' interiors: each bucket lists, in
+ * table order, the boxes overlapping it, so finding a cell's owner costs a
+ * handful of checks instead of one per box of the level. */
+struct OwnerIndex
+{
+    int edge = 1;
+    int org[3] = {0, 0, 0};
+    int nb[3] = {0, 0, 0};
+    std::vector<std::vector<int>> bucket;
+
+    void build(const std::vector<BoxDev> &hb)
+    {
+        int hi[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            org[d] = hb[0].lo[d];
+            hi[d] = hb[0].hi[d];
+        }
+        for (const BoxDev &b : hb)
+            for (int d = 0; d < 3; ++d)
+            {
+                org[d] = std::min(org[d], b.lo[d]);
+                hi[d] = std::max(hi[d], b.hi[d]);
+            }
+        int ext = 1;
+        for (int d = 0; d < 3; ++d)
+            ext = std::max(ext, hi[d] - org[d] + 1);
+        /* at most ~128 buckets per axis, at least 8 cells per bucket */
+        edge = std::max(8, (ext + 127) / 128);
+        for (int d = 0; d < 3; ++d)
+            nb[d] = (hi[d] - org[d]) / edge + 1;
+        bucket.assign((size_t)nb[0] * nb[1] * nb[2], std::vector<int>());
+        for (int n = 0; n < (int)hb.size(); ++n)
+        {
+            const BoxDev &b = hb[n];
+            int q0[3], q1[3];
+            for (int d = 0; d < 3; ++d)
+            {
+                q0[d] = (b.lo[d] - org[d]) / edge;
+                q1[d] = (b.hi[d] - org[d]) / edge;
+            }
+            for (int z = q0[2]; z <= q1[2]; ++z)
+                for (int y = q0[1]; y <= q1[1]; ++y)
+                    for (int x = q0[0]; x <= q1[0]; ++x)
+                        bucket[((size_t)z * nb[1] + y) * nb[0] + x]
+                            .push_back(n);
+        }
+    }
+
+    const std::vector<int> *at(const int w[3]) const
+    {
+        int q[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            if (w[d] < org[d])
+                return nullptr;
+            q[d] = (w[d] - org[d]) / edge;
+            if (q[d] >= nb[d])
+                return nullptr;
+        }
+        return &bucket[((size_t)q[2] * nb[1] + q[1]) * nb[0] + q[0]];
+    }
+};
+
 /* The box whose interior contains cell (i, j, k), wrapping periodic
- * directions of the domain; -1 if none. */
-int owner(const std::vector<BoxDev> &hb, int self, const int c[3], int w[3])
+ * directions of the domain; -1 if none.  The first such box in table
+ * order, skipping `self` when the cell was not wrapped. */
+int owner(const OwnerIndex &ix, const std::vector<BoxDev> &hb, int self,
+          const int c[3], int w[3])
 {
     const BoxDev &s = hb[self];
     for (int d = 0; d < 3; ++d)
@@ -186,7 +252,10 @@ int owner(const std::vector<BoxDev> &hb, int self, const int c[3], int w[3])
             w[d] = s.dlo[d] + (((c[d] - s.dlo[d]) % nd) + nd) % nd;
         }
     }
-    for (int b = 0; b < (int)hb.size(); ++b)
+    const std::vector<int> *cand = ix.at(w);
+    if (!cand)
+        return -1;
+    for (int b : *cand)
     {
         const BoxDev &o = hb[b];
         if (w[0] >= o.lo[0] && w[0] <= o.hi[0] && w[1] >= o.lo[1] &&
@@ -273,6 +342,8 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
     std::vector<int4> xl;
     if (!p.all_physical)
     {
+        OwnerIndex ix;
+        ix.build(p.hb);
         for (int n = 0; n < nbox; ++n)
         {
             const BoxDev &b = p.hb[n];
@@ -293,7 +364,7 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
                         if (interior || physical)
                             continue;
                         int w[3];
-                        const int src = owner(p.hb, n, c, w);
+                        const int src = owner(ix, p.hb, n, c, w);
                         if (src < 0)
                         {
                             p.exchange_ok = false;
