@@ -179,7 +179,9 @@ typedef struct grhd_error
     double residual;
 } grhd_error;
 
-/* Opaque scratch/stream context (device scratch buffers, status words). */
+/* Opaque scratch/stream context: device scratch buffers, status words,
+ * and two side streams on which the y and z face kernels of large calls
+ * run beside the x kernel (joined back into the caller's stream). */
 typedef struct grhd_ctx grhd_ctx;
 
 /* Create / destroy a context on the current HIP device.  No reference
@@ -392,7 +394,10 @@ int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
  * state is checkpointed every `checkpoint_every` steps and each segment is
  * recomputed in the reverse sweep.  With dJ_dconserved or dJ_dparams NULL
  * only the forward sweep runs and J is returned.  `state` is not
- * modified.  Scope:
+ * modified.  Horizon excision (grhd_params.excise_horizon) is honoured in
+ * both sweeps; near a black hole it keeps J differentiable, since without
+ * it recoveries that fail inside the horizon reset cells on either side
+ * of any perturbation.  Scope:
  * Kerr-Schild background, box == ProblemDomain (outflow ghosts), no
  * volume-time sources, FP64 faces; otherwise GRHD_ERR_INVALID_ARGUMENT. */
 int grhd_rk4_adjoint_box(grhd_ctx *ctx, const grhd_box *box,
