@@ -516,6 +516,135 @@ __device__ __forceinline__ bool interior(const BoxDev &b, int i, int j, int k)
 }
 
 /* ------------------------------------------------------------------ */
+/* The Kerr-Schild geometry of every cell and its tangent along the     */
+/* spin, evaluated once per adjoint run (the background is static) and  */
+/* read by the local-Jacobian kernels of every reverse stage.            */
+/* Component-major: values of component m at [m][cell], their spin      */
+/* tangents at [kGeoN + m][cell] (storage cells) / [kFullN + m][cell]   */
+/* (interior cells).  The values come from the same double functions   */
+/* the kernels evaluated before, so results are unchanged.              */
+/* ------------------------------------------------------------------ */
+constexpr int kGeoN = 22;  /* g (9), gu (9), lapse, shift (3)          */
+constexpr int kFullN = 73; /* geo (22), K (9), dlapse, dlogsqrtdet,
+                              dg (27), dshift (9)                      */
+
+struct MetricCache
+{
+    double *geo;  /* 2 kGeoN x storage cells   */
+    double *full; /* 2 kFullN x interior cells */
+};
+
+template <class T> __device__ __forceinline__ T &geo_at(GeoT<T> &g, int m)
+{
+    return m < 9    ? g.g[m / 3][m % 3]
+           : m < 18 ? g.gu[(m - 9) / 3][(m - 9) % 3]
+           : m < 19 ? g.lapse
+                    : g.shift[m - 19];
+}
+
+template <class T>
+__device__ __forceinline__ T &full_at(FullGeoT<T> &f, int m)
+{
+    if (m < kGeoN)
+        return geo_at(f.geo, m);
+    m -= kGeoN;
+    if (m < 9)
+        return f.K[m / 3][m % 3];
+    m -= 9;
+    if (m < 3)
+        return f.dlapse[m];
+    m -= 3;
+    if (m < 3)
+        return f.dlogsqrtdet[m];
+    m -= 3;
+    if (m < 27)
+        return f.dg[m / 9][(m / 3) % 3][m % 3];
+    m -= 27;
+    return f.dshift[m / 3][m % 3];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_adj_metric_cache(BoxDev b, AdjParams pa, MetricCache mc)
+{
+    const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= b.gnpts)
+        return;
+    const int i = b.glo[0] + (int)(it % b.gn[0]);
+    const int j = b.glo[1] + (int)((it / b.gn[0]) % b.gn[1]);
+    const int k = b.glo[2] + (int)(it / ((long long)b.gn[0] * b.gn[1]));
+    const long long n = b.gnpts;
+    double x[3];
+    coords(i, j, k, b.dx, b.center, x);
+    const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
+    Dual M, a, gamma;
+    seed_params(pa, SEED_SPIN, M, a, gamma);
+    {
+        Geo gv;
+        ks_geometry(pa.M, pa.a, x, gv);
+        GeoT<Dual> gd;
+        ks_geometry(M, a, xd, gd);
+#pragma unroll
+        for (int m = 0; m < kGeoN; ++m)
+        {
+            mc.geo[(long long)m * n + it] = geo_at(gv, m);
+            mc.geo[(long long)(kGeoN + m) * n + it] = geo_at(gd, m).d;
+        }
+    }
+    if (!interior(b, i, j, k))
+        return;
+    const long long ni = b.inpts, ii = iidx(b, i, j, k);
+    FullGeo fv;
+    source_geo(x, pa.M, pa.a, fv);
+    FullGeoT<Dual> fd;
+    source_geo(xd, M, a, fd);
+#pragma unroll
+    for (int m = 0; m < kFullN; ++m)
+    {
+        mc.full[(long long)m * ni + ii] = full_at(fv, m);
+        mc.full[(long long)(kFullN + m) * ni + ii] = full_at(fd, m).d;
+    }
+}
+
+/* The cached geometry of storage cell `so` (values; with SPIN their spin
+ * tangents as duals). */
+__device__ __forceinline__ void load_geo(const MetricCache &mc,
+                                         long long n, long long so, Geo &g)
+{
+#pragma unroll
+    for (int m = 0; m < kGeoN; ++m)
+        geo_at(g, m) = mc.geo[(long long)m * n + so];
+}
+
+__device__ __forceinline__ void load_geo(const MetricCache &mc,
+                                         long long n, long long so,
+                                         GeoT<Dual> &g)
+{
+#pragma unroll
+    for (int m = 0; m < kGeoN; ++m)
+        geo_at(g, m) = Dual(mc.geo[(long long)m * n + so],
+                            mc.geo[(long long)(kGeoN + m) * n + so]);
+}
+
+__device__ __forceinline__ void load_full(const MetricCache &mc,
+                                          long long ni, long long ii,
+                                          FullGeo &f)
+{
+#pragma unroll
+    for (int m = 0; m < kFullN; ++m)
+        full_at(f, m) = mc.full[(long long)m * ni + ii];
+}
+
+__device__ __forceinline__ void load_full(const MetricCache &mc,
+                                          long long ni, long long ii,
+                                          FullGeoT<Dual> &f)
+{
+#pragma unroll
+    for (int m = 0; m < kFullN; ++m)
+        full_at(f, m) = Dual(mc.full[(long long)m * ni + ii],
+                             mc.full[(long long)(kFullN + m) * ni + ii]);
+}
+
+/* ------------------------------------------------------------------ */
 /* k_adj_faces<D>: lambda_F of each face of direction D from the rhs    */
 /* adjoint of its two cells, then J_F^T lambda_F into lambda_W of the   */
 /* four stencil cells (atomics) and into the parameter adjoints.        */
@@ -523,8 +652,9 @@ __device__ __forceinline__ bool interior(const BoxDev &b, int i, int j, int k)
 /* ------------------------------------------------------------------ */
 template <int D, bool SPIN>
 __global__ void __launch_bounds__(kBlock)
-    k_adj_faces(BoxDev b, Options o, AdjParams pa, const double *W,
-                const double *lam_k, double *lam_W, double *lam_par)
+    k_adj_faces(BoxDev b, Options o, AdjParams pa, MetricCache mc,
+                const double *W, const double *lam_k, double *lam_W,
+                double *lam_par)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int n0 = b.in[0] + (D == 0), n1 = b.in[1] + (D == 1);
@@ -568,22 +698,20 @@ __global__ void __launch_bounds__(kBlock)
                 so[q] = goff(b, cc[0], cc[1], cc[2]);
                 wv[q] = load_prim(W, n, so[q]);
             }
-            double xl[3], xr[3];
-            coords(cl[0], cl[1], cl[2], b.dx, b.center, xl);
-            coords(c[0], c[1], c[2], b.dx, b.center, xr);
-            const Dual xld[3] = {Dual(xl[0]), Dual(xl[1]), Dual(xl[2])};
-            const Dual xrd[3] = {Dual(xr[0]), Dual(xr[1]), Dual(xr[2])};
             if constexpr (SPIN)
             {
-                /* the spin's tangent: the face metric with duals */
+                /* the spin's tangent: the face metric with duals, the mean
+                 * of the cached cell geometries and their tangents */
                 Dual M, a, gamma;
                 seed_params(pa, SEED_SPIN, M, a, gamma);
                 PrimT<Dual> wd[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     wd[q] = dual_prim(wv[q], -1);
-                GeoT<Dual> face;
-                face_geo(xld, xrd, M, a, face);
+                GeoT<Dual> gl, gr, face;
+                load_geo(mc, n, so[1], gl);
+                load_geo(mc, n, so[2], gr);
+                average_geo(gl, gr, face);
                 ConsT<Dual> f;
                 face_flux_geo(wd, face, gamma, o, D, f);
                 g1 = dot_tangent(f, lam);
@@ -591,8 +719,10 @@ __global__ void __launch_bounds__(kBlock)
             else
             {
                 /* the face metric in double: no pass here seeds it */
-                Geo facev;
-                face_geo(xl, xr, pa.M, pa.a, facev);
+                Geo gl, gr, facev;
+                load_geo(mc, n, so[1], gl);
+                load_geo(mc, n, so[2], gr);
+                average_geo(gl, gr, facev);
                 /* the face states (values), as k_faces forms them */
                 Prim lpv = wv[1], rpv = wv[2];
                 if (o.use_reconstruction)
@@ -679,9 +809,9 @@ __global__ void __launch_bounds__(kBlock)
 /* ------------------------------------------------------------------ */
 template <bool SPIN>
 __global__ void __launch_bounds__(kBlock)
-    k_adj_sources(BoxDev b, Options o, AdjParams pa, const double *U,
-                  const double *W, const double *lam_k, double *lam_W,
-                  double *lam_U, double *lam_par)
+    k_adj_sources(BoxDev b, Options o, AdjParams pa, MetricCache mc,
+                  const double *U, const double *W, const double *lam_k,
+                  double *lam_W, double *lam_U, double *lam_par)
 {
     const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double g0 = 0.0, g1 = 0.0;
@@ -704,16 +834,14 @@ __global__ void __launch_bounds__(kBlock)
         {
             const Prim wv = load_prim(W, n, so);
             const Cons uv = load_cons(U, n, so);
-            double x[3];
-            coords(i, j, k, b.dx, b.center, x);
-            const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
             if constexpr (SPIN)
             {
-                /* the spin's tangent: the full geometry with duals */
+                /* the spin's tangent: the cached full geometry with its
+                 * tangents */
                 Dual M, a, gamma;
                 seed_params(pa, SEED_SPIN, M, a, gamma);
                 FullGeoT<Dual> fg;
-                source_geo(xd, M, a, fg);
+                load_full(mc, ni, it, fg);
                 ConsT<Dual> src;
                 if (sources_geo(dual_prim(wv, -1), dual_cons(uv, -1), fg,
                                 gamma, o, src))
@@ -723,7 +851,7 @@ __global__ void __launch_bounds__(kBlock)
             {
                 /* the full geometry in double: no pass here seeds it */
                 FullGeo fgv;
-                source_geo(x, pa.M, pa.a, fgv);
+                load_full(mc, ni, it, fgv);
                 double gW[6] = {0, 0, 0, 0, 0, 0};
                 /* 6 W inputs, 5 U inputs, Gamma */
                 for (int s = 0; s < 12; ++s)
@@ -772,25 +900,23 @@ struct WChain
 };
 
 __device__ __forceinline__ void w_chain_values(const BoxDev &b,
-                                               const AdjParams &pa,
+                                               const MetricCache &mc,
                                                const double *W, int i, int j,
                                                int k, const int src[3],
                                                bool physical, WChain &v)
 {
     const long long n = b.gnpts;
-    double xt[3];
-    coords(i, j, k, b.dx, b.center, xt);
-    ks_geometry(pa.M, pa.a, xt, v.gt);
-    const double p_own = W[2 * n + goff(b, i, j, k)];
+    const long long st = goff(b, i, j, k);
+    load_geo(mc, n, st, v.gt);
+    const double p_own = W[2 * n + st];
     if (!physical)
     {
         v.p1 = p_own;
         return;
     }
-    double xs[3];
-    coords(src[0], src[1], src[2], b.dx, b.center, xs);
-    ks_geometry(pa.M, pa.a, xs, v.gs);
-    v.p1 = W[2 * n + goff(b, src[0], src[1], src[2])];
+    const long long ss = goff(b, src[0], src[1], src[2]);
+    load_geo(mc, n, ss, v.gs);
+    v.p1 = W[2 * n + ss];
     v.p2 = p_own;
 }
 
@@ -825,9 +951,9 @@ __device__ __forceinline__ void w_of_state(const GeoT<G> &gt,
 /* ------------------------------------------------------------------ */
 template <bool SPIN>
 __global__ void __launch_bounds__(kBlock)
-    k_adj_prims(BoxDev b, Options o, AdjParams pa, const double *U,
-                const double *W, const double *lam_W, double *lam_U,
-                double *lam_par)
+    k_adj_prims(BoxDev b, Options o, AdjParams pa, MetricCache mc,
+                const double *U, const double *W, const double *lam_W,
+                double *lam_U, double *lam_par)
 {
     const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double g0 = 0.0, g1 = 0.0;
@@ -868,7 +994,7 @@ __global__ void __launch_bounds__(kBlock)
             const Cons uv = load_cons(U, n, ss);
             const long long is = iidx(b, src[0], src[1], src[2]);
             WChain v;
-            w_chain_values(b, pa, W, i, j, k, src, physical, v);
+            w_chain_values(b, mc, W, i, j, k, src, physical, v);
             /* SPIN: the spin's pass alone; otherwise 5 U inputs and Gamma */
             const int s0 = SPIN ? 6 : 0, s1 = SPIN ? 7 : 6;
             for (int s = s0; s < s1; ++s)
@@ -880,17 +1006,10 @@ __global__ void __launch_bounds__(kBlock)
                 PrimT<Dual> w;
                 if constexpr (SPIN)
                 {
-                    double xt[3], xs[3];
-                    coords(i, j, k, b.dx, b.center, xt);
-                    coords(src[0], src[1], src[2], b.dx, b.center, xs);
-                    const Dual xtd[3] = {Dual(xt[0]), Dual(xt[1]),
-                                         Dual(xt[2])};
-                    const Dual xsd[3] = {Dual(xs[0]), Dual(xs[1]),
-                                         Dual(xs[2])};
                     GeoT<Dual> gt, gs;
-                    ks_geometry(M, a, xtd, gt);
+                    load_geo(mc, n, goff(b, i, j, k), gt);
                     if (physical)
-                        ks_geometry(M, a, xsd, gs);
+                        load_geo(mc, n, goff(b, src[0], src[1], src[2]), gs);
                     w_of_state(gt, gs, gamma, physical, v, o, ud, w);
                 }
                 else

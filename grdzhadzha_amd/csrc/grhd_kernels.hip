@@ -1147,6 +1147,7 @@ struct AdjRun
     hipStream_t s;
     double *work; /* 11-component FAB (comp_base 0) the forward kernels use */
     Pools p;
+    MetricCache mc; /* the cells' geometry and spin tangents (reverse) */
 };
 
 void fill0(double *x, long long n, hipStream_t s)
@@ -1219,35 +1220,35 @@ void adj_rhs_transpose(AdjRun &r, const double *X, double *lam_k,
     /* each local Jacobian in two launches: the passes on the double metric,
      * and the spin's pass, whose dual metric needs far more registers */
     hipLaunchKernelGGL((k_adj_faces<0, false>), dim3(grid_of(nf[0])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
                        lam_W, lam_par);
     hipLaunchKernelGGL((k_adj_faces<0, true>), dim3(grid_of(nf[0])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
                        lam_W, lam_par);
     hipLaunchKernelGGL((k_adj_faces<1, false>), dim3(grid_of(nf[1])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
                        lam_W, lam_par);
     hipLaunchKernelGGL((k_adj_faces<1, true>), dim3(grid_of(nf[1])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
                        lam_W, lam_par);
     hipLaunchKernelGGL((k_adj_faces<2, false>), dim3(grid_of(nf[2])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
                        lam_W, lam_par);
     hipLaunchKernelGGL((k_adj_faces<2, true>), dim3(grid_of(nf[2])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.c->W, lam_k,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
                        lam_W, lam_par);
     hipLaunchKernelGGL(k_adj_sources<false>, dim3(grid_of(b.inpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
-                       lam_k, lam_W, lam_X, lam_par);
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
+                       r.c->W, lam_k, lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_sources<true>, dim3(grid_of(b.inpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
-                       lam_k, lam_W, lam_X, lam_par);
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
+                       r.c->W, lam_k, lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_prims<false>, dim3(grid_of(b.gnpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
-                       lam_W, lam_X, lam_par);
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
+                       r.c->W, lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_prims<true>, dim3(grid_of(b.gnpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work, r.c->W,
-                       lam_W, lam_X, lam_par);
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
+                       r.c->W, lam_W, lam_X, lam_par);
 }
 
 int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
@@ -1309,6 +1310,8 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     double *lam_X = mem.get(nadj * 5 * I + 1);
     double *lam_t = mem.get(nadj * 5 * I + 1);
     double *lam_W = mem.get(nadj * 6 * G + 1);
+    r.mc.geo = mem.get(nadj * 2 * kGeoN * G + 1);
+    r.mc.full = mem.get(nadj * 2 * kFullN * I + 1);
     double *lam_par = mem.get(2);
     double *terms = mem.get(I);
     const long long nb = (I + kBlock - 1) / kBlock;
@@ -1324,7 +1327,7 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
         fk[q] = mem.get(5 * I);
     if (!r.work || !ckpt || !seg || !acc || !lam || !lam_k[3] || !lam_X ||
         !lam_t || !lam_W || !lam_par || !terms || !part || !mdot || !X0 ||
-        !fx[4] || !fk[3] || !facc3)
+        !fx[4] || !fk[3] || !facc3 || !r.mc.geo || !r.mc.full)
         return GRHD_ERR_DEVICE;
     r.p = make_pools(c, r.work, 0);
     hipStream_t s = r.s;
@@ -1372,6 +1375,10 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     *J = Jv;
     if (!grad)
         return GRHD_OK;
+
+    /* the cells' geometry and its spin tangent, once for the whole sweep */
+    hipLaunchKernelGGL(k_adj_metric_cache, dim3(grid_of(G)), dim3(kBlock), 0,
+                       s, b, r.pa, r.mc);
 
     /* reverse sweep, one checkpoint segment at a time */
     const unsigned gi = grid_of(I);
