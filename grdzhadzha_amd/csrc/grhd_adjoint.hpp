@@ -516,22 +516,21 @@ __device__ __forceinline__ bool interior(const BoxDev &b, int i, int j, int k)
 }
 
 /* ------------------------------------------------------------------ */
-/* The Kerr-Schild geometry of every cell and its tangent along the     */
-/* spin, evaluated once per adjoint run (the background is static) and  */
-/* read by the local-Jacobian kernels of every reverse stage.            */
-/* Component-major: values of component m at [m][cell], their spin      */
-/* tangents at [kGeoN + m][cell] (storage cells) / [kFullN + m][cell]   */
-/* (interior cells).  The values come from the same double functions   */
-/* the kernels evaluated before, so results are unchanged.              */
+/* The Kerr-Schild geometry of every storage cell with its tangent      */
+/* along the spin, evaluated once per adjoint run (the background is    */
+/* static) and read by the spin-tangent kernels of every reverse stage, */
+/* which would otherwise evaluate it in dual arithmetic per face and    */
+/* per cell.  Component-major: the value of component m at [m][cell],   */
+/* its spin tangent at [kGeoN + m][cell].  The kernels working in plain */
+/* double recompute the metric: there the 22 loads per cell cost more  */
+/* than the arithmetic they replace.  The values come from the same     */
+/* double functions, so results are unchanged.                          */
 /* ------------------------------------------------------------------ */
-constexpr int kGeoN = 22;  /* g (9), gu (9), lapse, shift (3)          */
-constexpr int kFullN = 73; /* geo (22), K (9), dlapse, dlogsqrtdet,
-                              dg (27), dshift (9)                      */
+constexpr int kGeoN = 22; /* g (9), gu (9), lapse, shift (3) */
 
 struct MetricCache
 {
-    double *geo;  /* 2 kGeoN x storage cells   */
-    double *full; /* 2 kFullN x interior cells */
+    double *geo; /* 2 kGeoN x storage cells */
 };
 
 template <class T> __device__ __forceinline__ T &geo_at(GeoT<T> &g, int m)
@@ -540,27 +539,6 @@ template <class T> __device__ __forceinline__ T &geo_at(GeoT<T> &g, int m)
            : m < 18 ? g.gu[(m - 9) / 3][(m - 9) % 3]
            : m < 19 ? g.lapse
                     : g.shift[m - 19];
-}
-
-template <class T>
-__device__ __forceinline__ T &full_at(FullGeoT<T> &f, int m)
-{
-    if (m < kGeoN)
-        return geo_at(f.geo, m);
-    m -= kGeoN;
-    if (m < 9)
-        return f.K[m / 3][m % 3];
-    m -= 9;
-    if (m < 3)
-        return f.dlapse[m];
-    m -= 3;
-    if (m < 3)
-        return f.dlogsqrtdet[m];
-    m -= 3;
-    if (m < 27)
-        return f.dg[m / 9][(m / 3) % 3][m % 3];
-    m -= 27;
-    return f.dshift[m / 3][m % 3];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -590,31 +568,10 @@ __global__ void __launch_bounds__(kBlock)
             mc.geo[(long long)(kGeoN + m) * n + it] = geo_at(gd, m).d;
         }
     }
-    if (!interior(b, i, j, k))
-        return;
-    const long long ni = b.inpts, ii = iidx(b, i, j, k);
-    FullGeo fv;
-    source_geo(x, pa.M, pa.a, fv);
-    FullGeoT<Dual> fd;
-    source_geo(xd, M, a, fd);
-#pragma unroll
-    for (int m = 0; m < kFullN; ++m)
-    {
-        mc.full[(long long)m * ni + ii] = full_at(fv, m);
-        mc.full[(long long)(kFullN + m) * ni + ii] = full_at(fd, m).d;
-    }
 }
 
-/* The cached geometry of storage cell `so` (values; with SPIN their spin
- * tangents as duals). */
-__device__ __forceinline__ void load_geo(const MetricCache &mc,
-                                         long long n, long long so, Geo &g)
-{
-#pragma unroll
-    for (int m = 0; m < kGeoN; ++m)
-        geo_at(g, m) = mc.geo[(long long)m * n + so];
-}
-
+/* The cached geometry of storage cell `so` with its spin tangents as
+ * duals. */
 __device__ __forceinline__ void load_geo(const MetricCache &mc,
                                          long long n, long long so,
                                          GeoT<Dual> &g)
@@ -623,25 +580,6 @@ __device__ __forceinline__ void load_geo(const MetricCache &mc,
     for (int m = 0; m < kGeoN; ++m)
         geo_at(g, m) = Dual(mc.geo[(long long)m * n + so],
                             mc.geo[(long long)(kGeoN + m) * n + so]);
-}
-
-__device__ __forceinline__ void load_full(const MetricCache &mc,
-                                          long long ni, long long ii,
-                                          FullGeo &f)
-{
-#pragma unroll
-    for (int m = 0; m < kFullN; ++m)
-        full_at(f, m) = mc.full[(long long)m * ni + ii];
-}
-
-__device__ __forceinline__ void load_full(const MetricCache &mc,
-                                          long long ni, long long ii,
-                                          FullGeoT<Dual> &f)
-{
-#pragma unroll
-    for (int m = 0; m < kFullN; ++m)
-        full_at(f, m) = Dual(mc.full[(long long)m * ni + ii],
-                             mc.full[(long long)(kFullN + m) * ni + ii]);
 }
 
 /* ------------------------------------------------------------------ */
@@ -719,10 +657,11 @@ __global__ void __launch_bounds__(kBlock)
             else
             {
                 /* the face metric in double: no pass here seeds it */
-                Geo gl, gr, facev;
-                load_geo(mc, n, so[1], gl);
-                load_geo(mc, n, so[2], gr);
-                average_geo(gl, gr, facev);
+                double xl[3], xr[3];
+                coords(cl[0], cl[1], cl[2], b.dx, b.center, xl);
+                coords(c[0], c[1], c[2], b.dx, b.center, xr);
+                Geo facev;
+                face_geo(xl, xr, pa.M, pa.a, facev);
                 /* the face states (values), as k_faces forms them */
                 Prim lpv = wv[1], rpv = wv[2];
                 if (o.use_reconstruction)
@@ -809,8 +748,7 @@ __global__ void __launch_bounds__(kBlock)
 /* ------------------------------------------------------------------ */
 template <bool SPIN>
 __global__ void __launch_bounds__(kBlock)
-    k_adj_sources(BoxDev b, Options o, AdjParams pa, MetricCache mc,
-                  const double *U, const double *W, const double *lam_k,
+    k_adj_sources(BoxDev b, Options o, AdjParams pa, const double *U, const double *W, const double *lam_k,
                   double *lam_W, double *lam_U, double *lam_par)
 {
     const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -834,14 +772,16 @@ __global__ void __launch_bounds__(kBlock)
         {
             const Prim wv = load_prim(W, n, so);
             const Cons uv = load_cons(U, n, so);
+            double x[3];
+            coords(i, j, k, b.dx, b.center, x);
             if constexpr (SPIN)
             {
-                /* the spin's tangent: the cached full geometry with its
-                 * tangents */
+                /* the spin's tangent: the full geometry with duals */
                 Dual M, a, gamma;
                 seed_params(pa, SEED_SPIN, M, a, gamma);
+                const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
                 FullGeoT<Dual> fg;
-                load_full(mc, ni, it, fg);
+                source_geo(xd, M, a, fg);
                 ConsT<Dual> src;
                 if (sources_geo(dual_prim(wv, -1), dual_cons(uv, -1), fg,
                                 gamma, o, src))
@@ -851,7 +791,7 @@ __global__ void __launch_bounds__(kBlock)
             {
                 /* the full geometry in double: no pass here seeds it */
                 FullGeo fgv;
-                load_full(mc, ni, it, fgv);
+                source_geo(x, pa.M, pa.a, fgv);
                 double gW[6] = {0, 0, 0, 0, 0, 0};
                 /* 6 W inputs, 5 U inputs, Gamma */
                 for (int s = 0; s < 12; ++s)
@@ -900,14 +840,16 @@ struct WChain
 };
 
 __device__ __forceinline__ void w_chain_values(const BoxDev &b,
-                                               const MetricCache &mc,
+                                               const AdjParams &pa,
                                                const double *W, int i, int j,
                                                int k, const int src[3],
                                                bool physical, WChain &v)
 {
     const long long n = b.gnpts;
     const long long st = goff(b, i, j, k);
-    load_geo(mc, n, st, v.gt);
+    double xt[3];
+    coords(i, j, k, b.dx, b.center, xt);
+    ks_geometry(pa.M, pa.a, xt, v.gt);
     const double p_own = W[2 * n + st];
     if (!physical)
     {
@@ -915,7 +857,9 @@ __device__ __forceinline__ void w_chain_values(const BoxDev &b,
         return;
     }
     const long long ss = goff(b, src[0], src[1], src[2]);
-    load_geo(mc, n, ss, v.gs);
+    double xs[3];
+    coords(src[0], src[1], src[2], b.dx, b.center, xs);
+    ks_geometry(pa.M, pa.a, xs, v.gs);
     v.p1 = W[2 * n + ss];
     v.p2 = p_own;
 }
@@ -994,7 +938,7 @@ __global__ void __launch_bounds__(kBlock)
             const Cons uv = load_cons(U, n, ss);
             const long long is = iidx(b, src[0], src[1], src[2]);
             WChain v;
-            w_chain_values(b, mc, W, i, j, k, src, physical, v);
+            w_chain_values(b, pa, W, i, j, k, src, physical, v);
             /* SPIN: the spin's pass alone; otherwise 5 U inputs and Gamma */
             const int s0 = SPIN ? 6 : 0, s1 = SPIN ? 7 : 6;
             for (int s = s0; s < s1; ++s)

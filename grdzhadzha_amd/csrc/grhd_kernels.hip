@@ -1238,10 +1238,10 @@ void adj_rhs_transpose(AdjRun &r, const double *X, double *lam_k,
                        dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
                        lam_W, lam_par);
     hipLaunchKernelGGL(k_adj_sources<false>, dim3(grid_of(b.inpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work,
                        r.c->W, lam_k, lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_sources<true>, dim3(grid_of(b.inpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
+                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work,
                        r.c->W, lam_k, lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_prims<false>, dim3(grid_of(b.gnpts)),
                        dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
@@ -1311,7 +1311,6 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     double *lam_t = mem.get(nadj * 5 * I + 1);
     double *lam_W = mem.get(nadj * 6 * G + 1);
     r.mc.geo = mem.get(nadj * 2 * kGeoN * G + 1);
-    r.mc.full = mem.get(nadj * 2 * kFullN * I + 1);
     double *lam_par = mem.get(2);
     double *terms = mem.get(I);
     const long long nb = (I + kBlock - 1) / kBlock;
@@ -1327,7 +1326,7 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
         fk[q] = mem.get(5 * I);
     if (!r.work || !ckpt || !seg || !acc || !lam || !lam_k[3] || !lam_X ||
         !lam_t || !lam_W || !lam_par || !terms || !part || !mdot || !X0 ||
-        !fx[4] || !fk[3] || !facc3 || !r.mc.geo || !r.mc.full)
+        !fx[4] || !fk[3] || !facc3 || !r.mc.geo)
         return GRHD_ERR_DEVICE;
     r.p = make_pools(c, r.work, 0);
     hipStream_t s = r.s;
