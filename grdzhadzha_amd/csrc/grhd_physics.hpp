@@ -735,28 +735,41 @@ __device__ __forceinline__ void prim2con(const PrimT<T> &p, T gamma,
     }
 }
 
+/* The pressure residual's intermediate values at p: E + p, the raw and
+ * clamped v^2 and W.  The Newton derivative at the same p and the final
+ * primitives reuse them instead of recomputing the same divisions and
+ * square roots (the values are the same operations on the same inputs). */
+struct PresidAt
+{
+    double epp, v2raw, v2, W;
+};
+
+__device__ __forceinline__ double presid_at(double p, double D, double tau,
+                                            double S2, double gamma,
+                                            double max_v2, PresidAt &s)
+{
+    s.epp = tau + D + p;
+    s.v2raw = S2 / (s.epp * s.epp);
+    s.v2 = s.v2raw > max_v2 ? max_v2 : s.v2raw;
+    s.W = 1.0 / sqrt(1.0 - s.v2);
+    return D * s.W + gamma * p * s.W * s.W / (gamma - 1.0) - s.epp;
+}
+
 __device__ __forceinline__ double presid(double p, double D, double tau,
                                          double S2, double gamma,
                                          double max_v2)
 {
-    const double epp = tau + D + p;
-    double v2 = S2 / (epp * epp);
-    if (v2 > max_v2)
-        v2 = max_v2;
-    const double W = 1.0 / sqrt(1.0 - v2);
-    return D * W + gamma * p * W * W / (gamma - 1.0) - epp;
+    PresidAt s;
+    return presid_at(p, D, tau, S2, gamma, max_v2, s);
 }
 
-__device__ __forceinline__ double presid_deriv(double p, double D, double tau,
-                                               double S2, double gamma,
-                                               double max_v2)
+/* d residual / dp at the p of `s` */
+__device__ __forceinline__ double presid_deriv_at(double p, double D,
+                                                  double gamma,
+                                                  const PresidAt &s)
 {
-    const double epp = tau + D + p;
-    double v2 = S2 / (epp * epp);
-    if (v2 > max_v2)
-        v2 = max_v2;
-    const double W = 1.0 / sqrt(1.0 - v2);
-    const double dW_dp = -W * W * W * v2 / epp;
+    const double W = s.W;
+    const double dW_dp = -W * W * W * s.v2 / s.epp;
     const double dW2_dp = 2.0 * W * dW_dp;
     return D * dW_dp + gamma * (W * W + p * dW2_dp) / (gamma - 1.0) - 1.0;
 }
@@ -800,7 +813,8 @@ __device__ __forceinline__ bool con2prim(const Cons &c, const Options &o,
     double p = (fabs(f_lower) <= o.tolerance)
                    ? lower
                    : std_min(std_max(c.tau * (gamma - 1.0), lower), upper);
-    double res = presid(p, c.D, c.tau, S2, gamma, mv2);
+    PresidAt at;
+    double res = presid_at(p, c.D, c.tau, S2, gamma, mv2, at);
     for (int iter = 0; iter < o.max_iterations; ++iter)
     {
         if (fabs(res) < o.tolerance)
@@ -812,21 +826,21 @@ __device__ __forceinline__ bool con2prim(const Cons &c, const Options &o,
             else
                 lower = p;
         }
-        const double der = presid_deriv(p, c.D, c.tau, S2, gamma, mv2);
+        const double der = presid_deriv_at(p, c.D, gamma, at);
         double np = p - res / der;
         if (!isfinite(np) || np <= lower || np >= upper)
             np = 0.5 * (lower + upper);
         p = np;
-        res = presid(p, c.D, c.tau, S2, gamma, mv2);
+        res = presid_at(p, c.D, c.tau, S2, gamma, mv2, at);
     }
     residual_out = res;
     if (fabs(res) >= o.tolerance)
         return false;
-    const double epp = c.tau + c.D + p;
-    const double v2 = S2 / (epp * epp);
-    if (v2 >= mv2)
+    /* E + p and v^2 at the final p; below the cap W is the residual's */
+    const double epp = at.epp;
+    if (at.v2raw >= mv2)
         return false;
-    const double W = 1.0 / sqrt(1.0 - v2);
+    const double W = at.W;
     out.rho = c.D / W;
     out.pressure = p;
     out.eps = p / ((gamma - 1.0) * out.rho);
