@@ -613,7 +613,15 @@ template <class T>
 void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
                     const Pools &p, hipStream_t s)
 {
-    if (c->plan.tot_i < kConcurrentFacesCells)
+    /* a failed fork runs the three kernels in order on `s`; the error
+     * stays pending for the call's closing hipGetLastError check */
+    const bool forked = c->plan.tot_i >= kConcurrentFacesCells &&
+                        hipEventRecord(c->fork, s) == hipSuccess &&
+                        hipStreamWaitEvent(c->side[0], c->fork, 0) ==
+                            hipSuccess &&
+                        hipStreamWaitEvent(c->side[1], c->fork, 0) ==
+                            hipSuccess;
+    if (!forked)
     {
         GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
         GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), s, o, bg, p);
@@ -621,16 +629,16 @@ void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
         tick(c, GRHD_KERNEL_FACES_X, s);
         return;
     }
-    (void)hipEventRecord(c->fork, s);
-    (void)hipStreamWaitEvent(c->side[0], c->fork, 0);
-    (void)hipStreamWaitEvent(c->side[1], c->fork, 0);
     GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), c->side[0], o, bg, p);
-    (void)hipEventRecord(c->join[0], c->side[0]);
     GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), c->side[1], o, bg, p);
-    (void)hipEventRecord(c->join[1], c->side[1]);
     GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
-    (void)hipStreamWaitEvent(s, c->join[0], 0);
-    (void)hipStreamWaitEvent(s, c->join[1], 0);
+    /* join: `s` waits for both side streams; if an event operation fails,
+     * the host waits for the side stream instead, so k_cells never runs
+     * before the y and z fluxes are written */
+    for (int q = 0; q < 2; ++q)
+        if (hipEventRecord(c->join[q], c->side[q]) != hipSuccess ||
+            hipStreamWaitEvent(s, c->join[q], 0) != hipSuccess)
+            (void)hipStreamSynchronize(c->side[q]);
     tick(c, GRHD_KERNEL_FACES_X, s);
 }
 
