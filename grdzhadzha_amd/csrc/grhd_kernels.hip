@@ -314,6 +314,9 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
     }
     if (2 * p.tot_g >= (1ll << kKeyBits))
         return GRHD_ERR_INVALID_ARGUMENT;
+    const int seg = march_segment(p.tot_i);
+    for (BoxDev &d : p.hb)
+        d.seg = seg;
 
     /* block tables */
     for (int c = 0; c < NCLASS; ++c)

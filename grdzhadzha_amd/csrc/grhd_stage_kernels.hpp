@@ -47,7 +47,22 @@ namespace grhd
 constexpr int kBlock = 256;
 constexpr unsigned long long kNoFail = ~0ull;
 constexpr int kKeyBits = 43; /* 2 x (cells of a level) < 2^43 */
-constexpr int kSeg = 8;      /* faces per thread of the marching kernels */
+
+/* Faces per walker of the marching y/z face kernels on a level of `cells`
+ * interior cells.  A walk shares each cell's metric and slopes between the
+ * two faces it touches, but it is sequential: walks of 8 faces on a 32^3
+ * box leave 80 waves for 1024 SIMDs.  Walks get shorter until the level
+ * yields at least 2^16 walkers (one wave per SIMD).  Measured per RK4 step
+ * (ms, walk length 1 / 2 / 4 / 8): 32^3 0.19 / 0.21 / 0.26 / 0.36,
+ * 64^3 0.48 / 0.45 / 0.43 / 0.52, 96^3 1.19 / 1.13 / 1.12 / 1.09; the
+ * fluxes do not depend on it. */
+__host__ inline int march_segment(long long cells)
+{
+    int seg = 8;
+    while (seg > 1 && cells / seg < (1ll << 16))
+        seg /= 2;
+    return seg;
+}
 
 /* Device-side status block of one call.  The failure key, which every
  * wavefront of the stage kernels reads, has a 128-byte line of its own,
@@ -83,6 +98,7 @@ struct BoxDev
     long long off_f[3]; /* faces of direction d of the boxes before it */
     long long nface[3]; /* faces of direction d of this box */
     double inv_dx;      /* 1 / dx, rounded once on the host */
+    int seg;            /* faces per walker of the marching y/z kernels */
 };
 
 /* Kinds of work item, one block table each. */
@@ -99,7 +115,7 @@ enum ItemClass
 
 __host__ __device__ inline int face_segments(const BoxDev &b, int d)
 {
-    return (b.in[d] + 1 + kSeg - 1) / kSeg;
+    return (b.in[d] + 1 + b.seg - 1) / b.seg;
 }
 
 __host__ __device__ inline long long fx_cells(const BoxDev &b)
@@ -662,7 +678,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 /* ------------------------------------------------------------------ */
-/* k_faces<D> (D = 1, 2): each thread walks kSeg consecutive faces     */
+/* k_faces<D> (D = 1, 2): each thread walks b.seg consecutive faces    */
 /* along D carrying the right cell's primitives, the next face's left  */
 /* state and the right cell's metric, so a cell's slope and metric are */
 /* computed once per walk; lanes are consecutive along i.              */
@@ -683,8 +699,8 @@ __global__ void __launch_bounds__(kBlock)
     const int ib = (int)((it / na) % nb);
     const int sg = (int)(it / ((long long)na * nb));
     const int fnd = b.in[D] + 1;
-    const int p0 = sg * kSeg;
-    const int p1 = min(p0 + kSeg, fnd);
+    const int p0 = sg * b.seg;
+    const int p1 = min(p0 + b.seg, fnd);
     const long long n = b.gnpts;
     const double *W = p.W + 6 * b.off_g;
     const long long stride =

@@ -109,6 +109,15 @@ def test_rhs_torus_kerr(hip_context):
     _rhs_case(small_torus(), hip_context)
 
 
+@pytest.mark.parametrize("n", [64, 96])
+def test_rhs_torus_longer_face_walks(hip_context, n):
+    """Boxes below 48^3 give the marching y/z face kernels walks of one
+    face; 64^3 gives walks of 4 faces, 96^3 walks of 8, where each walker
+    carries a cell's metric and slopes to its next face.  The rhs still
+    equals the oracle's."""
+    _rhs_case(workloads.torus_box(n=n, nsteps=1), hip_context)
+
+
 def test_rhs_no_reconstruction_no_sources(hip_context):
     w = small_torus()
     w.options.use_reconstruction = False
