@@ -319,15 +319,20 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
         d.seg = seg;
 
     /* block tables */
+    std::vector<int2> tables[NCLASS];
     for (int c = 0; c < NCLASS; ++c)
     {
-        std::vector<int2> items;
-        for (int n = 0; n < nbox; ++n)
-        {
-            const long long m = class_items(p.hb[n], c);
-            for (long long first = 0; first < m; first += kBlock)
-                items.push_back(make_int2(n, (int)first));
-        }
+        std::vector<int2> &items = tables[c];
+        if (c == C_FALL)
+            for (int f : {C_FX, C_FY, C_FZ})
+                items.insert(items.end(), tables[f].begin(), tables[f].end());
+        else
+            for (int n = 0; n < nbox; ++n)
+            {
+                const long long m = class_items(p.hb[n], c);
+                for (long long first = 0; first < m; first += kBlock)
+                    items.push_back(make_int2(n, (int)first));
+            }
         p.nblocks[c] = (int)items.size();
         if (!items.empty())
         {
@@ -602,24 +607,40 @@ void launch_prims(grhd_ctx *c, const Options &o, const Background &bg,
     tick(c, GRHD_KERNEL_PRIMS, s);
 }
 
-/* Interior cells of a call from which the face kernels run concurrently:
- * below it the fork/join events cost more than the overlap gains. */
-constexpr long long kConcurrentFacesCells = 1ll << 20;
+/* Interior cells of a call from which the three face kernels run as
+ * separate launches on concurrent streams; below it one fused launch
+ * (k_faces_all) does all three directions.  Measured per RK4 step, fused
+ * against streams: 32^3 0.148 / 0.262 ms, 64^3 0.394 / 0.473, 128^3
+ * 2.260 / 2.293, 160^3 4.232 / 4.145, 256^3 16.86 / 16.54, the box-set
+ * levels (6M cells each) 22.4 / 21.2 ms for all three levels. */
+constexpr long long kFusedFacesCells = 3ll << 20;
 
 /* The three face directions are independent (W in, each its own flux
- * pool out).  On calls of at least kConcurrentFacesCells interior cells, y
- * and z run on the context's side streams beside x, forked after
- * everything already queued on s (the ghost fill and recovery of this
- * stage, the previous stage's cell kernel) and joined back into s before
- * the cell kernel.  The faces timing class covers all three. */
+ * pool out).  Small calls launch them as one kernel.  On calls of at least
+ * kFusedFacesCells interior cells, where the fused kernel's register
+ * count costs the x faces more than the launch boundaries, y and z run on
+ * the context's side streams beside x, forked after everything already
+ * queued on s (the ghost fill and recovery of this stage, the previous
+ * stage's cell kernel) and joined back into s before the cell kernel.
+ * The faces timing class covers all three. */
 template <class T>
 void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
                     const Pools &p, hipStream_t s)
 {
+    if (c->plan.tot_i < kFusedFacesCells)
+    {
+        if (c->plan.nblocks[C_FALL] > 0)
+            hipLaunchKernelGGL(k_faces_all<T>, dim3(c->plan.nblocks[C_FALL]),
+                               dim3(kBlock), 0, s, c->plan.d_boxes,
+                               c->plan.d_items[C_FALL],
+                               c->plan.nblocks[C_FX], c->plan.nblocks[C_FY],
+                               o, bg, p);
+        tick(c, GRHD_KERNEL_FACES_X, s);
+        return;
+    }
     /* a failed fork runs the three kernels in order on `s`; the error
      * stays pending for the call's closing hipGetLastError check */
-    const bool forked = c->plan.tot_i >= kConcurrentFacesCells &&
-                        hipEventRecord(c->fork, s) == hipSuccess &&
+    const bool forked = hipEventRecord(c->fork, s) == hipSuccess &&
                         hipStreamWaitEvent(c->side[0], c->fork, 0) ==
                             hipSuccess &&
                         hipStreamWaitEvent(c->side[1], c->fork, 0) ==

@@ -110,7 +110,8 @@ enum ItemClass
     C_FX = 3,  /* x-face lanes: 64 per 63 cells of a row */
     C_FY = 4,  /* marching y-face walkers                */
     C_FZ = 5,  /* marching z-face walkers                */
-    NCLASS = 6
+    C_FALL = 6, /* the C_FX, C_FY and C_FZ tables, in that order */
+    NCLASS = 7
 };
 
 __host__ __device__ inline int face_segments(const BoxDev &b, int d)
@@ -137,8 +138,10 @@ __host__ __device__ inline long long class_items(const BoxDev &b, int c)
         return (fx_cells(b) - 1 + 62) / 63 * 64;
     case C_FY:
         return (long long)b.in[0] * b.in[2] * face_segments(b, 1);
-    default:
+    case C_FZ:
         return (long long)b.in[0] * b.in[1] * face_segments(b, 2);
+    default: /* C_FALL is made of the three face tables */
+        return 0;
     }
 }
 
@@ -609,11 +612,11 @@ template <class T> __device__ __forceinline__ T shfl_next(T v)
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-    k_faces_x(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
-              Background bg, Pools p)
+__device__ __forceinline__ void faces_x_block(const BoxDev *__restrict__ boxes,
+                                              const int2 bt, const Options &o,
+                                              const Background &bg,
+                                              const Pools &p)
 {
-    const int2 bt = items[blockIdx.x];
     const BoxDev &b = boxes[bt.x];
     const int lane = threadIdx.x & 63;
     const long long wave = ((long long)bt.y + threadIdx.x) >> 6;
@@ -677,6 +680,14 @@ __global__ void __launch_bounds__(kBlock)
     store_flux(face_pool<T>(p, b, 0), nf, fo, f);
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_faces_x(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
+              Background bg, Pools p)
+{
+    faces_x_block<T>(boxes, items[blockIdx.x], o, bg, p);
+}
+
 /* ------------------------------------------------------------------ */
 /* k_faces<D> (D = 1, 2): each thread walks b.seg consecutive faces    */
 /* along D carrying the right cell's primitives, the next face's left  */
@@ -684,14 +695,13 @@ __global__ void __launch_bounds__(kBlock)
 /* computed once per walk; lanes are consecutive along i.              */
 /* ------------------------------------------------------------------ */
 template <int D, class T>
-__global__ void __launch_bounds__(kBlock)
-    k_faces(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o, Background bg,
-            Pools p)
+__device__ __forceinline__ void faces_march_block(
+    const BoxDev *__restrict__ boxes, const int2 bt, const Options &o,
+    const Background &bg, const Pools &p)
 {
-    const BoxDev *bp;
-    const long long it = block_item(items, boxes, D == 1 ? C_FY : C_FZ, bp);
-    const BoxDev &b = *bp;
-    if (it < 0)
+    const BoxDev &b = boxes[bt.x];
+    const long long it = (long long)bt.y + threadIdx.x;
+    if (it >= class_items(b, D == 1 ? C_FY : C_FZ))
         return;
     constexpr int B = (D == 2) ? 1 : 2; /* the other slow axis */
     const int na = b.in[0], nb = b.in[B];
@@ -761,6 +771,35 @@ __global__ void __launch_bounds__(kBlock)
             fo += f_stride;
         }
     }
+}
+
+template <int D, class T>
+__global__ void __launch_bounds__(kBlock)
+    k_faces(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o, Background bg,
+            Pools p)
+{
+    faces_march_block<D, T>(boxes, items[blockIdx.x], o, bg, p);
+}
+
+/* All three face directions of a call in one launch, for levels too small
+ * to fill the device with one direction at a time: the blocks of the
+ * C_FALL table are the x blocks, then the y blocks, then the z blocks,
+ * so every wavefront runs one direction's code.  The kernel needs the
+ * registers of the marching kernels, which costs the x blocks occupancy;
+ * on small levels that matters less than the two launch boundaries it
+ * removes. */
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_faces_all(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, int nbx, int nby,
+                Options o, Background bg, Pools p)
+{
+    const int2 bt = items[blockIdx.x];
+    if ((int)blockIdx.x < nbx)
+        faces_x_block<T>(boxes, bt, o, bg, p);
+    else if ((int)blockIdx.x < nbx + nby)
+        faces_march_block<1, T>(boxes, bt, o, bg, p);
+    else
+        faces_march_block<2, T>(boxes, bt, o, bg, p);
 }
 
 enum CellMode

@@ -430,6 +430,37 @@ def test_level_64_box_tiling_equals_single_box(hip_context):
                               whole[sl])
 
 
+def test_face_launch_forms_agree(hip_context):
+    """Calls below 3 * 2^20 interior cells launch the three face directions
+    as one fused kernel with walks of one face (on a 32^3 box); larger calls
+    launch them as three kernels on concurrent streams with walks of 8.
+    The 32^3 Michel box advanced alone equals the same box advanced in a
+    level whose 160^3 second box (its own domain, at shifted indices) puts
+    the call above the threshold, bit for bit."""
+    from grdzhadzha_amd.options import BoxSpec
+    w = workloads.michel_box_32()
+    alone = LevelOperator(w.background, [w.box], w.options,
+                          context=hip_context)
+    pool = alone.pack([w.state])
+    alone.advance_rk4(pool, w.dt, nsteps=2)
+    ref = alone.unpack(pool)[0]
+    big = workloads.michel_box_32(n=160, half_width=40.0)
+    off = 1000
+    b = big.box
+    shifted = BoxSpec((b.lo[0] + off, b.lo[1], b.lo[2]),
+                      (b.hi[0] + off, b.hi[1], b.hi[2]), b.ghosts, b.dx,
+                      (b.center[0] + off * b.dx, b.center[1], b.center[2]))
+    assert big.options == w.options and big.background == w.background
+    lev = LevelOperator(w.background, [w.box, shifted], w.options,
+                        context=hip_context)
+    assert lev.num_interior >= 3 << 20
+    p2 = lev.pack([w.state, big.state])
+    del big
+    lev.advance_rk4(p2, w.dt, nsteps=2)
+    got = lev.unpack(p2)[0]
+    assert np.array_equal(got, ref)
+
+
 def test_level_plan_for_4096_boxes_builds_quickly(hip_context):
     """A 4096-box level (8^3 boxes tiling 128^3, shared domain): the launch
     plan, with its exchange list of 9.1M ghost cells, builds in seconds;
