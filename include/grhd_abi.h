@@ -463,7 +463,8 @@ enum
 {
     GRHD_KERNEL_PRIMS = 0,   /* ghost fill + con2prim over the storage box */
     GRHD_KERNEL_FACES_X = 1, /* MUSCL-HLLE faces: the three directions run
-                                concurrently and are timed together here  */
+                                as one launch or concurrently and are
+                                timed together here                       */
     GRHD_KERNEL_FACES_Y = 2, /* (no longer used)                          */
     GRHD_KERNEL_FACES_Z = 3, /* (no longer used)                          */
     GRHD_KERNEL_CELLS = 4,   /* divergence + sources + stage update       */
