@@ -51,9 +51,15 @@ ALG_BYTES = {"faces_x": 88, "faces_y": 88, "faces_z": 88,
 STEP_ALG_BYTES = 120 + 200 + 200 + 120 + 48 + 1
 
 
+# The "faces_x" timing class covers the three face directions of a stage
+# (one fused launch, or three concurrent ones), so its unit count, FLOPs and
+# traffic are those of all three.
+FACE_KERNELS = ("faces_x", "faces_y", "faces_z")
+
+
 def units_per_launch(cls, n, g):
-    if cls.startswith("faces"):
-        return (n + 1) * n * n
+    if cls == "faces_x":
+        return 3 * (n + 1) * n * n
     if cls == "cells":
         return n ** 3
     if cls == "prims":
@@ -70,7 +76,11 @@ def measured_traffic(cls, n):
         return None
     with open(path) as f:
         data = json.load(f)
-    return data.get("bytes_per_launch", {}).get(cls)
+    per = data.get("bytes_per_launch", {})
+    if cls == "faces_x":
+        parts = [per.get(k) for k in FACE_KERNELS]
+        return None if None in parts else sum(parts)
+    return per.get(cls)
 
 
 def measured_flops(cls, n):
@@ -82,7 +92,11 @@ def measured_flops(cls, n):
         return None
     with open(path) as f:
         data = json.load(f)
-    return data.get("flops_per_launch", {}).get(cls)
+    per = data.get("flops_per_launch", {})
+    if cls == "faces_x":
+        parts = [per.get(k) for k in FACE_KERNELS]
+        return None if None in parts else sum(parts)
+    return per.get(cls)
 
 
 def cpu_baseline(w, steps=3):
