@@ -619,9 +619,9 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not "
-              "supported (the box would shard by z-slabs across GPUs, which "
-              "is not built)" % args.gpus, file=sys.stderr)
+        print("bench.py: the operator runs on one GPU only; --gpus %d is "
+              "not supported (the box would shard by z-slabs across GPUs, "
+              "which is not implemented)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batched":

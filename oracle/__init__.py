@@ -1,7 +1,8 @@
 """Python view of the CPU oracle (oracle/libgrhd_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-the cpu_baseline leg of bench.py as the checker; never by the product.
+the cpu_baseline leg of bench.py, to check and time against; never by the
+product.
 Arrays are numpy float64 in FArrayBox layout (ncomp, nz, ny, nx).
 """
 import ctypes
