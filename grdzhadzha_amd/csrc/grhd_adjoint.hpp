@@ -4,10 +4,15 @@
  * Every kernel of the forward step is a function of a few cells' data (a
  * face flux of its 4-cell stencil, a cell's sources and recovery of that
  * cell).  The adjoint needs the transposed local Jacobians of these
- * functions applied to adjoint vectors.  They are formed column by column
- * in forward mode: the same templated physics as the forward kernels is
- * evaluated on Dual numbers (value + one tangent), once per local input,
- * and the tangent of the outputs is dotted with the adjoint of the outputs.
+ * functions applied to adjoint vectors.
+ * - The face flux's, in its two face states and Gamma, is applied in one
+ *   hand-written reverse sweep (face_flux_reverse); the reconstruction's
+ *   per-component Jacobian then takes it to the four stencil cells.
+ * - The others, and the spin's tangent everywhere, are formed column by
+ *   column in forward mode: the same templated physics as the forward
+ *   kernels is evaluated on Dual numbers (value + one tangent), once per
+ *   local input, and the tangent of the outputs is dotted with the
+ *   adjoint of the outputs.
  * Branches (limiter choice, floors, HLLE wave-speed signs, atmosphere
  * skips and resets, the v^2 clamp) compare values only, so they are frozen
  * at the forward pass's decisions.  con2prim is differentiated at its
@@ -280,53 +285,282 @@ __device__ __forceinline__ void face_flux_geo(const PrimT<T> w[4],
     hlle_face_g(lp, rp, o, gamma, face, D, f);
 }
 
-/* One state of the HLLE face (hlle_face_g's per-state part, in its
- * order): floors on the face metric, conserved variables, flux and
- * characteristic speeds. */
-template <class T> struct FaceSide
+/* ------------------------------------------------------------------ */
+/* Reverse mode of one face flux in its two face states.               */
+/* side_forward evaluates face_side in double, keeping the values the   */
+/* backward sweep needs and the branches it took; hlle_reverse and      */
+/* side_reverse then apply the transposed Jacobian of hlle_combine and  */
+/* of each side (floors, prim2con, flux_of, char_speeds) to lambda in   */
+/* one sweep.  Every branch (the floors, the v^2 clamp, the radicand    */
+/* clamp, the wave-speed selections) is taken as the forward evaluation */
+/* takes it, with the same tie rules (std_max keeps its first argument  */
+/* on a tie, std_min too).  One sweep replaces the 12 + 1 forward-mode  */
+/* passes over the face states and Gamma.                              */
+/* ------------------------------------------------------------------ */
+struct SideFwd
 {
-    ConsT<T> c, f;
-    T lo, hi;
+    Prim q;                  /* the state after the floors */
+    bool rho_kept, p_kept;   /* floors left rho / P as they were */
+    bool scaled;             /* the v^2 clamp rescaled v */
+    double v_in[3], v2pre, scale;
+    double vL[3], v2, W, h, rhW2;
+    double c[5], f[5];       /* D, S1, S2, S3, tau */
+    double tv;
+    double cs2, cs, den, rad, sq, lo, hi, lower, upper;
 };
 
-template <class T, class G>
-__device__ __forceinline__ void face_side(PrimT<T> p, const Options &o,
-                                          T gamma, const GeoT<G> &face,
-                                          int d, FaceSide<T> &s)
+__device__ __forceinline__ void side_forward(const Prim &p, const Options &o,
+                                             double gamma, const Geo &g,
+                                             int d, SideFwd &s)
 {
-    floors_g(p, o, gamma, face.g);
-    prim2con(p, gamma, face.g, s.c);
-    flux_of(p, s.c, d, face.lapse, face.shift, s.f);
-    char_speeds(p, gamma, face, d, s.lo, s.hi);
-}
-
-/* hlle_face_g's combination of the two states. */
-template <class T>
-__device__ __forceinline__ void hlle_combine(const FaceSide<T> &l,
-                                             const FaceSide<T> &r,
-                                             ConsT<T> &f)
-{
-    const T sl = std_min(T(0), std_min(l.lo, r.lo));
-    const T su = std_max(T(0), std_max(l.hi, r.hi));
-    if (sl >= T(0))
-    {
-        f = l.f;
-        return;
-    }
-    if (su <= T(0))
-    {
-        f = r.f;
-        return;
-    }
-    const T inv = T(1) / (su - sl);
-    f.D = (su * l.f.D - sl * r.f.D + sl * su * (r.c.D - l.c.D)) * inv;
-    f.tau = (su * l.f.tau - sl * r.f.tau + sl * su * (r.c.tau - l.c.tau)) *
-            inv;
+    /* floors_g */
+    Prim q = p;
+    s.rho_kept = !(q.rho < o.rho_floor);
+    s.p_kept = !(q.pressure < o.pressure_floor);
+    if (!s.rho_kept)
+        q.rho = o.rho_floor;
+    if (!s.p_kept)
+        q.pressure = o.pressure_floor;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        f.S[i] = (su * l.f.S[i] - sl * r.f.S[i] +
-                  sl * su * (r.c.S[i] - l.c.S[i])) *
-                 inv;
+        s.v_in[i] = q.v[i];
+    s.v2pre = vsq(q.v, g.g);
+    s.scaled = s.v2pre >= o.max_v2;
+    s.scale = 1.0;
+    if (s.scaled)
+    {
+        s.scale = sqrt(o.max_v2 / s.v2pre);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            q.v[i] *= s.scale;
+    }
+    q.eps = q.pressure / ((gamma - 1.0) * q.rho);
+    s.q = q;
+    /* prim2con */
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        s.vL[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            s.vL[i] += g.g[i][j] * q.v[j];
+    }
+    s.v2 = vsq(q.v, g.g);
+    s.W = 1.0 / sqrt(1.0 - s.v2);
+    s.h = 1.0 + q.eps + q.pressure / q.rho;
+    s.rhW2 = q.rho * s.h * s.W * s.W;
+    s.c[0] = q.rho * s.W;
+    s.c[4] = s.rhW2 - q.pressure - s.c[0];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        s.c[1 + i] = s.rhW2 * s.vL[i];
+    /* flux_of */
+    const double vel = q.v[d];
+    s.tv = g.lapse * vel - g.shift[d];
+    s.f[0] = s.c[0] * s.tv;
+    s.f[4] = s.c[4] * s.tv + g.lapse * q.pressure * vel;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+    {
+        s.f[1 + j] = s.c[1 + j] * s.tv;
+        if (j == d)
+            s.f[1 + j] += g.lapse * q.pressure;
+    }
+    /* char_speeds */
+    s.cs2 = gamma * q.pressure / (q.rho * s.h);
+    s.cs = sqrt(s.cs2);
+    s.den = 1.0 - s.v2 * s.cs2;
+    s.rad = (1.0 - s.v2) *
+            (g.gu[d][d] * (1.0 - s.v2 * s.cs2) - vel * vel * (1.0 - s.cs2));
+    s.sq = sqrt(0.0 < s.rad ? s.rad : 0.0);
+    const double root = s.cs * s.sq;
+    s.lo = (vel * (1.0 - s.cs2) - root) / s.den;
+    s.hi = (vel * (1.0 - s.cs2) + root) / s.den;
+    s.lower = g.lapse * s.lo - g.shift[d];
+    s.upper = g.lapse * s.hi - g.shift[d];
+}
+
+/* vb += w d(v^T g v)/dv */
+__device__ __forceinline__ void vsq_bar(const double v[3], const double g[3][3],
+                                        double w, double vb[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            t += (g[k][j] + g[j][k]) * v[j];
+        vb[k] += w * t;
+    }
+}
+
+/* The adjoints of one side's conserved state, flux and wave speeds
+ * (cb, fb, lob, hib) taken back to the face state before the floors
+ * (pb: rho, eps, P, v1..v3) and to Gamma (added into gb). */
+__device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
+                                             const Geo &g, int d,
+                                             const double cb[5],
+                                             const double fb[5], double lob,
+                                             double hib, double pb[6],
+                                             double &gb)
+{
+    const Prim &q = s.q;
+    const double vel = q.v[d];
+    double rho_b = 0.0, P_b = 0.0, eps_b = 0.0, gam_b = 0.0, v2_b = 0.0;
+    double v_b[3] = {0.0, 0.0, 0.0};
+    /* char_speeds */
+    const double lo_b = g.lapse * lob, hi_b = g.lapse * hib;
+    const double A_b = (lo_b + hi_b) / s.den;
+    const double root_b = (hi_b - lo_b) / s.den;
+    const double den_b = -(lo_b * s.lo + hi_b * s.hi) / s.den;
+    double vel_b = A_b * (1.0 - s.cs2);
+    double cs2_b = -A_b * vel;
+    const double cs_b = root_b * s.sq;
+    const double sq_b = root_b * s.cs;
+    const double rad_b =
+        (0.0 < s.rad && s.sq > 0.0) ? sq_b / (2.0 * s.sq) : 0.0;
+    const double B = g.gu[d][d] * (1.0 - s.v2 * s.cs2) - vel * vel * (1.0 - s.cs2);
+    const double B_b = rad_b * (1.0 - s.v2);
+    v2_b -= rad_b * B;
+    v2_b -= B_b * g.gu[d][d] * s.cs2;
+    cs2_b -= B_b * g.gu[d][d] * s.v2;
+    vel_b -= B_b * 2.0 * vel * (1.0 - s.cs2);
+    cs2_b += B_b * vel * vel;
+    v2_b -= den_b * s.cs2;
+    cs2_b -= den_b * s.v2;
+    if (s.cs > 0.0)
+        cs2_b += cs_b / (2.0 * s.cs);
+    gam_b += cs2_b * q.pressure / (q.rho * s.h);
+    P_b += cs2_b * gamma / (q.rho * s.h);
+    rho_b -= cs2_b * s.cs2 / q.rho;
+    double h_b = -cs2_b * s.cs2 / s.h;
+    v_b[d] += vel_b;
+    /* flux_of */
+    double tv_b = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        tv_b += fb[k] * s.c[k];
+    double c_b[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        c_b[k] = cb[k] + fb[k] * s.tv;
+    P_b += fb[4] * g.lapse * vel + fb[1 + d] * g.lapse;
+    v_b[d] += fb[4] * g.lapse * q.pressure + tv_b * g.lapse;
+    /* prim2con: tau = rhW2 - P - D, S_i = rhW2 vL_i, D = rho W */
+    double rhW2_b = c_b[4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        rhW2_b += c_b[1 + i] * s.vL[i];
+    P_b -= c_b[4];
+    const double D_b = c_b[0] - c_b[4];
+    rho_b += D_b * s.W;
+    double W_b = D_b * q.rho;
+    rho_b += rhW2_b * s.h * s.W * s.W;
+    h_b += rhW2_b * q.rho * s.W * s.W;
+    W_b += rhW2_b * 2.0 * q.rho * s.h * s.W;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            v_b[j] += c_b[1 + i] * s.rhW2 * g.g[i][j];
+    /* h = 1 + eps + P / rho (prim2con's and char_speeds' alike) */
+    eps_b += h_b;
+    P_b += h_b / q.rho;
+    rho_b -= h_b * q.pressure / (q.rho * q.rho);
+    /* W = (1 - v^2)^(-1/2), v^2 shared with char_speeds */
+    v2_b += W_b * 0.5 * s.W * s.W * s.W;
+    vsq_bar(q.v, g.g, v2_b, v_b);
+    /* floors: eps = P / ((Gamma - 1) rho), then the v^2 clamp */
+    P_b += eps_b / ((gamma - 1.0) * q.rho);
+    rho_b -= eps_b * q.eps / q.rho;
+    gam_b -= eps_b * q.eps / (gamma - 1.0);
+    double vin_b[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        vin_b[i] = v_b[i];
+    if (s.scaled)
+    {
+        double scale_b = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            scale_b += v_b[i] * s.v_in[i];
+            vin_b[i] = v_b[i] * s.scale;
+        }
+        vsq_bar(s.v_in, g.g, -0.5 * scale_b * s.scale / s.v2pre, vin_b);
+    }
+    pb[0] = s.rho_kept ? rho_b : 0.0;
+    pb[1] = 0.0; /* eps is recomputed by the floors */
+    pb[2] = s.p_kept ? P_b : 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        pb[3 + i] = vin_b[i];
+    gb += gam_b;
+}
+
+/* lambda . F(lp, rp) differentiated in the two face states (lpb, rpb, in
+ * the order rho, eps, P, v1..v3) and in Gamma (added into gb). */
+__device__ __forceinline__ void face_flux_reverse(const Prim &lp,
+                                                  const Prim &rp,
+                                                  const Options &o,
+                                                  double gamma, const Geo &g,
+                                                  int d, const double lam[5],
+                                                  double lpb[6], double rpb[6],
+                                                  double &gb)
+{
+    SideFwd l, r;
+    side_forward(lp, o, gamma, g, d, l);
+    side_forward(rp, o, gamma, g, d, r);
+    double lcb[5] = {0, 0, 0, 0, 0}, lfb[5] = {0, 0, 0, 0, 0};
+    double rcb[5] = {0, 0, 0, 0, 0}, rfb[5] = {0, 0, 0, 0, 0};
+    double llob = 0.0, lhib = 0.0, rlob = 0.0, rhib = 0.0;
+    /* hlle_combine */
+    const double in_lo = (r.lower < l.lower) ? r.lower : l.lower;
+    const double sl = (in_lo < 0.0) ? in_lo : 0.0;
+    const double in_hi = (l.upper < r.upper) ? r.upper : l.upper;
+    const double su = (0.0 < in_hi) ? in_hi : 0.0;
+    if (sl >= 0.0)
+    {
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            lfb[k] = lam[k];
+    }
+    else if (su <= 0.0)
+    {
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            rfb[k] = lam[k];
+    }
+    else
+    {
+        /* f = (su F_L - sl F_R + sl su (U_R - U_L)) / (su - sl), with sl
+         * from the smaller lower speed and su from the larger upper one */
+        const double inv = 1.0 / (su - sl);
+        double sl_b = 0.0, su_b = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+        {
+            const double dc = r.c[k] - l.c[k];
+            const double fk = (su * l.f[k] - sl * r.f[k] + sl * su * dc) * inv;
+            lfb[k] = lam[k] * su * inv;
+            rfb[k] = -lam[k] * sl * inv;
+            rcb[k] = lam[k] * sl * su * inv;
+            lcb[k] = -rcb[k];
+            su_b += lam[k] * ((l.f[k] + sl * dc) * inv - fk * inv);
+            sl_b += lam[k] * ((-r.f[k] + su * dc) * inv + fk * inv);
+        }
+        if (r.lower < l.lower)
+            rlob = sl_b;
+        else
+            llob = sl_b;
+        if (l.upper < r.upper)
+            rhib = su_b;
+        else
+            lhib = su_b;
+    }
+    side_reverse(l, gamma, g, d, lcb, lfb, llob, lhib, lpb, gb);
+    side_reverse(r, gamma, g, d, rcb, rfb, rlob, rhib, rpb, gb);
 }
 
 /* Component m of a primitive state, in W's order (rho, eps, P, v). */
@@ -426,17 +660,6 @@ __device__ __forceinline__ ConsT<Dual> dual_cons(const Cons &c, int slot)
         q.S[m] = Dual(c.S[m], slot == 1 + m ? 1.0 : 0.0);
     q.tau = Dual(c.tau, slot == 4 ? 1.0 : 0.0);
     return q;
-}
-
-/* A double face state as duals with zero tangents: the state a pass does
- * not seed. */
-__device__ __forceinline__ void dual_side(const FaceSide<double> &v,
-                                          FaceSide<Dual> &s)
-{
-    s.c = dual_cons(v.c, -1);
-    s.f = dual_cons(v.f, -1);
-    s.lo = Dual(v.lo);
-    s.hi = Dual(v.hi);
 }
 
 /* conserved component m of ConsT (D, S1, S2, S3, tau) */
@@ -589,7 +812,7 @@ __device__ __forceinline__ void load_geo(const MetricCache &mc,
 /* One thread per face of the interior face box.                       */
 /* ------------------------------------------------------------------ */
 template <int D, bool SPIN>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
     k_adj_faces(BoxDev b, Options o, AdjParams pa, MetricCache mc,
                 const double *W, const double *lam_k, double *lam_W,
                 double *lam_par)
@@ -670,44 +893,11 @@ __global__ void __launch_bounds__(kBlock)
                     slope_step(wv[0], wv[1], wv[2], o.theta, unused, lpv);
                     slope_step(wv[1], wv[2], wv[3], o.theta, rpv, unused);
                 }
-                /* lambda . dF/d(face states): 6 + 6 passes, each with the
-                 * seeded state in duals and the other from its values,
-                 * then Gamma (both states) */
-                FaceSide<double> lv, rv;
-                face_side(lpv, o, pa.gamma, facev, D, lv);
-                face_side(rpv, o, pa.gamma, facev, D, rv);
-                FaceSide<Dual> l0, r0;
-                dual_side(lv, l0);
-                dual_side(rv, r0);
+                /* lambda . dF/d(face states) and d/dGamma in one reverse
+                 * sweep */
                 double lam_lp[6], lam_rp[6];
-                for (int s = 0; s < 12; ++s)
-                {
-                    const Dual gamma(pa.gamma);
-                    FaceSide<Dual> sd;
-                    ConsT<Dual> f;
-                    if (s < 6)
-                    {
-                        face_side(dual_prim(lpv, s), o, gamma, facev, D, sd);
-                        hlle_combine(sd, r0, f);
-                        lam_lp[s] = dot_tangent(f, lam);
-                    }
-                    else
-                    {
-                        face_side(dual_prim(rpv, s - 6), o, gamma, facev, D,
-                                  sd);
-                        hlle_combine(l0, sd, f);
-                        lam_rp[s - 6] = dot_tangent(f, lam);
-                    }
-                }
-                {
-                    const Dual gamma(pa.gamma, 1.0);
-                    FaceSide<Dual> ls, rs;
-                    face_side(dual_prim(lpv, -1), o, gamma, facev, D, ls);
-                    face_side(dual_prim(rpv, -1), o, gamma, facev, D, rs);
-                    ConsT<Dual> f;
-                    hlle_combine(ls, rs, f);
-                    g0 += dot_tangent(f, lam);
-                }
+                face_flux_reverse(lpv, rpv, o, pa.gamma, facev, D, lam, lam_lp,
+                                  lam_rp, g0);
                 /* back through the reconstruction, component by component:
                  * lp = W_L + s_L/2 from (W_{L-1}, W_L, W_R), rp = W_R - s_R/2
                  * from (W_L, W_R, W_{R+1}) */
@@ -717,14 +907,16 @@ __global__ void __launch_bounds__(kBlock)
                     double gq[4] = {0.0, lam_lp[m], lam_rp[m], 0.0};
                     if (o.use_reconstruction)
                     {
+                        /* the stencil's W read again rather than kept in
+                         * registers through the reverse sweep */
+                        double wm[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            wm[q] = W[(long long)m * n + so[q]];
                         double jl[3], jr[3];
-                        half_step_jacobian(prim_comp(wv[0], m),
-                                           prim_comp(wv[1], m),
-                                           prim_comp(wv[2], m), o.theta, 0.5,
+                        half_step_jacobian(wm[0], wm[1], wm[2], o.theta, 0.5,
                                            jl);
-                        half_step_jacobian(prim_comp(wv[1], m),
-                                           prim_comp(wv[2], m),
-                                           prim_comp(wv[3], m), o.theta, -0.5,
+                        half_step_jacobian(wm[1], wm[2], wm[3], o.theta, -0.5,
                                            jr);
                         gq[0] = lam_lp[m] * jl[0];
                         gq[1] = lam_lp[m] * jl[1] + lam_rp[m] * jr[0];
