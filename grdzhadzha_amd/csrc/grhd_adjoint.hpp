@@ -628,6 +628,156 @@ __device__ __forceinline__ bool sources_geo(const PrimT<T> &w,
     return true;
 }
 
+/* lambda . S(w, u), the static sources after the floors and the
+ * atmosphere skip (sources_geo on a double geometry), differentiated in
+ * one reverse sweep: in w (wb: rho, eps, P, v1..v3), u (ub: D, S1..S3,
+ * tau) and Gamma (added into gb).  Branches as the forward evaluation
+ * takes them (floors_g's ties keep the state).  Returns false where the
+ * cell is skipped: the sources do not depend on it. */
+__device__ __forceinline__ bool sources_reverse(const Prim &w, const Cons &u,
+                                                const FullGeo &fg,
+                                                double gamma, const Options &o,
+                                                const double lam[5],
+                                                double wb[6], double ub[5],
+                                                double &gb)
+{
+    const Geo &g = fg.geo;
+    /* floors_g */
+    Prim q = w;
+    const bool rho_kept = !(q.rho < o.rho_floor);
+    const bool p_kept = !(q.pressure < o.pressure_floor);
+    if (!rho_kept)
+        q.rho = o.rho_floor;
+    if (!p_kept)
+        q.pressure = o.pressure_floor;
+    double v_in[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        v_in[i] = q.v[i];
+    const double v2pre = vsq(q.v, g.g);
+    const bool scaled = v2pre >= o.max_v2;
+    double scale = 1.0;
+    if (scaled)
+    {
+        scale = sqrt(o.max_v2 / v2pre);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            q.v[i] *= scale;
+    }
+    q.eps = q.pressure / ((gamma - 1.0) * q.rho);
+    if (is_atmosphere(q, o, o.source_atmosphere_factor))
+        return false;
+    /* static_sources' values */
+    const double W = 1.0 / sqrt(1.0 - vsq(q.v, g.g));
+    const double h = 1.0 + q.eps + q.pressure / q.rho;
+    const double rhW2 = q.rho * h * W * W;
+    const double lD = lam[0], lT = lam[4];
+    const double lS[3] = {lam[1], lam[2], lam[3]};
+    double rho_b = 0.0, P_b = 0.0, eps_b = 0.0, gam_b = 0.0;
+    double v_b[3] = {0.0, 0.0, 0.0};
+    double D_b = 0.0, tau_b = 0.0, S_b[3] = {0.0, 0.0, 0.0};
+    /* E = D + tau enters S_m as -E dalpha_m */
+    double E_b = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        E_b -= lS[m] * fg.dlapse[m];
+    D_b += E_b;
+    tau_b += E_b;
+    /* gamma^ij S_j enters tau as -mU_d dalpha_d; S_s dbeta^s_m enters S_m */
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const double mU_b = -lT * fg.dlapse[d];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            S_b[j] += mU_b * g.gu[d][j];
+    }
+#pragma unroll
+    for (int sdir = 0; sdir < 3; ++sdir)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            S_b[sdir] += lS[m] * fg.dshift[sdir][m];
+    /* the fluxes F^d enter as -F^d d_d ln sqrt(gamma) */
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const double tv = g.lapse * q.v[d] - g.shift[d];
+        const double cD_b = -lD * fg.dlogsqrtdet[d];
+        const double cT_b = -lT * fg.dlogsqrtdet[d];
+        double cS_b[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            cS_b[m] = -lS[m] * fg.dlogsqrtdet[d];
+        double tv_b = cD_b * u.D + cT_b * u.tau;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            tv_b += cS_b[j] * u.S[j];
+        D_b += cD_b * tv;
+        tau_b += cT_b * tv;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            S_b[j] += cS_b[j] * tv;
+        P_b += cT_b * g.lapse * q.v[d] + cS_b[d] * g.lapse;
+        v_b[d] += cT_b * g.lapse * q.pressure + tv_b * g.lapse;
+    }
+    /* T^ij = rho h W^2 v^i v^j + P gamma^ij, contracted with alpha K_ij
+     * (tau) and alpha dgamma_ij / 2 (S_m) */
+    double rhW2_b = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            double t_b = lT * g.lapse * fg.K[i][j];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                t_b += lS[m] * 0.5 * g.lapse * fg.dg[m][i][j];
+            rhW2_b += t_b * q.v[i] * q.v[j];
+            v_b[i] += t_b * rhW2 * q.v[j];
+            v_b[j] += t_b * rhW2 * q.v[i];
+            P_b += t_b * g.gu[i][j];
+        }
+    rho_b += rhW2_b * h * W * W;
+    const double h_b = rhW2_b * q.rho * W * W;
+    const double W_b = rhW2_b * 2.0 * q.rho * h * W;
+    eps_b += h_b;
+    P_b += h_b / q.rho;
+    rho_b -= h_b * q.pressure / (q.rho * q.rho);
+    vsq_bar(q.v, g.g, W_b * 0.5 * W * W * W, v_b);
+    /* floors: eps = P / ((Gamma - 1) rho), then the v^2 clamp */
+    P_b += eps_b / ((gamma - 1.0) * q.rho);
+    rho_b -= eps_b * q.eps / q.rho;
+    gam_b -= eps_b * q.eps / (gamma - 1.0);
+    double vin_b[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        vin_b[i] = v_b[i];
+    if (scaled)
+    {
+        double scale_b = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            scale_b += v_b[i] * v_in[i];
+            vin_b[i] = v_b[i] * scale;
+        }
+        vsq_bar(v_in, g.g, -0.5 * scale_b * scale / v2pre, vin_b);
+    }
+    wb[0] = rho_kept ? rho_b : 0.0;
+    wb[1] = 0.0; /* eps is recomputed by the floors */
+    wb[2] = p_kept ? P_b : 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        wb[3 + i] = vin_b[i];
+    ub[0] = D_b;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        ub[1 + i] = S_b[i];
+    ub[4] = tau_b;
+    gb += gam_b;
+    return true;
+}
+
 /* ------------------------------------------------------------------ */
 /* Seeding helpers.                                                    */
 /* ------------------------------------------------------------------ */
@@ -984,31 +1134,18 @@ __global__ void __launch_bounds__(kBlock)
                 /* the full geometry in double: no pass here seeds it */
                 FullGeo fgv;
                 source_geo(x, pa.M, pa.a, fgv);
-                double gW[6] = {0, 0, 0, 0, 0, 0};
-                /* 6 W inputs, 5 U inputs, Gamma */
-                for (int s = 0; s < 12; ++s)
+                /* 6 W inputs, 5 U inputs and Gamma in one reverse sweep */
+                double gW[6], gU[5];
+                if (sources_reverse(wv, uv, fgv, pa.gamma, o, lam, gW, gU, g0))
                 {
-                    const int seed = s < 11 ? 0 : SEED_GAMMA;
-                    Dual M, a, gamma;
-                    seed_params(pa, seed, M, a, gamma);
-                    const PrimT<Dual> wd = dual_prim(wv, s < 6 ? s : -1);
-                    const ConsT<Dual> ud =
-                        dual_cons(uv, (s >= 6 && s < 11) ? s - 6 : -1);
-                    ConsT<Dual> src;
-                    if (!sources_geo(wd, ud, fgv, gamma, o, src))
-                        break; /* skipped cell: no dependence */
-                    const double g = dot_tangent(src, lam);
-                    if (s < 6)
-                        gW[s] = g;
-                    else if (s < 11)
-                        lam_U[(long long)(s - 6) * ni + it] += g;
-                    else
-                        g0 += g;
-                }
 #pragma unroll
-                for (int m = 0; m < 6; ++m)
-                    if (gW[m] != 0.0)
-                        atomicAdd(&lam_W[m * n + so], gW[m]);
+                    for (int m = 0; m < 5; ++m)
+                        lam_U[(long long)m * ni + it] += gU[m];
+#pragma unroll
+                    for (int m = 0; m < 6; ++m)
+                        if (gW[m] != 0.0)
+                            atomicAdd(&lam_W[m * n + so], gW[m]);
+                }
             }
         }
     }
