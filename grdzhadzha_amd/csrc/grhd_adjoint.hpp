@@ -1218,6 +1218,238 @@ __device__ __forceinline__ void w_of_state(const GeoT<G> &gt,
 }
 
 /* ------------------------------------------------------------------ */
+/* Reverse mode of the recovery chain (w_of_state in double).           */
+/* ------------------------------------------------------------------ */
+
+/* con2prim_at_root in double: the values its reverse sweep needs. */
+struct RootFwd
+{
+    double S2, p, epp, v2, W;
+    Prim w;
+    bool free_p; /* the root moves with U (not the recovery floor) */
+};
+
+__device__ __forceinline__ void root_forward(const Cons &c, const Options &o,
+                                             double gamma,
+                                             const double gu[3][3], double ps,
+                                             RootFwd &r)
+{
+    r.S2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+            r.S2 += gu[m][n] * c.S[m] * c.S[n];
+    r.p = ps;
+    r.free_p = ps != o.rec_pressure_floor;
+    r.epp = c.tau + c.D + ps;
+    r.v2 = r.S2 / (r.epp * r.epp);
+    r.W = 1.0 / sqrt(1.0 - r.v2);
+    r.w.rho = c.D / r.W;
+    r.w.pressure = ps;
+    r.w.eps = ps / ((gamma - 1.0) * r.w.rho);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        double vi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            vi += gu[i][j] * c.S[j] / r.epp;
+        r.w.v[i] = vi;
+    }
+}
+
+/* The adjoint of the recovered primitives (wb: rho, eps, P, v1..v3) back
+ * to the conserved state (cb: D, S1..S3, tau) and Gamma (added into gb).
+ * The root pressure moves by the implicit-function theorem,
+ * dp = -(df with p held) / (df/dp), on presid with the v^2 clamp as the
+ * solve left it; a root at the recovery floor is a constant. */
+__device__ __forceinline__ void root_reverse(const Cons &c, const Options &o,
+                                             double gamma,
+                                             const double gu[3][3],
+                                             const RootFwd &r,
+                                             const double wb[6], double cb[5],
+                                             double &gb)
+{
+    const Prim &w = r.w;
+    double p_b = wb[2], rho_b = wb[0], D_b = 0.0, tau_b = 0.0, epp_b = 0.0;
+    double S_b[3] = {0.0, 0.0, 0.0};
+    /* eps = p / ((Gamma - 1) rho) */
+    p_b += wb[1] / ((gamma - 1.0) * w.rho);
+    rho_b -= wb[1] * w.eps / w.rho;
+    gb -= wb[1] * w.eps / (gamma - 1.0);
+    /* v^i = gamma^ij S_j / (E + p) */
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            S_b[j] += wb[3 + i] * gu[i][j] / r.epp;
+        epp_b -= wb[3 + i] * w.v[i] / r.epp;
+    }
+    /* rho = D / W, W = (1 - v^2)^(-1/2), v^2 = S^2 / (E + p)^2 */
+    D_b += rho_b / r.W;
+    const double W_b = -rho_b * w.rho / r.W;
+    const double v2_b = W_b * 0.5 * r.W * r.W * r.W;
+    double S2_b = v2_b / (r.epp * r.epp);
+    epp_b -= 2.0 * v2_b * r.v2 / r.epp;
+    D_b += epp_b;
+    tau_b += epp_b;
+    p_b += epp_b;
+    if (r.free_p)
+    {
+        /* presid(p) = D W' + Gamma p W'^2 / (Gamma - 1) - (E + p), with
+         * v'^2 = min-clamped S^2 / (E + p)^2 */
+        const double e = r.epp;
+        const double v2r = r.S2 / (e * e);
+        const bool clamped = v2r > o.rec_max_v2;
+        const double v2c = clamped ? o.rec_max_v2 : v2r;
+        const double Wc = 1.0 / sqrt(1.0 - v2c);
+        const double f_W = c.D + 2.0 * gamma * r.p * Wc / (gamma - 1.0);
+        const double dW_dv2 = clamped ? 0.0 : 0.5 * Wc * Wc * Wc;
+        const double via_e = f_W * dW_dv2 * (-2.0 * v2r / e);
+        const double f_D = Wc + via_e - 1.0;
+        const double f_tau = via_e - 1.0;
+        const double f_p = gamma * Wc * Wc / (gamma - 1.0) + via_e - 1.0;
+        const double f_S2 = f_W * dW_dv2 / (e * e);
+        const double f_g = -r.p * Wc * Wc / ((gamma - 1.0) * (gamma - 1.0));
+        const double k = -p_b / f_p;
+        D_b += k * f_D;
+        tau_b += k * f_tau;
+        S2_b += k * f_S2;
+        gb += k * f_g;
+    }
+    /* S^2 = gamma^mn S_m S_n */
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+    {
+        double t = 0.0;
+#pragma unroll
+        for (int n2 = 0; n2 < 3; ++n2)
+            t += (gu[m][n2] + gu[n2][m]) * c.S[n2];
+        S_b[m] += S2_b * t;
+    }
+    cb[0] = D_b;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        cb[1 + m] = S_b[m];
+    cb[4] = tau_b;
+}
+
+/* The chain's adjoint: lambda_W of a storage cell (lw) back to the state
+ * of the interior cell it was recovered from (ub, D, S1..S3, tau) and to
+ * Gamma (added into gb), in one reverse sweep at the roots of `v`. */
+__device__ __forceinline__ void w_chain_reverse(const WChain &v, double gamma,
+                                                bool physical,
+                                                const Options &o,
+                                                const Cons &u,
+                                                const double lw[6],
+                                                double ub[5], double &gb)
+{
+    if (!physical)
+    {
+        RootFwd r;
+        root_forward(u, o, gamma, v.gt.gu, v.p1, r);
+        root_reverse(u, o, gamma, v.gt.gu, r, lw, ub, gb);
+        return;
+    }
+    /* forward: pr = C(u; gs), floors on gt, cg = prim2con(gt), w = C(cg; gt) */
+    RootFwd r1;
+    root_forward(u, o, gamma, v.gs.gu, v.p1, r1);
+    Prim q = r1.w;
+    const bool rho_kept = !(q.rho < o.rho_floor);
+    const bool p_kept = !(q.pressure < o.pressure_floor);
+    if (!rho_kept)
+        q.rho = o.rho_floor;
+    if (!p_kept)
+        q.pressure = o.pressure_floor;
+    double v_in[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        v_in[i] = q.v[i];
+    const double v2pre = vsq(q.v, v.gt.g);
+    const bool scaled = v2pre >= o.max_v2;
+    double scale = 1.0;
+    if (scaled)
+    {
+        scale = sqrt(o.max_v2 / v2pre);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            q.v[i] *= scale;
+    }
+    q.eps = q.pressure / ((gamma - 1.0) * q.rho);
+    Cons cg;
+    prim2con(q, gamma, v.gt.g, cg);
+    RootFwd r2;
+    root_forward(cg, o, gamma, v.gt.gu, v.p2, r2);
+    /* reverse */
+    double cgb[5];
+    root_reverse(cg, o, gamma, v.gt.gu, r2, lw, cgb, gb);
+    /* prim2con on gt */
+    double vL[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        vL[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            vL[i] += v.gt.g[i][j] * q.v[j];
+    }
+    const double W = 1.0 / sqrt(1.0 - vsq(q.v, v.gt.g));
+    const double h = 1.0 + q.eps + q.pressure / q.rho;
+    const double rhW2 = q.rho * h * W * W;
+    double rho_b = 0.0, P_b = 0.0, eps_b = 0.0, v_b[3] = {0.0, 0.0, 0.0};
+    double rhW2_b = cgb[4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        rhW2_b += cgb[1 + i] * vL[i];
+    P_b -= cgb[4];
+    const double D_b = cgb[0] - cgb[4];
+    rho_b += D_b * W;
+    double W_b = D_b * q.rho;
+    rho_b += rhW2_b * h * W * W;
+    const double h_b = rhW2_b * q.rho * W * W;
+    W_b += rhW2_b * 2.0 * q.rho * h * W;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            v_b[j] += cgb[1 + i] * rhW2 * v.gt.g[i][j];
+    eps_b += h_b;
+    P_b += h_b / q.rho;
+    rho_b -= h_b * q.pressure / (q.rho * q.rho);
+    vsq_bar(q.v, v.gt.g, W_b * 0.5 * W * W * W, v_b);
+    /* floors on gt */
+    P_b += eps_b / ((gamma - 1.0) * q.rho);
+    rho_b -= eps_b * q.eps / q.rho;
+    gb -= eps_b * q.eps / (gamma - 1.0);
+    double prb[6];
+    prb[0] = rho_kept ? rho_b : 0.0;
+    prb[1] = 0.0; /* eps is recomputed by the floors */
+    prb[2] = p_kept ? P_b : 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        prb[3 + i] = v_b[i];
+    if (scaled)
+    {
+        double scale_b = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            scale_b += v_b[i] * v_in[i];
+            prb[3 + i] = v_b[i] * scale;
+        }
+        double vin_b[3] = {prb[3], prb[4], prb[5]};
+        vsq_bar(v_in, v.gt.g, -0.5 * scale_b * scale / v2pre, vin_b);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            prb[3 + i] = vin_b[i];
+    }
+    /* the source cell's recovery on gs */
+    root_reverse(u, o, gamma, v.gs.gu, r1, prb, ub, gb);
+}
+
+/* ------------------------------------------------------------------ */
 /* k_adj_prims: lambda_W of every storage cell back to the interior     */
 /* state it was recovered from (interior: itself; outflow ghost: its    */
 /* clamped source, atomics), plus the parameter adjoints.               */
@@ -1268,8 +1500,18 @@ __global__ void __launch_bounds__(kBlock)
             const long long is = iidx(b, src[0], src[1], src[2]);
             WChain v;
             w_chain_values(b, pa, W, i, j, k, src, physical, v);
-            /* SPIN: the spin's pass alone; otherwise 5 U inputs and Gamma */
-            const int s0 = SPIN ? 6 : 0, s1 = SPIN ? 7 : 6;
+            if constexpr (!SPIN)
+            {
+                /* 5 U inputs and Gamma in one reverse sweep */
+                double gu5[5];
+                w_chain_reverse(v, pa.gamma, physical, o, uv, lw, gu5, g0);
+#pragma unroll
+                for (int m = 0; m < 5; ++m)
+                    if (gu5[m] != 0.0)
+                        atomicAdd(&lam_U[(long long)m * b.inpts + is], gu5[m]);
+            }
+            /* the spin's tangent pass */
+            const int s0 = 6, s1 = SPIN ? 7 : 6;
             for (int s = s0; s < s1; ++s)
             {
                 const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
