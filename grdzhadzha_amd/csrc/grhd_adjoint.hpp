@@ -395,15 +395,38 @@ __device__ __forceinline__ void vsq_bar(const double v[3], const double g[3][3],
     }
 }
 
+/* The spin tangents of the face-geometry components a face flux of
+ * direction d reads: gamma_ij, gamma^dd, alpha and beta^d. */
+struct FaceTan
+{
+    double g[9], gudd, lapse, shiftd;
+};
+
+/* w d(v^T gamma v)/d gamma_ij contracted with the tangents of gamma_ij */
+__device__ __forceinline__ double vsq_metric_dot(const double v[3], double w,
+                                                 const FaceTan &t)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            s += v[i] * v[j] * t.g[3 * i + j];
+    return w * s;
+}
+
 /* The adjoints of one side's conserved state, flux and wave speeds
  * (cb, fb, lob, hib) taken back to the face state before the floors
- * (pb: rho, eps, P, v1..v3) and to Gamma (added into gb). */
+ * (pb: rho, eps, P, v1..v3), to Gamma (added into gb), and through the
+ * face geometry to the spin (added into ab), given the geometry's spin
+ * tangents ft. */
 __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
                                              const Geo &g, int d,
                                              const double cb[5],
                                              const double fb[5], double lob,
-                                             double hib, double pb[6],
-                                             double &gb)
+                                             double hib, const FaceTan &ft,
+                                             double pb[6], double &gb,
+                                             double &ab)
 {
     const Prim &q = s.q;
     const double vel = q.v[d];
@@ -414,6 +437,8 @@ __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
     const double A_b = (lo_b + hi_b) / s.den;
     const double root_b = (hi_b - lo_b) / s.den;
     const double den_b = -(lo_b * s.lo + hi_b * s.hi) / s.den;
+    ab += lob * (s.lo * ft.lapse - ft.shiftd) +
+          hib * (s.hi * ft.lapse - ft.shiftd);
     double vel_b = A_b * (1.0 - s.cs2);
     double cs2_b = -A_b * vel;
     const double cs_b = root_b * s.sq;
@@ -422,6 +447,7 @@ __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
         (0.0 < s.rad && s.sq > 0.0) ? sq_b / (2.0 * s.sq) : 0.0;
     const double B = g.gu[d][d] * (1.0 - s.v2 * s.cs2) - vel * vel * (1.0 - s.cs2);
     const double B_b = rad_b * (1.0 - s.v2);
+    ab += B_b * (1.0 - s.v2 * s.cs2) * ft.gudd;
     v2_b -= rad_b * B;
     v2_b -= B_b * g.gu[d][d] * s.cs2;
     cs2_b -= B_b * g.gu[d][d] * s.v2;
@@ -447,6 +473,8 @@ __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
         c_b[k] = cb[k] + fb[k] * s.tv;
     P_b += fb[4] * g.lapse * vel + fb[1 + d] * g.lapse;
     v_b[d] += fb[4] * g.lapse * q.pressure + tv_b * g.lapse;
+    ab += tv_b * (vel * ft.lapse - ft.shiftd) +
+          (fb[4] * vel + fb[1 + d]) * q.pressure * ft.lapse;
     /* prim2con: tau = rhW2 - P - D, S_i = rhW2 vL_i, D = rho W */
     double rhW2_b = c_b[4];
 #pragma unroll
@@ -463,7 +491,10 @@ __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
+        {
             v_b[j] += c_b[1 + i] * s.rhW2 * g.g[i][j];
+            ab += c_b[1 + i] * s.rhW2 * q.v[j] * ft.g[3 * i + j];
+        }
     /* h = 1 + eps + P / rho (prim2con's and char_speeds' alike) */
     eps_b += h_b;
     P_b += h_b / q.rho;
@@ -471,6 +502,7 @@ __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
     /* W = (1 - v^2)^(-1/2), v^2 shared with char_speeds */
     v2_b += W_b * 0.5 * s.W * s.W * s.W;
     vsq_bar(q.v, g.g, v2_b, v_b);
+    ab += vsq_metric_dot(q.v, v2_b, ft);
     /* floors: eps = P / ((Gamma - 1) rho), then the v^2 clamp */
     P_b += eps_b / ((gamma - 1.0) * q.rho);
     rho_b -= eps_b * q.eps / q.rho;
@@ -488,7 +520,9 @@ __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
             scale_b += v_b[i] * s.v_in[i];
             vin_b[i] = v_b[i] * s.scale;
         }
-        vsq_bar(s.v_in, g.g, -0.5 * scale_b * s.scale / s.v2pre, vin_b);
+        const double v2pre_b = -0.5 * scale_b * s.scale / s.v2pre;
+        vsq_bar(s.v_in, g.g, v2pre_b, vin_b);
+        ab += vsq_metric_dot(s.v_in, v2pre_b, ft);
     }
     pb[0] = s.rho_kept ? rho_b : 0.0;
     pb[1] = 0.0; /* eps is recomputed by the floors */
@@ -500,14 +534,16 @@ __device__ __forceinline__ void side_reverse(const SideFwd &s, double gamma,
 }
 
 /* lambda . F(lp, rp) differentiated in the two face states (lpb, rpb, in
- * the order rho, eps, P, v1..v3) and in Gamma (added into gb). */
+ * the order rho, eps, P, v1..v3), in Gamma (added into gb) and, through
+ * the face geometry with spin tangents ft, in the spin (added into ab). */
 __device__ __forceinline__ void face_flux_reverse(const Prim &lp,
                                                   const Prim &rp,
                                                   const Options &o,
                                                   double gamma, const Geo &g,
-                                                  int d, const double lam[5],
+                                                  const FaceTan &ft, int d,
+                                                  const double lam[5],
                                                   double lpb[6], double rpb[6],
-                                                  double &gb)
+                                                  double &gb, double &ab)
 {
     SideFwd l, r;
     side_forward(lp, o, gamma, g, d, l);
@@ -559,8 +595,8 @@ __device__ __forceinline__ void face_flux_reverse(const Prim &lp,
         else
             lhib = su_b;
     }
-    side_reverse(l, gamma, g, d, lcb, lfb, llob, lhib, lpb, gb);
-    side_reverse(r, gamma, g, d, rcb, rfb, rlob, rhib, rpb, gb);
+    side_reverse(l, gamma, g, d, lcb, lfb, llob, lhib, ft, lpb, gb, ab);
+    side_reverse(r, gamma, g, d, rcb, rfb, rlob, rhib, ft, rpb, gb, ab);
 }
 
 /* Component m of a primitive state, in W's order (rho, eps, P, v). */
@@ -958,11 +994,12 @@ __device__ __forceinline__ void load_geo(const MetricCache &mc,
 /* ------------------------------------------------------------------ */
 /* k_adj_faces<D>: lambda_F of each face of direction D from the rhs    */
 /* adjoint of its two cells, then J_F^T lambda_F into lambda_W of the   */
-/* four stencil cells (atomics) and into the parameter adjoints.        */
+/* four stencil cells (atomics) and into the parameter adjoints, the    */
+/* spin's through the face geometry and its cached tangents.            */
 /* One thread per face of the interior face box.                       */
 /* ------------------------------------------------------------------ */
-template <int D, bool SPIN>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+template <int D>
+__global__ void __launch_bounds__(kBlock)
     k_adj_faces(BoxDev b, Options o, AdjParams pa, MetricCache mc,
                 const double *W, const double *lam_k, double *lam_W,
                 double *lam_par)
@@ -1009,75 +1046,68 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
                 so[q] = goff(b, cc[0], cc[1], cc[2]);
                 wv[q] = load_prim(W, n, so[q]);
             }
-            if constexpr (SPIN)
+            /* the face metric in double, and its spin tangent: the mean of
+             * the two cells' cached tangents (average_geo) */
+            double xl[3], xr[3];
+            coords(cl[0], cl[1], cl[2], b.dx, b.center, xl);
+            coords(c[0], c[1], c[2], b.dx, b.center, xr);
+            Geo facev;
+            face_geo(xl, xr, pa.M, pa.a, facev);
+            FaceTan ft;
             {
-                /* the spin's tangent: the face metric with duals, the mean
-                 * of the cached cell geometries and their tangents */
-                Dual M, a, gamma;
-                seed_params(pa, SEED_SPIN, M, a, gamma);
-                PrimT<Dual> wd[4];
+                /* geo component m's tangent at [kGeoN + m] (gamma_ij at
+                 * 3i + j, gamma^ij at 9 + 3i + j, alpha 18, beta^i 19 + i) */
+                auto tan = [&](int m) {
+                    return 0.5 * (mc.geo[(long long)(kGeoN + m) * n + so[1]] +
+                                  mc.geo[(long long)(kGeoN + m) * n + so[2]]);
+                };
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    wd[q] = dual_prim(wv[q], -1);
-                GeoT<Dual> gl, gr, face;
-                load_geo(mc, n, so[1], gl);
-                load_geo(mc, n, so[2], gr);
-                average_geo(gl, gr, face);
-                ConsT<Dual> f;
-                face_flux_geo(wd, face, gamma, o, D, f);
-                g1 = dot_tangent(f, lam);
+                for (int m = 0; m < 9; ++m)
+                    ft.g[m] = tan(m);
+                ft.gudd = tan(9 + 4 * D);
+                ft.lapse = tan(18);
+                ft.shiftd = tan(19 + D);
             }
-            else
+            /* the face states (values), as k_faces forms them */
+            Prim lpv = wv[1], rpv = wv[2];
+            if (o.use_reconstruction)
             {
-                /* the face metric in double: no pass here seeds it */
-                double xl[3], xr[3];
-                coords(cl[0], cl[1], cl[2], b.dx, b.center, xl);
-                coords(c[0], c[1], c[2], b.dx, b.center, xr);
-                Geo facev;
-                face_geo(xl, xr, pa.M, pa.a, facev);
-                /* the face states (values), as k_faces forms them */
-                Prim lpv = wv[1], rpv = wv[2];
+                Prim unused;
+                slope_step(wv[0], wv[1], wv[2], o.theta, unused, lpv);
+                slope_step(wv[1], wv[2], wv[3], o.theta, rpv, unused);
+            }
+            /* lambda . dF/d(face states), d/dGamma and d/da in one reverse
+             * sweep */
+            double lam_lp[6], lam_rp[6];
+            face_flux_reverse(lpv, rpv, o, pa.gamma, facev, ft, D, lam, lam_lp,
+                              lam_rp, g0, g1);
+            /* back through the reconstruction, component by component:
+             * lp = W_L + s_L/2 from (W_{L-1}, W_L, W_R), rp = W_R - s_R/2
+             * from (W_L, W_R, W_{R+1}) */
+#pragma unroll
+            for (int m = 0; m < 6; ++m)
+            {
+                double gq[4] = {0.0, lam_lp[m], lam_rp[m], 0.0};
                 if (o.use_reconstruction)
                 {
-                    Prim unused;
-                    slope_step(wv[0], wv[1], wv[2], o.theta, unused, lpv);
-                    slope_step(wv[1], wv[2], wv[3], o.theta, rpv, unused);
-                }
-                /* lambda . dF/d(face states) and d/dGamma in one reverse
-                 * sweep */
-                double lam_lp[6], lam_rp[6];
-                face_flux_reverse(lpv, rpv, o, pa.gamma, facev, D, lam, lam_lp,
-                                  lam_rp, g0);
-                /* back through the reconstruction, component by component:
-                 * lp = W_L + s_L/2 from (W_{L-1}, W_L, W_R), rp = W_R - s_R/2
-                 * from (W_L, W_R, W_{R+1}) */
-#pragma unroll
-                for (int m = 0; m < 6; ++m)
-                {
-                    double gq[4] = {0.0, lam_lp[m], lam_rp[m], 0.0};
-                    if (o.use_reconstruction)
-                    {
-                        /* the stencil's W read again rather than kept in
-                         * registers through the reverse sweep */
-                        double wm[4];
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            wm[q] = W[(long long)m * n + so[q]];
-                        double jl[3], jr[3];
-                        half_step_jacobian(wm[0], wm[1], wm[2], o.theta, 0.5,
-                                           jl);
-                        half_step_jacobian(wm[1], wm[2], wm[3], o.theta, -0.5,
-                                           jr);
-                        gq[0] = lam_lp[m] * jl[0];
-                        gq[1] = lam_lp[m] * jl[1] + lam_rp[m] * jr[0];
-                        gq[2] = lam_lp[m] * jl[2] + lam_rp[m] * jr[1];
-                        gq[3] = lam_rp[m] * jr[2];
-                    }
+                    /* the stencil's W read again rather than kept in
+                     * registers through the reverse sweep */
+                    double wm[4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        if (gq[q] != 0.0)
-                            atomicAdd(&lam_W[(long long)m * n + so[q]], gq[q]);
+                        wm[q] = W[(long long)m * n + so[q]];
+                    double jl[3], jr[3];
+                    half_step_jacobian(wm[0], wm[1], wm[2], o.theta, 0.5, jl);
+                    half_step_jacobian(wm[1], wm[2], wm[3], o.theta, -0.5, jr);
+                    gq[0] = lam_lp[m] * jl[0];
+                    gq[1] = lam_lp[m] * jl[1] + lam_rp[m] * jr[0];
+                    gq[2] = lam_lp[m] * jl[2] + lam_rp[m] * jr[1];
+                    gq[3] = lam_rp[m] * jr[2];
                 }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (gq[q] != 0.0)
+                        atomicAdd(&lam_W[(long long)m * n + so[q]], gq[q]);
             }
         }
     }

@@ -1249,26 +1249,20 @@ void adj_rhs_transpose(AdjRun &r, const double *X, double *lam_k,
     const long long nf[3] = {(long long)(b.in[0] + 1) * b.in[1] * b.in[2],
                              (long long)b.in[0] * (b.in[1] + 1) * b.in[2],
                              (long long)b.in[0] * b.in[1] * (b.in[2] + 1)};
-    /* each local Jacobian in two launches: the passes on the double metric,
-     * and the spin's pass, whose dual metric needs far more registers */
-    hipLaunchKernelGGL((k_adj_faces<0, false>), dim3(grid_of(nf[0])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
-                       lam_W, lam_par);
-    hipLaunchKernelGGL((k_adj_faces<0, true>), dim3(grid_of(nf[0])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
-                       lam_W, lam_par);
-    hipLaunchKernelGGL((k_adj_faces<1, false>), dim3(grid_of(nf[1])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
-                       lam_W, lam_par);
-    hipLaunchKernelGGL((k_adj_faces<1, true>), dim3(grid_of(nf[1])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
-                       lam_W, lam_par);
-    hipLaunchKernelGGL((k_adj_faces<2, false>), dim3(grid_of(nf[2])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
-                       lam_W, lam_par);
-    hipLaunchKernelGGL((k_adj_faces<2, true>), dim3(grid_of(nf[2])),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k,
-                       lam_W, lam_par);
+    /* the face Jacobians in one reverse sweep each, the spin's through the
+     * cached metric tangents */
+    hipLaunchKernelGGL((k_adj_faces<0>), dim3(grid_of(nf[0])), dim3(kBlock),
+                       0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k, lam_W,
+                       lam_par);
+    hipLaunchKernelGGL((k_adj_faces<1>), dim3(grid_of(nf[1])), dim3(kBlock),
+                       0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k, lam_W,
+                       lam_par);
+    hipLaunchKernelGGL((k_adj_faces<2>), dim3(grid_of(nf[2])), dim3(kBlock),
+                       0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k, lam_W,
+                       lam_par);
+    /* sources and recoveries: the passes on the double metric in a reverse
+     * sweep, and the spin's forward-mode pass, whose dual metric needs far
+     * more registers, in a launch of its own */
     hipLaunchKernelGGL(k_adj_sources<false>, dim3(grid_of(b.inpts)),
                        dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work,
                        r.c->W, lam_k, lam_W, lam_X, lam_par);
