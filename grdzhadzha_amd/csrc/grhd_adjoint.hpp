@@ -925,21 +925,18 @@ __device__ __forceinline__ bool interior(const BoxDev &b, int i, int j, int k)
 }
 
 /* ------------------------------------------------------------------ */
-/* The Kerr-Schild geometry of every storage cell with its tangent      */
-/* along the spin, evaluated once per adjoint run (the background is    */
-/* static) and read by the spin-tangent kernels of every reverse stage, */
-/* which would otherwise evaluate it in dual arithmetic per face and    */
-/* per cell.  Component-major: the value of component m at [m][cell],   */
-/* its spin tangent at [kGeoN + m][cell].  The kernels working in plain */
-/* double recompute the metric: there the 22 loads per cell cost more  */
-/* than the arithmetic they replace.  The values come from the same     */
-/* double functions, so results are unchanged.                          */
+/* The spin tangent of the Kerr-Schild geometry of every storage cell,  */
+/* evaluated once per adjoint run (the background is static) and read   */
+/* by the reverse sweeps of every stage, which contract the adjoint of  */
+/* the metric they read with it to give the spin's adjoint.  Component- */
+/* major: the tangent of component m (gamma_ij at 3i + j, gamma^ij at   */
+/* 9 + 3i + j, alpha at 18, beta^i at 19 + i) at [m][cell].             */
 /* ------------------------------------------------------------------ */
 constexpr int kGeoN = 22; /* g (9), gu (9), lapse, shift (3) */
 
 struct MetricCache
 {
-    double *geo; /* 2 kGeoN x storage cells */
+    double *geo; /* kGeoN x storage cells */
 };
 
 template <class T> __device__ __forceinline__ T &geo_at(GeoT<T> &g, int m)
@@ -965,30 +962,11 @@ __global__ void __launch_bounds__(kBlock)
     const Dual xd[3] = {Dual(x[0]), Dual(x[1]), Dual(x[2])};
     Dual M, a, gamma;
     seed_params(pa, SEED_SPIN, M, a, gamma);
-    {
-        Geo gv;
-        ks_geometry(pa.M, pa.a, x, gv);
-        GeoT<Dual> gd;
-        ks_geometry(M, a, xd, gd);
-#pragma unroll
-        for (int m = 0; m < kGeoN; ++m)
-        {
-            mc.geo[(long long)m * n + it] = geo_at(gv, m);
-            mc.geo[(long long)(kGeoN + m) * n + it] = geo_at(gd, m).d;
-        }
-    }
-}
-
-/* The cached geometry of storage cell `so` with its spin tangents as
- * duals. */
-__device__ __forceinline__ void load_geo(const MetricCache &mc,
-                                         long long n, long long so,
-                                         GeoT<Dual> &g)
-{
+    GeoT<Dual> gd;
+    ks_geometry(M, a, xd, gd);
 #pragma unroll
     for (int m = 0; m < kGeoN; ++m)
-        geo_at(g, m) = Dual(mc.geo[(long long)m * n + so],
-                            mc.geo[(long long)(kGeoN + m) * n + so]);
+        mc.geo[(long long)m * n + it] = geo_at(gd, m).d;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1055,11 +1033,11 @@ __global__ void __launch_bounds__(kBlock)
             face_geo(xl, xr, pa.M, pa.a, facev);
             FaceTan ft;
             {
-                /* geo component m's tangent at [kGeoN + m] (gamma_ij at
+                /* geo component m's tangent at [m] (gamma_ij at
                  * 3i + j, gamma^ij at 9 + 3i + j, alpha 18, beta^i 19 + i) */
                 auto tan = [&](int m) {
-                    return 0.5 * (mc.geo[(long long)(kGeoN + m) * n + so[1]] +
-                                  mc.geo[(long long)(kGeoN + m) * n + so[2]]);
+                    return 0.5 * (mc.geo[(long long)m * n + so[1]] +
+                                  mc.geo[(long long)m * n + so[2]]);
                 };
 #pragma unroll
                 for (int m = 0; m < 9; ++m)
@@ -1223,32 +1201,8 @@ __device__ __forceinline__ void w_chain_values(const BoxDev &b,
     v.p2 = p_own;
 }
 
-/* One tangent pass of the chain at the roots of `v` on the geometries
- * gt (the cell) and gs (its source): the double ones of `v`, or dual ones
- * for the spin's tangent. */
-template <class G>
-__device__ __forceinline__ void w_of_state(const GeoT<G> &gt,
-                                           const GeoT<G> &gs, Dual gamma,
-                                           bool physical, const WChain &v,
-                                           const Options &o,
-                                           const ConsT<Dual> &us,
-                                           PrimT<Dual> &w)
-{
-    if (!physical)
-    {
-        con2prim_at_root(us, o, gamma, gt.gu, v.p1, w);
-        return;
-    }
-    PrimT<Dual> pr;
-    con2prim_at_root(us, o, gamma, gs.gu, v.p1, pr);
-    floors_g(pr, o, gamma, gt.g);
-    ConsT<Dual> cg;
-    prim2con(pr, gamma, gt.g, cg);
-    con2prim_at_root(cg, o, gamma, gt.gu, v.p2, w);
-}
-
 /* ------------------------------------------------------------------ */
-/* Reverse mode of the recovery chain (w_of_state in double).           */
+/* Reverse mode of the recovery chain (k_prims' W as a function of U).  */
 /* ------------------------------------------------------------------ */
 
 /* con2prim_at_root in double: the values its reverse sweep needs. */
@@ -1290,16 +1244,18 @@ __device__ __forceinline__ void root_forward(const Cons &c, const Options &o,
 }
 
 /* The adjoint of the recovered primitives (wb: rho, eps, P, v1..v3) back
- * to the conserved state (cb: D, S1..S3, tau) and Gamma (added into gb).
+ * to the conserved state (cb: D, S1..S3, tau), Gamma (added into gb) and,
+ * through gamma^ij with spin tangents tgu, the spin (added into ab).
  * The root pressure moves by the implicit-function theorem,
  * dp = -(df with p held) / (df/dp), on presid with the v^2 clamp as the
  * solve left it; a root at the recovery floor is a constant. */
 __device__ __forceinline__ void root_reverse(const Cons &c, const Options &o,
                                              double gamma,
                                              const double gu[3][3],
+                                             const double tgu[9],
                                              const RootFwd &r,
                                              const double wb[6], double cb[5],
-                                             double &gb)
+                                             double &gb, double &ab)
 {
     const Prim &w = r.w;
     double p_b = wb[2], rho_b = wb[0], D_b = 0.0, tau_b = 0.0, epp_b = 0.0;
@@ -1314,7 +1270,10 @@ __device__ __forceinline__ void root_reverse(const Cons &c, const Options &o,
     {
 #pragma unroll
         for (int j = 0; j < 3; ++j)
+        {
             S_b[j] += wb[3 + i] * gu[i][j] / r.epp;
+            ab += wb[3 + i] * (c.S[j] / r.epp) * tgu[3 * i + j];
+        }
         epp_b -= wb[3 + i] * w.v[i] / r.epp;
     }
     /* rho = D / W, W = (1 - v^2)^(-1/2), v^2 = S^2 / (E + p)^2 */
@@ -1356,7 +1315,10 @@ __device__ __forceinline__ void root_reverse(const Cons &c, const Options &o,
         double t = 0.0;
 #pragma unroll
         for (int n2 = 0; n2 < 3; ++n2)
+        {
             t += (gu[m][n2] + gu[n2][m]) * c.S[n2];
+            ab += S2_b * c.S[m] * c.S[n2] * tgu[3 * m + n2];
+        }
         S_b[m] += S2_b * t;
     }
     cb[0] = D_b;
@@ -1366,21 +1328,31 @@ __device__ __forceinline__ void root_reverse(const Cons &c, const Options &o,
     cb[4] = tau_b;
 }
 
+/* The spin tangents of the metric components the recovery chain reads:
+ * gamma_ij and gamma^ij of the cell, gamma^ij of its source cell. */
+struct ChainTan
+{
+    double gt_g[9], gt_gu[9], gs_gu[9];
+};
+
 /* The chain's adjoint: lambda_W of a storage cell (lw) back to the state
- * of the interior cell it was recovered from (ub, D, S1..S3, tau) and to
- * Gamma (added into gb), in one reverse sweep at the roots of `v`. */
+ * of the interior cell it was recovered from (ub, D, S1..S3, tau), to
+ * Gamma (added into gb) and, through the metric with tangents t, to the
+ * spin (added into ab), in one reverse sweep at the roots of `v`. */
 __device__ __forceinline__ void w_chain_reverse(const WChain &v, double gamma,
                                                 bool physical,
                                                 const Options &o,
                                                 const Cons &u,
+                                                const ChainTan &t,
                                                 const double lw[6],
-                                                double ub[5], double &gb)
+                                                double ub[5], double &gb,
+                                                double &ab)
 {
     if (!physical)
     {
         RootFwd r;
         root_forward(u, o, gamma, v.gt.gu, v.p1, r);
-        root_reverse(u, o, gamma, v.gt.gu, r, lw, ub, gb);
+        root_reverse(u, o, gamma, v.gt.gu, t.gt_gu, r, lw, ub, gb, ab);
         return;
     }
     /* forward: pr = C(u; gs), floors on gt, cg = prim2con(gt), w = C(cg; gt) */
@@ -1414,7 +1386,7 @@ __device__ __forceinline__ void w_chain_reverse(const WChain &v, double gamma,
     root_forward(cg, o, gamma, v.gt.gu, v.p2, r2);
     /* reverse */
     double cgb[5];
-    root_reverse(cg, o, gamma, v.gt.gu, r2, lw, cgb, gb);
+    root_reverse(cg, o, gamma, v.gt.gu, t.gt_gu, r2, lw, cgb, gb, ab);
     /* prim2con on gt */
     double vL[3];
 #pragma unroll
@@ -1444,11 +1416,20 @@ __device__ __forceinline__ void w_chain_reverse(const WChain &v, double gamma,
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
+        {
             v_b[j] += cgb[1 + i] * rhW2 * v.gt.g[i][j];
+            ab += cgb[1 + i] * rhW2 * q.v[j] * t.gt_g[3 * i + j];
+        }
     eps_b += h_b;
     P_b += h_b / q.rho;
     rho_b -= h_b * q.pressure / (q.rho * q.rho);
-    vsq_bar(q.v, v.gt.g, W_b * 0.5 * W * W * W, v_b);
+    const double v2_b = W_b * 0.5 * W * W * W;
+    vsq_bar(q.v, v.gt.g, v2_b, v_b);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            ab += v2_b * q.v[i] * q.v[j] * t.gt_g[3 * i + j];
     /* floors on gt */
     P_b += eps_b / ((gamma - 1.0) * q.rho);
     rho_b -= eps_b * q.eps / q.rho;
@@ -1470,21 +1451,27 @@ __device__ __forceinline__ void w_chain_reverse(const WChain &v, double gamma,
             prb[3 + i] = v_b[i] * scale;
         }
         double vin_b[3] = {prb[3], prb[4], prb[5]};
-        vsq_bar(v_in, v.gt.g, -0.5 * scale_b * scale / v2pre, vin_b);
+        const double v2pre_b = -0.5 * scale_b * scale / v2pre;
+        vsq_bar(v_in, v.gt.g, v2pre_b, vin_b);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                ab += v2pre_b * v_in[i] * v_in[j] * t.gt_g[3 * i + j];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             prb[3 + i] = vin_b[i];
     }
     /* the source cell's recovery on gs */
-    root_reverse(u, o, gamma, v.gs.gu, r1, prb, ub, gb);
+    root_reverse(u, o, gamma, v.gs.gu, t.gs_gu, r1, prb, ub, gb, ab);
 }
 
 /* ------------------------------------------------------------------ */
 /* k_adj_prims: lambda_W of every storage cell back to the interior     */
 /* state it was recovered from (interior: itself; outflow ghost: its    */
-/* clamped source, atomics), plus the parameter adjoints.               */
+/* clamped source, atomics), plus the parameter adjoints, the spin's    */
+/* through the cached metric tangents.                                  */
 /* ------------------------------------------------------------------ */
-template <bool SPIN>
 __global__ void __launch_bounds__(kBlock)
     k_adj_prims(BoxDev b, Options o, AdjParams pa, MetricCache mc,
                 const double *U, const double *W, const double *lam_W,
@@ -1530,49 +1517,24 @@ __global__ void __launch_bounds__(kBlock)
             const long long is = iidx(b, src[0], src[1], src[2]);
             WChain v;
             w_chain_values(b, pa, W, i, j, k, src, physical, v);
-            if constexpr (!SPIN)
-            {
-                /* 5 U inputs and Gamma in one reverse sweep */
-                double gu5[5];
-                w_chain_reverse(v, pa.gamma, physical, o, uv, lw, gu5, g0);
+            /* spin tangents of gamma_ij (components 0..8 of the cache) and
+             * gamma^ij (9..17) of the cell, and of its source's gamma^ij */
+            ChainTan t;
+            const long long st = goff(b, i, j, k);
 #pragma unroll
-                for (int m = 0; m < 5; ++m)
-                    if (gu5[m] != 0.0)
-                        atomicAdd(&lam_U[(long long)m * b.inpts + is], gu5[m]);
-            }
-            /* the spin's tangent pass */
-            const int s0 = 6, s1 = SPIN ? 7 : 6;
-            for (int s = s0; s < s1; ++s)
+            for (int m = 0; m < 9; ++m)
             {
-                const int seed = s < 5 ? 0 : (s == 5 ? SEED_GAMMA : SEED_SPIN);
-                Dual M, a, gamma;
-                seed_params(pa, seed, M, a, gamma);
-                const ConsT<Dual> ud = dual_cons(uv, s < 5 ? s : -1);
-                PrimT<Dual> w;
-                if constexpr (SPIN)
-                {
-                    GeoT<Dual> gt, gs;
-                    load_geo(mc, n, goff(b, i, j, k), gt);
-                    if (physical)
-                        load_geo(mc, n, goff(b, src[0], src[1], src[2]), gs);
-                    w_of_state(gt, gs, gamma, physical, v, o, ud, w);
-                }
-                else
-                    w_of_state(v.gt, v.gs, gamma, physical, v, o, ud, w);
-                double g = 0.0;
-#pragma unroll
-                for (int m = 0; m < 6; ++m)
-                    g += lw[m] * prim_tangent(w, m);
-                if (s < 5)
-                {
-                    if (g != 0.0)
-                        atomicAdd(&lam_U[(long long)s * b.inpts + is], g);
-                }
-                else if (s == 5)
-                    g0 += g;
-                else
-                    g1 += g;
+                t.gt_gu[m] = mc.geo[(long long)(9 + m) * n + st];
+                t.gt_g[m] = physical ? mc.geo[(long long)m * n + st] : 0.0;
+                t.gs_gu[m] = physical ? mc.geo[(long long)(9 + m) * n + ss] : 0.0;
             }
+            /* 5 U inputs, Gamma and the spin in one reverse sweep */
+            double gu5[5];
+            w_chain_reverse(v, pa.gamma, physical, o, uv, t, lw, gu5, g0, g1);
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                if (gu5[m] != 0.0)
+                    atomicAdd(&lam_U[(long long)m * b.inpts + is], gu5[m]);
         }
     }
     block_add_params(lam_par, g0, g1);

@@ -1179,7 +1179,7 @@ struct AdjRun
     hipStream_t s;
     double *work; /* 11-component FAB (comp_base 0) the forward kernels use */
     Pools p;
-    MetricCache mc; /* the cells' geometry and spin tangents (reverse) */
+    MetricCache mc; /* the cells' metric spin tangents (reverse) */
 };
 
 void fill0(double *x, long long n, hipStream_t s)
@@ -1260,21 +1260,19 @@ void adj_rhs_transpose(AdjRun &r, const double *X, double *lam_k,
     hipLaunchKernelGGL((k_adj_faces<2>), dim3(grid_of(nf[2])), dim3(kBlock),
                        0, r.s, b, r.o, r.pa, r.mc, r.c->W, lam_k, lam_W,
                        lam_par);
-    /* sources and recoveries: the passes on the double metric in a reverse
-     * sweep, and the spin's forward-mode pass, whose dual metric needs far
-     * more registers, in a launch of its own */
+    /* sources: the double metric's terms in a reverse sweep, and the
+     * spin's forward-mode pass, whose dual metric needs far more registers,
+     * in a launch of its own; recoveries: one reverse sweep with the spin
+     * through the cached metric tangents */
     hipLaunchKernelGGL(k_adj_sources<false>, dim3(grid_of(b.inpts)),
                        dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work,
                        r.c->W, lam_k, lam_W, lam_X, lam_par);
     hipLaunchKernelGGL(k_adj_sources<true>, dim3(grid_of(b.inpts)),
                        dim3(kBlock), 0, r.s, b, r.o, r.pa, r.work,
                        r.c->W, lam_k, lam_W, lam_X, lam_par);
-    hipLaunchKernelGGL(k_adj_prims<false>, dim3(grid_of(b.gnpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
-                       r.c->W, lam_W, lam_X, lam_par);
-    hipLaunchKernelGGL(k_adj_prims<true>, dim3(grid_of(b.gnpts)),
-                       dim3(kBlock), 0, r.s, b, r.o, r.pa, r.mc, r.work,
-                       r.c->W, lam_W, lam_X, lam_par);
+    hipLaunchKernelGGL(k_adj_prims, dim3(grid_of(b.gnpts)), dim3(kBlock), 0,
+                       r.s, b, r.o, r.pa, r.mc, r.work, r.c->W, lam_W, lam_X,
+                       lam_par);
 }
 
 int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
@@ -1336,7 +1334,7 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     double *lam_X = mem.get(nadj * 5 * I + 1);
     double *lam_t = mem.get(nadj * 5 * I + 1);
     double *lam_W = mem.get(nadj * 6 * G + 1);
-    r.mc.geo = mem.get(nadj * 2 * kGeoN * G + 1);
+    r.mc.geo = mem.get(nadj * kGeoN * G + 1);
     double *lam_par = mem.get(2);
     double *terms = mem.get(I);
     const long long nb = (I + kBlock - 1) / kBlock;
@@ -1401,7 +1399,7 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     if (!grad)
         return GRHD_OK;
 
-    /* the cells' geometry and its spin tangent, once for the whole sweep */
+    /* the cells' metric spin tangents, once for the whole sweep */
     hipLaunchKernelGGL(k_adj_metric_cache, dim3(grid_of(G)), dim3(kBlock), 0,
                        s, b, r.pa, r.mc);
 
