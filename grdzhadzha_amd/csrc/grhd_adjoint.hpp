@@ -261,41 +261,21 @@ struct AdjParams
 enum
 {
     SEED_GAMMA = -1,
-    SEED_SPIN = -2,
-    SEED_NONE = -3
+    SEED_SPIN = -2
 };
-
-/* Face flux of direction D from the W of cells L-1, L, R, R+1 along D on a
- * given face geometry (k_faces_x / k_faces<D> after the metric).  The
- * metric is double in every pass but the spin's: it depends on a and x
- * only, so its tangent is zero there and it is evaluated once. */
-template <class T, class G>
-__device__ __forceinline__ void face_flux_geo(const PrimT<T> w[4],
-                                              const GeoT<G> &face, T gamma,
-                                              const Options &o, int D,
-                                              ConsT<T> &f)
-{
-    PrimT<T> lp = w[1], rp = w[2];
-    if (o.use_reconstruction)
-    {
-        PrimT<T> unused;
-        slope_step(w[0], w[1], w[2], T(o.theta), unused, lp);
-        slope_step(w[1], w[2], w[3], T(o.theta), rp, unused);
-    }
-    hlle_face_g(lp, rp, o, gamma, face, D, f);
-}
 
 /* ------------------------------------------------------------------ */
 /* Reverse mode of one face flux in its two face states.               */
-/* side_forward evaluates face_side in double, keeping the values the   */
-/* backward sweep needs and the branches it took; hlle_reverse and      */
-/* side_reverse then apply the transposed Jacobian of hlle_combine and  */
-/* of each side (floors, prim2con, flux_of, char_speeds) to lambda in   */
-/* one sweep.  Every branch (the floors, the v^2 clamp, the radicand    */
-/* clamp, the wave-speed selections) is taken as the forward evaluation */
-/* takes it, with the same tie rules (std_max keeps its first argument  */
-/* on a tie, std_min too).  One sweep replaces the 12 + 1 forward-mode  */
-/* passes over the face states and Gamma.                              */
+/* side_forward evaluates one state's part of hlle_face_g (floors,      */
+/* prim2con, flux_of, char_speeds) in double, keeping the values the    */
+/* backward sweep needs and the branches it took; face_flux_reverse     */
+/* applies the transposed Jacobian of the HLLE combination to lambda    */
+/* and side_reverse that of each state's part.  Every branch (the       */
+/* floors, the v^2 clamp, the radicand clamp, the wave-speed            */
+/* selections) is taken as the forward evaluation takes it, with the    */
+/* same tie rules (std_max keeps its first argument on a tie, std_min   */
+/* too).  One sweep gives what one forward-mode pass per face-state     */
+/* component and Gamma would.                                           */
 /* ------------------------------------------------------------------ */
 struct SideFwd
 {
@@ -551,7 +531,7 @@ __device__ __forceinline__ void face_flux_reverse(const Prim &lp,
     double lcb[5] = {0, 0, 0, 0, 0}, lfb[5] = {0, 0, 0, 0, 0};
     double rcb[5] = {0, 0, 0, 0, 0}, rfb[5] = {0, 0, 0, 0, 0};
     double llob = 0.0, lhib = 0.0, rlob = 0.0, rhib = 0.0;
-    /* hlle_combine */
+    /* the HLLE combination (hlle_face_g) */
     const double in_lo = (r.lower < l.lower) ? r.lower : l.lower;
     const double sl = (in_lo < 0.0) ? in_lo : 0.0;
     const double in_hi = (l.upper < r.upper) ? r.upper : l.upper;
@@ -597,12 +577,6 @@ __device__ __forceinline__ void face_flux_reverse(const Prim &lp,
     }
     side_reverse(l, gamma, g, d, lcb, lfb, llob, lhib, ft, lpb, gb, ab);
     side_reverse(r, gamma, g, d, rcb, rfb, rlob, rhib, ft, rpb, gb, ab);
-}
-
-/* Component m of a primitive state, in W's order (rho, eps, P, v). */
-__device__ __forceinline__ double prim_comp(const Prim &w, int m)
-{
-    return m == 0 ? w.rho : (m == 1 ? w.eps : (m == 2 ? w.pressure : w.v[m - 3]));
 }
 
 /* d(c + h * limited_slope(l, c, r, theta)) / d(l, c, r): the reconstruction
@@ -848,34 +822,11 @@ __device__ __forceinline__ ConsT<Dual> dual_cons(const Cons &c, int slot)
     return q;
 }
 
-/* conserved component m of ConsT (D, S1, S2, S3, tau) */
-template <class T> __device__ __forceinline__ T cons_comp(const ConsT<T> &c, int m)
-{
-    return m == 0 ? c.D : (m == 4 ? c.tau : c.S[m - 1]);
-}
-
 __device__ __forceinline__ double dot_tangent(const ConsT<Dual> &f,
                                               const double lam[5])
 {
     return lam[0] * f.D.d + lam[1] * f.S[0].d + lam[2] * f.S[1].d +
            lam[3] * f.S[2].d + lam[4] * f.tau.d;
-}
-
-/* prim component k (rho, eps, P, v1, v2, v3) of a Dual primitive's
- * tangent */
-__device__ __forceinline__ double prim_tangent(const PrimT<Dual> &w, int k)
-{
-    switch (k)
-    {
-    case 0:
-        return w.rho.d;
-    case 1:
-        return w.eps.d;
-    case 2:
-        return w.pressure.d;
-    default:
-        return w.v[k - 3].d;
-    }
 }
 
 /* block sum of two doubles, one atomicAdd each by thread 0 */
