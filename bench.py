@@ -140,6 +140,7 @@ def bench_c1(args):
     w = workloads.michel_box_32()
     cells = w.box.num_interior
     op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, w.options)
+    op.ctx.set_autotune(args.autotune)
     init = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
     steps = 200
     op.advance_rk4(init.clone(), w.dt, 2)
@@ -172,6 +173,7 @@ def bench_c1(args):
                         "plain FP64 RK4",
             "interior_cells": cells, "steps": steps,
             "value": cells * steps / tg, "ms_per_step": tg / steps * 1e3,
+            "face_launch": op.ctx.face_launch(),
             "cpu_oracle": {"value": c_all, "cores": full, "steps": 20},
             "cpu_oracle_1_thread": {"value": c_one, "cores": 1, "steps": 3}}
 
@@ -252,6 +254,7 @@ def bench_batched(args):
     bg, opts, levels, nsteps = workloads.binary_pn_levels()
     steps = args.steps if args.steps_given else nsteps
     ctx = Context()
+    ctx.set_autotune(args.autotune)
     ops = [LevelOperator(bg, boxes, opts, time_dependent=True, context=ctx)
            for boxes, _, _ in levels]
     init = [lev.pack(fabs) for lev, (_, fabs, _) in zip(ops, levels)]
@@ -351,6 +354,13 @@ def bench_batched(args):
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
     plain = cells * plain_steps / tp
+    # the face launch each level used (a call of no steps selects the
+    # level's plan, with what was tuned on it, and runs nothing)
+    face_launch = []
+    for lev, pool, (_, _, dt) in zip(ops, init, levels):
+        lev.advance_rk4(pool, dt, 0, time=0.0)
+        face_launch.append({"fp64": ctx.face_launch(),
+                            "fp32": ctx.face_launch(mixed=True)})
     print("bench: batched %.3f s for %d steps (%.3e cell-updates/s); plain "
           "%.3f s for %d steps (%.3e)" % (tb, steps, batched, tp,
                                            plain_steps, plain),
@@ -367,6 +377,7 @@ def bench_batched(args):
             "plain_per_box": {"value": plain, "steps": plain_steps,
                               "ms_per_step": tp / plain_steps * 1e3},
             "speedup_vs_plain": batched / plain,
+            "face_launch_per_level": face_launch,
             "batched_mixed": {
                 "value": mixed, "ms_per_step": tm / steps * 1e3,
                 "speedup_vs_plain_per_box": mixed / plain,
@@ -399,6 +410,7 @@ def bench_mixed(args):
     o32 = dataclasses.replace(w.options, mixed_precision=True)
     op32 = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, o32,
                                                 context=op64.ctx)
+    op64.ctx.set_autotune(args.autotune)
     init = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
     cells = w.box.num_interior
     print("bench: mixed precision, %s, %d interior cells, %d steps"
@@ -610,6 +622,10 @@ def main():
                          "run")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the one-call-per-box comparison run")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every face launch (walk length, fused or "
+                         "not) on the first call of each level and keep "
+                         "the fastest, instead of the size rules")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 512 MiB device copy (x5), the "
                          "reference kernel for scaling FETCH_SIZE/WRITE_SIZE")
@@ -648,6 +664,7 @@ def main():
 
     w = workloads.torus_box(n=args.n, nsteps=args.steps)
     op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, w.options)
+    op.ctx.set_autotune(args.autotune)
     init = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
     print("bench: %s, %d interior cells, dt %.4g; warmup %d, timed %d steps"
           % (w.name, w.box.num_interior, w.dt, args.warmup, args.steps),
@@ -708,6 +725,7 @@ def main():
                       "interior_cells": cells, "ghosts": w.box.ghosts,
                       "dt": w.dt, "parallelism": "single GPU"},
            "roofline": roofline,
+           "face_launch": op.ctx.face_launch(),
            "step_hbm": {"alg_bytes_per_cell_update": STEP_ALG_BYTES,
                         "achieved_gbs": value * STEP_ALG_BYTES / 1e9,
                         "frac": value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS}}
@@ -723,6 +741,8 @@ def main():
                           "ms_per_step": b["ms_per_step"],
                           "plain_per_box": b["plain_per_box"],
                           "speedup_vs_plain": b["speedup_vs_plain"],
+                          "face_launch_per_level":
+                              b["face_launch_per_level"],
                           "batched_mixed": b["batched_mixed"]}
     if not args.no_mixed:
         torch.cuda.empty_cache()

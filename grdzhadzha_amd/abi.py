@@ -165,6 +165,9 @@ SIGNATURES = {
                                     _P(GrhdDiagnostics), _P(GrhdError), _vp]),
     "grhd_ctx_set_timing": (_i, [_vp, _i]),
     "grhd_ctx_get_timing": (_i, [_vp, _P(_d), _P(ctypes.c_longlong)]),
+    "grhd_ctx_set_autotune": (_i, [_vp, _i]),
+    "grhd_ctx_get_face_launch": (_i, [_vp, _i, _P(_i), _P(_i), _P(_i),
+                                      _P(_d)]),
 }
 
 KERNEL_CLASSES = ("prims", "faces_x", "faces_y", "faces_z", "cells", "other")

@@ -135,7 +135,24 @@ int check_common(const grhd_params *o, const grhd_background *bg, int ncomp,
     return GRHD_OK;
 }
 
-/* Device copies of a level's box table, block tables and exchange list. */
+/* Walk lengths of the marching y/z face kernels that the tuner times. */
+constexpr int kNumSegs = 4;
+constexpr int kSegs[kNumSegs] = {1, 2, 4, 8};
+
+/* The device copies that depend on the walk length: the box table (each
+ * BoxDev carries it) and the y, z and fused face block tables. */
+struct SegTables
+{
+    bool built = false;
+    BoxDev *d_boxes = nullptr;
+    int2 *d_items[3] = {nullptr, nullptr, nullptr}; /* C_FY, C_FZ, C_FALL */
+    int nblocks[3] = {0, 0, 0};
+};
+
+/* Device copies of a level's box table, block tables and exchange list.
+ * d_boxes and the C_FY, C_FZ and C_FALL entries of d_items point into the
+ * SegTables of the walk length in use (seg_cur); the plan owns the other
+ * tables itself. */
 struct Plan
 {
     /* what the plan was built from */
@@ -152,15 +169,131 @@ struct Plan
     BoxDev *d_boxes = nullptr;
     int2 *d_items[NCLASS] = {nullptr};
     int4 *d_exchange = nullptr;
+    /* walk-length variants of the face tables, and the face launch chosen
+     * for FP64 ([0]) and FP32 ([1]) faces: by the size rules, or by timing
+     * every variant on the level when the context tunes */
+    SegTables segt[kNumSegs];
+    int seg_rule = 0; /* index into kSegs */
+    int seg_cur = -1;
+    bool tuned[2] = {false, false};
+    int seg_pick[2] = {0, 0};
+    bool fused_pick[2] = {false, false};
+    float tuned_ms[2] = {0.0f, 0.0f};
 };
+
+/* The face-launch choices measured on one level (box list). */
+struct TuneRecord
+{
+    std::vector<grhd_level_box> key;
+    int ncomp = -1;
+    bool tuned[2] = {false, false};
+    int seg_pick[2] = {0, 0};
+    bool fused_pick[2] = {false, false};
+    float tuned_ms[2] = {0.0f, 0.0f};
+};
+
+/* Levels whose tuned choices a context remembers (oldest dropped first). */
+constexpr size_t kMaxTuneRecords = 64;
+
+bool is_seg_class(int c)
+{
+    return c == C_FY || c == C_FZ || c == C_FALL;
+}
+
+int seg_slot(int c)
+{
+    return c == C_FY ? 0 : (c == C_FZ ? 1 : 2);
+}
 
 void free_plan(Plan &p)
 {
-    (void)hipFree(p.d_boxes);
     for (int c = 0; c < NCLASS; ++c)
-        (void)hipFree(p.d_items[c]);
+        if (!is_seg_class(c))
+            (void)hipFree(p.d_items[c]);
+    for (SegTables &t : p.segt)
+    {
+        (void)hipFree(t.d_boxes);
+        for (int2 *q : t.d_items)
+            (void)hipFree(q);
+    }
     (void)hipFree(p.d_exchange);
     p = Plan();
+}
+
+/* Block table of one item class: blocks of kBlock items, box by box. */
+std::vector<int2> class_table(const std::vector<BoxDev> &hb, int c)
+{
+    std::vector<int2> items;
+    if (c == C_FALL)
+    {
+        for (int f : {C_FX, C_FY, C_FZ})
+        {
+            const std::vector<int2> t = class_table(hb, f);
+            items.insert(items.end(), t.begin(), t.end());
+        }
+        return items;
+    }
+    for (int n = 0; n < (int)hb.size(); ++n)
+    {
+        const long long m = class_items(hb[n], c);
+        for (long long first = 0; first < m; first += kBlock)
+            items.push_back(make_int2(n, (int)first));
+    }
+    return items;
+}
+
+int upload_table(const std::vector<int2> &items, int2 **d)
+{
+    if (items.empty())
+        return GRHD_OK;
+    if (hipMalloc((void **)d, sizeof(int2) * items.size()) != hipSuccess ||
+        hipMemcpy(*d, items.data(), sizeof(int2) * items.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return GRHD_ERR_DEVICE;
+    return GRHD_OK;
+}
+
+/* Build (once) the box table and face tables of walk length kSegs[k]. */
+int build_seg_tables(Plan &p, int k)
+{
+    SegTables &t = p.segt[k];
+    if (t.built)
+        return GRHD_OK;
+    std::vector<BoxDev> hb = p.hb;
+    for (BoxDev &d : hb)
+        d.seg = kSegs[k];
+    for (int c : {C_FY, C_FZ, C_FALL})
+    {
+        const std::vector<int2> items = class_table(hb, c);
+        t.nblocks[seg_slot(c)] = (int)items.size();
+        const int rc = upload_table(items, &t.d_items[seg_slot(c)]);
+        if (rc)
+            return rc;
+    }
+    if (hipMalloc((void **)&t.d_boxes, sizeof(BoxDev) * hb.size()) !=
+            hipSuccess ||
+        hipMemcpy(t.d_boxes, hb.data(), sizeof(BoxDev) * hb.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return GRHD_ERR_DEVICE;
+    t.built = true;
+    return GRHD_OK;
+}
+
+/* Point the plan's box table and face tables at walk length kSegs[k]
+ * (built).  Kernels take these pointers by value at launch, so switching
+ * between launches needs no synchronisation. */
+void select_seg(Plan &p, int k)
+{
+    if (p.seg_cur == k)
+        return;
+    const SegTables &t = p.segt[k];
+    p.d_boxes = t.d_boxes;
+    for (int c : {C_FY, C_FZ, C_FALL})
+    {
+        p.d_items[c] = t.d_items[seg_slot(c)];
+        p.nblocks[c] = t.nblocks[seg_slot(c)];
+    }
+    p.seg_cur = k;
 }
 
 bool same_key(const Plan &p, const grhd_level_box *boxes, int nbox,
@@ -315,34 +448,30 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
     if (2 * p.tot_g >= (1ll << kKeyBits))
         return GRHD_ERR_INVALID_ARGUMENT;
     const int seg = march_segment(p.tot_i);
+    for (int k = 0; k < kNumSegs; ++k)
+        if (kSegs[k] == seg)
+            p.seg_rule = k;
     for (BoxDev &d : p.hb)
         d.seg = seg;
 
-    /* block tables */
-    std::vector<int2> tables[NCLASS];
+    /* block tables: the classes that do not depend on the walk length
+     * here, the face tables of the rule's walk length in segt */
     for (int c = 0; c < NCLASS; ++c)
     {
-        std::vector<int2> &items = tables[c];
-        if (c == C_FALL)
-            for (int f : {C_FX, C_FY, C_FZ})
-                items.insert(items.end(), tables[f].begin(), tables[f].end());
-        else
-            for (int n = 0; n < nbox; ++n)
-            {
-                const long long m = class_items(p.hb[n], c);
-                for (long long first = 0; first < m; first += kBlock)
-                    items.push_back(make_int2(n, (int)first));
-            }
+        if (is_seg_class(c))
+            continue;
+        const std::vector<int2> items = class_table(p.hb, c);
         p.nblocks[c] = (int)items.size();
-        if (!items.empty())
-        {
-            if (hipMalloc((void **)&p.d_items[c],
-                          sizeof(int2) * items.size()) != hipSuccess ||
-                hipMemcpy(p.d_items[c], items.data(),
-                          sizeof(int2) * items.size(),
-                          hipMemcpyHostToDevice) != hipSuccess)
-                return GRHD_ERR_DEVICE;
-        }
+        const int rc = upload_table(items, &p.d_items[c]);
+        if (rc)
+            return rc;
+    }
+    {
+        const int rc = build_seg_tables(p, p.seg_rule);
+        if (rc)
+            return rc;
+        select_seg(p, p.seg_rule);
+        p.seg_pick[0] = p.seg_pick[1] = p.seg_rule;
     }
 
     /* internal-ghost exchange list: every exterior cell that is not an
@@ -398,11 +527,6 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
                       hipMemcpyHostToDevice) != hipSuccess)
             return GRHD_ERR_DEVICE;
     }
-    if (hipMalloc((void **)&p.d_boxes, sizeof(BoxDev) * p.hb.size()) !=
-            hipSuccess ||
-        hipMemcpy(p.d_boxes, p.hb.data(), sizeof(BoxDev) * p.hb.size(),
-                  hipMemcpyHostToDevice) != hipSuccess)
-        return GRHD_ERR_DEVICE;
     return GRHD_OK;
 }
 
@@ -431,6 +555,12 @@ struct grhd_ctx
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t fork = nullptr;
     hipEvent_t join[2] = {nullptr, nullptr};
+    /* face-launch tuning (grhd_ctx_set_autotune), its two timing events,
+     * and the choices measured on the levels this context has seen (the
+     * plan is rebuilt whenever a call names another box list) */
+    bool autotune = false;
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
+    std::vector<TuneRecord> tuned_levels;
     /* instrumentation */
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -511,7 +641,51 @@ int prepare(grhd_ctx *c, const grhd_level_box *boxes, int nbox, int ncomp)
     }
     if (same_key(c->plan, boxes, nbox, ncomp))
         return GRHD_OK;
-    return build_plan(c->plan, boxes, nbox, ncomp);
+    const int rc = build_plan(c->plan, boxes, nbox, ncomp);
+    if (rc)
+        return rc;
+    /* a level tuned before keeps what was measured on it */
+    Plan &pl = c->plan;
+    for (const TuneRecord &t : c->tuned_levels)
+        if (t.ncomp == ncomp && t.key.size() == (size_t)nbox &&
+            std::memcmp(t.key.data(), boxes,
+                        sizeof(grhd_level_box) * (size_t)nbox) == 0)
+            for (int q = 0; q < 2; ++q)
+            {
+                pl.tuned[q] = t.tuned[q];
+                pl.seg_pick[q] = t.seg_pick[q];
+                pl.fused_pick[q] = t.fused_pick[q];
+                pl.tuned_ms[q] = t.tuned_ms[q];
+            }
+    return GRHD_OK;
+}
+
+/* Remember the plan's tuned choices for its box list. */
+void remember_tuning(grhd_ctx *c)
+{
+    const Plan &pl = c->plan;
+    TuneRecord *rec = nullptr;
+    for (TuneRecord &t : c->tuned_levels)
+        if (t.ncomp == pl.ncomp && t.key.size() == pl.key.size() &&
+            std::memcmp(t.key.data(), pl.key.data(),
+                        sizeof(grhd_level_box) * pl.key.size()) == 0)
+            rec = &t;
+    if (!rec)
+    {
+        if (c->tuned_levels.size() >= kMaxTuneRecords)
+            c->tuned_levels.erase(c->tuned_levels.begin());
+        c->tuned_levels.emplace_back();
+        rec = &c->tuned_levels.back();
+        rec->key = pl.key;
+        rec->ncomp = pl.ncomp;
+    }
+    for (int q = 0; q < 2; ++q)
+    {
+        rec->tuned[q] = pl.tuned[q];
+        rec->seg_pick[q] = pl.seg_pick[q];
+        rec->fused_pick[q] = pl.fused_pick[q];
+        rec->tuned_ms[q] = pl.tuned_ms[q];
+    }
 }
 
 /* ---------------- instrumentation ---------------- */
@@ -616,18 +790,16 @@ void launch_prims(grhd_ctx *c, const Options &o, const Background &bg,
 constexpr long long kFusedFacesCells = 3ll << 20;
 
 /* The three face directions are independent (W in, each its own flux
- * pool out).  Small calls launch them as one kernel.  On calls of at least
- * kFusedFacesCells interior cells, where the fused kernel's register
- * count costs the x faces more than the launch boundaries, y and z run on
- * the context's side streams beside x, forked after everything already
- * queued on s (the ghost fill and recovery of this stage, the previous
- * stage's cell kernel) and joined back into s before the cell kernel.
- * The faces timing class covers all three. */
+ * pool out), so they can run as one fused launch or as three kernels.
+ * Unfused, y and z run on the context's side streams beside x, forked
+ * after everything already queued on s (the ghost fill and recovery of
+ * this stage, the previous stage's cell kernel) and joined back into s
+ * before the cell kernel. */
 template <class T>
-void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
-                    const Pools &p, hipStream_t s)
+void faces_launches(grhd_ctx *c, const Options &o, const Background &bg,
+                    const Pools &p, hipStream_t s, bool fused)
 {
-    if (c->plan.tot_i < kFusedFacesCells)
+    if (fused)
     {
         if (c->plan.nblocks[C_FALL] > 0)
             hipLaunchKernelGGL(k_faces_all<T>, dim3(c->plan.nblocks[C_FALL]),
@@ -635,7 +807,6 @@ void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
                                c->plan.d_items[C_FALL],
                                c->plan.nblocks[C_FX], c->plan.nblocks[C_FY],
                                o, bg, p);
-        tick(c, GRHD_KERNEL_FACES_X, s);
         return;
     }
     /* a failed fork runs the three kernels in order on `s`; the error
@@ -650,7 +821,6 @@ void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
         GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
         GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), s, o, bg, p);
         GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), s, o, bg, p);
-        tick(c, GRHD_KERNEL_FACES_X, s);
         return;
     }
     GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), c->side[0], o, bg, p);
@@ -663,6 +833,94 @@ void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
         if (hipEventRecord(c->join[q], c->side[q]) != hipSuccess ||
             hipStreamWaitEvent(s, c->join[q], 0) != hipSuccess)
             (void)hipStreamSynchronize(c->side[q]);
+}
+
+/* Launches per candidate in a tuning measurement (after one untimed). */
+constexpr int kTuneReps = 3;
+
+/* Time every face launch of the level -- each walk length of the marching
+ * kernels, fused and unfused -- on the call's own primitives, and keep the
+ * fastest for faces of type T.  The faces only read W and write their
+ * flux pools, and the fluxes do not depend on the walk length or the
+ * launch form, so the extra launches change no result: the call's own
+ * face launch follows and writes the same fluxes.  The rule's choice is
+ * timed first and kept on ties.  The host waits for each measurement, so
+ * a tuning call synchronises once per candidate; it runs once per plan
+ * and precision.  If an event operation fails, the rule's choice stays. */
+template <class T>
+void tune_faces(grhd_ctx *c, const Options &o, const Background &bg,
+                const Pools &p, hipStream_t s)
+{
+    Plan &pl = c->plan;
+    const int prec = sizeof(T) < sizeof(double) ? 1 : 0;
+    pl.tuned[prec] = true;
+    int best_k = pl.seg_rule;
+    bool best_f = pl.tot_i < kFusedFacesCells;
+    float best = 0.0f;
+    bool have = false;
+    int order[kNumSegs];
+    order[0] = pl.seg_rule;
+    for (int k = 0, n = 1; k < kNumSegs; ++k)
+        if (k != pl.seg_rule)
+            order[n++] = k;
+    for (int k : order)
+    {
+        if (build_seg_tables(pl, k) != GRHD_OK)
+            continue;
+        select_seg(pl, k);
+        const bool rule_fused = pl.tot_i < kFusedFacesCells;
+        for (bool fused : {rule_fused, !rule_fused})
+        {
+            faces_launches<T>(c, o, bg, p, s, fused);
+            float ms = 0.0f;
+            if (hipEventRecord(c->tune_ev[0], s) != hipSuccess)
+                return;
+            for (int r = 0; r < kTuneReps; ++r)
+                faces_launches<T>(c, o, bg, p, s, fused);
+            if (hipEventRecord(c->tune_ev[1], s) != hipSuccess ||
+                hipEventSynchronize(c->tune_ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]) !=
+                    hipSuccess)
+                return;
+            if (!have || ms < best)
+            {
+                best = ms;
+                best_k = k;
+                best_f = fused;
+                have = true;
+            }
+        }
+    }
+    pl.seg_pick[prec] = best_k;
+    pl.fused_pick[prec] = best_f;
+    pl.tuned_ms[prec] = have ? best / kTuneReps : 0.0f;
+    remember_tuning(c);
+}
+
+/* The faces of one stage.  By default the size rules choose: the walk
+ * length from march_segment, and one fused launch below kFusedFacesCells
+ * interior cells, where the fused kernel's register count costs the x
+ * faces less than the two launch boundaries it removes.  A tuning context
+ * uses what tune_faces measured instead.  The faces timing class covers
+ * all three directions. */
+template <class T>
+void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
+                    const Pools &p, hipStream_t s)
+{
+    Plan &pl = c->plan;
+    const int prec = sizeof(T) < sizeof(double) ? 1 : 0;
+    if (c->autotune && !pl.tuned[prec])
+        tune_faces<T>(c, o, bg, p, s);
+    bool fused = pl.tot_i < kFusedFacesCells;
+    int k = pl.seg_rule;
+    if (c->autotune && pl.tuned[prec] &&
+        build_seg_tables(pl, pl.seg_pick[prec]) == GRHD_OK)
+    {
+        k = pl.seg_pick[prec];
+        fused = pl.fused_pick[prec];
+    }
+    select_seg(pl, k);
+    faces_launches<T>(c, o, bg, p, s, fused);
     tick(c, GRHD_KERNEL_FACES_X, s);
 }
 
@@ -1533,7 +1791,9 @@ int grhd_ctx_create(grhd_ctx **out)
         hipEventCreateWithFlags(&c->join[0], hipEventDisableTiming) !=
             hipSuccess ||
         hipEventCreateWithFlags(&c->join[1], hipEventDisableTiming) !=
-            hipSuccess)
+            hipSuccess ||
+        hipEventCreate(&c->tune_ev[0]) != hipSuccess ||
+        hipEventCreate(&c->tune_ev[1]) != hipSuccess)
     {
         grhd_ctx_destroy(c);
         return GRHD_ERR_DEVICE;
@@ -1564,6 +1824,8 @@ void grhd_ctx_destroy(grhd_ctx *c)
             (void)hipStreamDestroy(c->side[i]);
         if (c->join[i])
             (void)hipEventDestroy(c->join[i]);
+        if (c->tune_ev[i])
+            (void)hipEventDestroy(c->tune_ev[i]);
     }
     if (c->fork)
         (void)hipEventDestroy(c->fork);
@@ -1870,6 +2132,32 @@ int grhd_ctx_set_timing(grhd_ctx *ctx, int enable)
         ctx->total_ms[i] = 0.0;
         ctx->launches[i] = 0;
     }
+    return GRHD_OK;
+}
+
+int grhd_ctx_set_autotune(grhd_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    ctx->autotune = enable != 0;
+    return GRHD_OK;
+}
+
+int grhd_ctx_get_face_launch(grhd_ctx *ctx, int mixed, int *walk, int *fused,
+                             int *tuned, double *ms)
+{
+    if (!ctx || ctx->plan.ncomp < 0 || (mixed != 0 && mixed != 1))
+        return GRHD_ERR_INVALID_ARGUMENT;
+    const Plan &pl = ctx->plan;
+    const bool t = ctx->autotune && pl.tuned[mixed];
+    if (walk)
+        *walk = kSegs[t ? pl.seg_pick[mixed] : pl.seg_rule];
+    if (fused)
+        *fused = t ? pl.fused_pick[mixed] : pl.tot_i < kFusedFacesCells;
+    if (tuned)
+        *tuned = t;
+    if (ms)
+        *ms = t ? (double)pl.tuned_ms[mixed] : 0.0;
     return GRHD_OK;
 }
 

@@ -79,6 +79,28 @@ class Context:
         return {name: (ms[i], n[i])
                 for i, name in enumerate(abi.KERNEL_CLASSES)}
 
+    def set_autotune(self, enable=True):
+        """Time every face launch (walk length 1/2/4/8, fused or not) on
+        the first call of each new level and precision, and keep the
+        fastest; off, size rules choose.  Results do not depend on it."""
+        rc = self.lib.grhd_ctx_set_autotune(self.handle, int(bool(enable)))
+        if rc != abi.GRHD_OK:
+            raise RuntimeError("grhd_ctx_set_autotune failed (%d)" % rc)
+
+    def face_launch(self, mixed=False):
+        """{'walk', 'fused', 'tuned', 'ms'} of the current level's face
+        launch for FP64 (mixed=False) or FP32 faces."""
+        walk, fused, tuned = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        rc = self.lib.grhd_ctx_get_face_launch(
+            self.handle, int(bool(mixed)), ctypes.byref(walk),
+            ctypes.byref(fused), ctypes.byref(tuned), ctypes.byref(ms))
+        if rc != abi.GRHD_OK:
+            raise RuntimeError("grhd_ctx_get_face_launch: no level call yet "
+                               "(%d)" % rc)
+        return {"walk": walk.value, "fused": bool(fused.value),
+                "tuned": bool(tuned.value), "ms": ms.value}
+
     def close(self):
         if self.handle:
             self.lib.grhd_ctx_destroy(self.handle)

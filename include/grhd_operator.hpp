@@ -237,6 +237,9 @@ class Context
     Context(const Context &) = delete;
     Context &operator=(const Context &) = delete;
     grhd_ctx *get() const { return m_ctx; }
+    /* time the face launches on each new level and keep the fastest
+     * (results are the same either way) */
+    void set_autotune(bool on) { (void)grhd_ctx_set_autotune(m_ctx, on); }
 
   private:
     grhd_ctx *m_ctx = nullptr;

@@ -484,3 +484,40 @@ def test_level_plan_for_4096_boxes_builds_quickly(hip_context):
           % elapsed)
     assert elapsed < 20.0
     assert bool(torch.isfinite(r).all())
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_tuned_face_launch_equals_rule_launch(hip_context, mixed):
+    """A tuning context times the eight face launches (walk length 1, 2, 4
+    or 8; fused or on streams) on its first call over a 10-box binary-PN
+    level and keeps the fastest.  Its results equal those of a context on
+    the size rules bit for bit (state, mask, counts), on the tuning call and
+    on the next call, which reuses the choice without tuning again."""
+    from grdzhadzha_amd.operator import Context
+    bg, opts, levels, _ = workloads.binary_pn_levels(nbox=10, nlevels=1)
+    boxes, fabs, dt = levels[0]
+    opts = dataclasses.replace(opts, mixed_precision=mixed)
+    tuned_ctx = Context()
+    try:
+        tuned_ctx.set_autotune(True)
+        out = []
+        for ctx in (hip_context, tuned_ctx):
+            lev = LevelOperator(bg, boxes, opts, time_dependent=True,
+                                context=ctx)
+            pool = lev.pack(fabs)
+            mask = torch.zeros(lev.num_interior, dtype=torch.uint8,
+                               device="cuda")
+            d1 = lev.advance_rk4(pool, dt, nsteps=1, time=0.0)
+            d2 = lev.advance_rk4(pool, dt, nsteps=1, time=dt, mask=mask)
+            out.append((pool.cpu().numpy(), mask.cpu().numpy(), d1, d2))
+        choice = tuned_ctx.face_launch(mixed=mixed)
+        rule = hip_context.face_launch(mixed=mixed)
+    finally:
+        tuned_ctx.close()
+    print("face launch: rule", rule, "tuned", choice)
+    assert choice["tuned"] and not rule["tuned"]
+    assert choice["walk"] in (1, 2, 4, 8) and choice["ms"] > 0.0
+    (p0, m0, a0, b0), (p1, m1, a1, b1) = out
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(m0, m1)
+    assert (a0, b0) == (a1, b1)
