@@ -229,6 +229,16 @@ def test_rk4_torus_two_steps(hip_context):
     _step_case(small_torus(), hip_context, "rk4", 2)
 
 
+def test_rk4_torus_128_full_size_one_step(hip_context):
+    """The benchmark workload itself at full size: one RK4 step of the
+    128^3 Kerr a = 0.9 torus (2.1M interior cells, minmod, dt = 0.25 dx),
+    the size at which the face kernels walk 8 faces per thread and run
+    fused.  State within 1e-12 of the oracle (in practice bit-equal),
+    masks and recovery counts exact.  The oracle needs a few seconds on
+    the host's cores."""
+    _step_case(workloads.torus_box(n=128, nsteps=1), hip_context, "rk4", 1)
+
+
 def test_ssprk2_michel(hip_context):
     _step_case(workloads.michel_box_32(), hip_context, "ssprk2")
 
