@@ -259,6 +259,14 @@ int build_seg_tables(Plan &p, int k)
     SegTables &t = p.segt[k];
     if (t.built)
         return GRHD_OK;
+    /* what an earlier failed attempt left */
+    (void)hipFree(t.d_boxes);
+    t.d_boxes = nullptr;
+    for (int2 *&q : t.d_items)
+    {
+        (void)hipFree(q);
+        q = nullptr;
+    }
     std::vector<BoxDev> hb = p.hb;
     for (BoxDev &d : hb)
         d.seg = kSegs[k];
