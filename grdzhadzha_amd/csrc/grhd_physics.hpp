@@ -979,23 +979,29 @@ __device__ __forceinline__ void hlle_face_g(PrimT<T> lp, PrimT<T> rp,
     char_speeds(rp, gamma, face, d, rl, ru);
     const T sl = std_min(T(0), std_min(ll, rl));
     const T su = std_max(T(0), std_max(lu, ru));
-    if (sl >= T(0))
-    {
-        f = fl;
-        return;
-    }
-    if (su <= T(0))
-    {
-        f = fr;
-        return;
-    }
+    /* The left flux if s_lower >= 0, else the right flux if s_upper <= 0,
+     * else the HLLE combination.  The choice is made per component with
+     * selects rather than three exits: with exits the compiler merges the
+     * three stores of the momentum flux into one at a run-time offset and
+     * keeps that array in scratch memory.  The combination is evaluated on
+     * every lane; where it is not chosen (s_lower = 0 or s_upper = 0) its
+     * value, possibly 0 * inf, is discarded. */
+    const bool left = sl >= T(0);
+    const bool right = su <= T(0);
     const T inv = T(1) / (su - sl);
-    f.D = (su * fl.D - sl * fr.D + sl * su * (rc.D - lc.D)) * inv;
-    f.tau = (su * fl.tau - sl * fr.tau + sl * su * (rc.tau - lc.tau)) * inv;
+    const T hD = (su * fl.D - sl * fr.D + sl * su * (rc.D - lc.D)) * inv;
+    const T hT =
+        (su * fl.tau - sl * fr.tau + sl * su * (rc.tau - lc.tau)) * inv;
+    f.D = left ? fl.D : (right ? fr.D : hD);
+    f.tau = left ? fl.tau : (right ? fr.tau : hT);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        f.S[i] = (su * fl.S[i] - sl * fr.S[i] + sl * su * (rc.S[i] - lc.S[i])) *
-                 inv;
+    {
+        const T hS =
+            (su * fl.S[i] - sl * fr.S[i] + sl * su * (rc.S[i] - lc.S[i])) *
+            inv;
+        f.S[i] = left ? fl.S[i] : (right ? fr.S[i] : hS);
+    }
 }
 
 template <class T>
