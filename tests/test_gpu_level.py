@@ -521,3 +521,27 @@ def test_tuned_face_launch_equals_rule_launch(hip_context, mixed):
     assert np.array_equal(p0, p1)
     assert np.array_equal(m0, m1)
     assert (a0, b0) == (a1, b1)
+
+
+def test_tuning_is_remembered_per_level(hip_context):
+    """A tuning context shared by two levels tunes each once: after a call
+    on another level, returning to the first restores the choice measured
+    on it (the same milliseconds, not a new measurement)."""
+    from grdzhadzha_amd.operator import Context
+    bg, opts, levels, _ = workloads.binary_pn_levels(nbox=6, nlevels=2)
+    ctx = Context()
+    try:
+        ctx.set_autotune(True)
+        levs = [LevelOperator(bg, boxes, opts, time_dependent=True,
+                              context=ctx) for boxes, _, _ in levels]
+        pools = [lev.pack(fabs) for lev, (_, fabs, _) in zip(levs, levels)]
+        first = []
+        for lev, pool, (_, _, dt) in zip(levs, pools, levels):
+            lev.advance_rk4(pool, dt, nsteps=1, time=0.0)
+            first.append(ctx.face_launch())
+        for lev, pool, (_, _, dt), want in zip(levs, pools, levels, first):
+            lev.advance_rk4(pool, dt, nsteps=1, time=dt)
+            assert ctx.face_launch() == want
+        assert all(f["tuned"] and f["ms"] > 0.0 for f in first)
+    finally:
+        ctx.close()
