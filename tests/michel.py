@@ -307,3 +307,46 @@ def convergence_options(cfg, theta=2.0):
 def rhs_l1(box, rhs):
     """Volume-weighted L1 norm of each conserved rhs component."""
     return [float(np.abs(rhs[c]).sum()) * box.dx ** 3 for c in range(5)]
+
+
+# ------------------------------------------------------- accretion rate
+def accretion_box_state(cfg, n, half_width=16.0, inner_radius=12.0,
+                        ghosts=3):
+    """n^3 cells on [-half_width, half_width]^3 (box = ProblemDomain) with
+    the exact subsonic Michel state wherever |x| >= inner_radius and, inside
+    it, the exact state of the sphere |x| = inner_radius carried radially
+    inwards (any valid state: the accretion-rate shells lie outside).
+    Returns (box, state)."""
+    from grdzhadzha_amd import workloads
+    dx = 2.0 * half_width / n
+    box = BoxSpec((0, 0, 0), (n - 1,) * 3, ghosts, dx, (half_width,) * 3)
+    c = make_constants(cfg)
+    X, Y, Z = workloads.cell_centres(box)
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    rr = np.maximum(r, inner_radius)
+    rho = solve_density_many(c, cfg, rr.ravel()).reshape(r.shape)
+    p = c["K"] * rho ** cfg.gamma
+    eps = p / ((cfg.gamma - 1.0) * rho)
+    ur = c["mass_flux"] / (rr * rr * rho)
+    f = 1.0 - 2.0 * cfg.mass / rr
+    H = cfg.mass / rr
+    g_rr = 1.0 + 2.0 * H
+    lapse = 1.0 / np.sqrt(g_rr)
+    shift_r = 2.0 * H / g_rr
+    g_tt = -lapse * lapse + g_rr * shift_r * shift_r
+    W = lapse * (-np.sqrt(f + ur * ur) - g_rr * shift_r * ur) / g_tt
+    vr = ur / W + shift_r / lapse
+    v = np.stack([vr * X / r, vr * Y / r, vr * Z / r])
+    g, _, _ = workloads.kerr_schild_metric(cfg.mass, 0.0, X, Y, Z)
+    D, S, tau = workloads.prim_to_cons(rho, eps, p, v, g, cfg.gamma)
+    return box, workloads.pack_state(box, rho, eps, p, v, D, S, tau)
+
+
+def shell_cells(box, radius):
+    """Radii of the interior cells with ||x| - radius| < dx/2, the shell
+    the accretion-rate diagnostics sum over."""
+    from grdzhadzha_amd import workloads
+    X, Y, Z = workloads.cell_centres(box)
+    sl = box.interior_slices()
+    r = np.sqrt(X[sl] ** 2 + Y[sl] ** 2 + Z[sl] ** 2)
+    return r[np.abs(r - radius) < 0.5 * box.dx]

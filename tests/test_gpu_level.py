@@ -7,7 +7,8 @@
   poisoned internal ghosts are repaired by the exchange
   (GRHDLevelDataBinaryPNSmoke.cpp:204-233, logged error 0).
 - The reference's own 2-box LevelData Michel run (GRHD_PROGRESS.md:423-428)
-  reproduces its printed numbers on the GPU.
+  and its 2-box moving-binary ring run (GRHD_PROGRESS.md:308-311, :339-343)
+  reproduce their printed numbers on the GPU.
 """
 import dataclasses
 
@@ -20,7 +21,7 @@ import oracle
 from grdzhadzha_amd import workloads
 from grdzhadzha_amd.level import LevelOperator, split_box, sub_fab
 from grdzhadzha_amd.operator import FixedBGLevelDataFiniteVolumeOperator
-from helpers import assert_conserved_close
+from helpers import assert_bitwise
 
 pytestmark = pytest.mark.gpu
 
@@ -60,7 +61,8 @@ def test_level_rk4_equals_each_box_alone(hip_context):
         dref, mref = oracle.Oracle(bg, boxes[n], opts,
                                    time_dependent=True).advance_rk4(
             ref, dt, 2, time=0.0)
-        assert_conserved_close(got[n], ref, 1e-12, what="box %d" % n)
+        sl = (slice(None),) + boxes[n].interior_slices()
+        assert_bitwise(got[n][sl], ref[sl], "box %d interior" % n)
         assert np.array_equal(masks[n], mref)
 
 
@@ -150,6 +152,49 @@ def test_level_michel_anchor_on_gpu(hip_context, michel_split):
         sl = (slice(None),) + h.interior_slices()
         ref = sub_fab(full, box, h)[sl]
         assert np.array_equal(f[sl], ref)
+
+
+def test_level_binary_pn_ring_anchor_on_gpu(hip_context):
+    """The reference's binary-PN ring smoke run (tests/binary_ring.py) as
+    the 2-box level it is: time-dependent operator, internal ghosts
+    exchanged on the device, CFL from the device max 1/dt, 4 SSPRK2 steps
+    to t = 0.2.  The printed numbers come out, and the state equals the
+    oracle's single-box run bit for bit."""
+    import binary_ring
+    box, halves, bg, opts, state = binary_ring.setup()
+    lev = LevelOperator(bg, halves, opts, time_dependent=True,
+                        context=hip_context)
+    pool = lev.pack([binary_ring.half_fab(state, box, h) for h in halves])
+    d0 = lev.recover_primitives(pool, time=0.0)
+    assert d0["num_failed_recoveries"] == 0
+    assert d0["num_conserved_resets"] == 0
+    geo0 = [oracle.Oracle(bg, h, opts).cell_geometry(time=0.0)
+            for h in halves]
+    di = binary_ring.diagnostics(halves, lev.unpack(pool), geo0, opts)
+    O = oracle.Oracle(bg, box, opts, time_dependent=True)
+    ref = state.copy()
+    O.recover_primitives(ref, time=0.0)
+    t = 0.0
+    for _ in range(4):
+        dt = min(0.05, 0.25 * lev.compute_stable_dt(pool, 0.2, time=t))
+        dt_ref = min(0.05, 0.25 * O.compute_stable_dt(ref, 0.2, time=t))
+        assert dt == dt_ref
+        lev.advance_ssprk2(pool, dt, time=t)
+        O.advance_ssprk2(ref, dt, time=t)
+        t += dt
+    fabs = lev.unpack(pool)
+    geo1 = [oracle.Oracle(bg, h, opts).cell_geometry(time=t)
+            for h in halves]
+    df = binary_ring.diagnostics(halves, fabs, geo1, opts)
+    a = binary_ring.ANCHOR
+    assert float("%.6g" % t) == a["final_time"]
+    assert float("%.6g" % di["mass"]) == a["initial_mass"]
+    assert float("%.6g" % df["mass"]) == a["final_mass"]
+    assert float("%.6g" % df["max_rho"]) == a["max_rho"]
+    assert float("%.6g" % df["max_v2"]) == a["max_v2"]
+    for f, h in zip(fabs, halves):
+        sl = (slice(None),) + h.interior_slices()
+        assert np.array_equal(f[sl], binary_ring.half_fab(ref, box, h)[sl])
 
 
 def test_level_rk4_split_equals_single_box(hip_context, michel_split):
@@ -303,11 +348,10 @@ def test_level_rhs_periodic_box_matches_oracle(hip_context, periodic):
     rhs_ref, mask_ref = oracle.Oracle(bg, box, opts).compute_rhs(
         ref_state, with_mask=True)
     assert np.abs(rhs_ref[0:5]).max() > 0.0
-    assert_conserved_close(got, rhs_ref, 1e-12, what="periodic rhs")
+    assert_bitwise(got, rhs_ref, "periodic rhs")
     assert np.array_equal(lev.split_interior(smask)[0], mask_ref)
     # the state's ghosts end as the oracle's: wrapped, then outflow-filled
-    assert_conserved_close(lev.unpack(pool)[0], ref_state, 1e-12,
-                           what="periodic ghosts")
+    assert_bitwise(lev.unpack(pool)[0], ref_state, "periodic ghosts")
 
 
 def test_level_rk4_periodic_split_equals_single_box(hip_context):
