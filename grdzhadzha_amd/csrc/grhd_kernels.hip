@@ -135,6 +135,16 @@ int check_common(const grhd_params *o, const grhd_background *bg, int ncomp,
     return GRHD_OK;
 }
 
+inline grhd_level_box one_box(const grhd_box *b)
+{
+    grhd_level_box lb;
+    std::memset(&lb, 0, sizeof(lb));
+    if (b)
+        lb.box = *b;
+    lb.state_offset = 0;
+    return lb;
+}
+
 /* Walk lengths of the marching y/z face kernels that the tuner times. */
 constexpr int kNumSegs = 4;
 constexpr int kSegs[kNumSegs] = {1, 2, 4, 8};
@@ -175,7 +185,8 @@ struct Plan
     SegTables segt[kNumSegs];
     int seg_rule = 0; /* index into kSegs */
     int seg_cur = -1;
-    bool tuned[2] = {false, false};
+    bool tuned[2] = {false, false};      /* measured choice in seg/fused_pick */
+    bool tune_tried[2] = {false, false}; /* a measurement was started */
     int seg_pick[2] = {0, 0};
     bool fused_pick[2] = {false, false};
     float tuned_ms[2] = {0.0f, 0.0f};
@@ -273,10 +284,10 @@ int build_seg_tables(Plan &p, int k)
     for (int c : {C_FY, C_FZ, C_FALL})
     {
         const std::vector<int2> items = class_table(hb, c);
-        t.nblocks[seg_slot(c)] = (int)items.size();
         const int rc = upload_table(items, &t.d_items[seg_slot(c)]);
         if (rc)
             return rc;
+        t.nblocks[seg_slot(c)] = (int)items.size();
     }
     if (hipMalloc((void **)&t.d_boxes, sizeof(BoxDev) * hb.size()) !=
             hipSuccess ||
@@ -410,6 +421,48 @@ int owner(const OwnerIndex &ix, const std::vector<BoxDev> &hb, int self,
     return -1;
 }
 
+/* The kernels' view of box `lb`, placed after the storage cells, interior
+ * cells and faces (tot_g, tot_i, tot_f) of the boxes before it, which it
+ * then advances.  Returns whether every exterior cell of the box is an
+ * outflow ghost (box = ProblemDomain, no periodic direction). */
+bool add_boxdev(const grhd_level_box &lb, BoxDev &d, long long &tot_g,
+                long long &tot_i, long long tot_f[3])
+{
+    const grhd_box *b = &lb.box;
+    bool physical = true;
+    for (int i = 0; i < 3; ++i)
+    {
+        d.lo[i] = b->lo[i];
+        d.hi[i] = b->hi[i];
+        d.glo[i] = b->glo[i];
+        d.ghi[i] = b->ghi[i];
+        d.dlo[i] = b->dlo[i];
+        d.dhi[i] = b->dhi[i];
+        d.periodic[i] = b->periodic[i];
+        d.gn[i] = b->ghi[i] - b->glo[i] + 1;
+        d.in[i] = b->hi[i] - b->lo[i] + 1;
+        d.center[i] = b->center[i];
+        if (b->periodic[i] || b->lo[i] != b->dlo[i] || b->hi[i] != b->dhi[i])
+            physical = false;
+    }
+    d.gnpts = (long long)d.gn[0] * d.gn[1] * d.gn[2];
+    d.inpts = (long long)d.in[0] * d.in[1] * d.in[2];
+    d.dx = b->dx;
+    d.off_u = lb.state_offset;
+    d.off_g = tot_g;
+    d.off_i = tot_i;
+    for (int k = 0; k < 3; ++k)
+    {
+        d.off_f[k] = tot_f[k];
+        d.nface[k] = faces(d, k);
+        tot_f[k] += d.nface[k];
+    }
+    d.inv_dx = 1.0 / d.dx;
+    tot_g += d.gnpts;
+    tot_i += d.inpts;
+    return physical;
+}
+
 int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
 {
     free_plan(p);
@@ -417,42 +470,8 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
     p.ncomp = ncomp;
     p.hb.resize(nbox);
     for (int n = 0; n < nbox; ++n)
-    {
-        const grhd_box *b = &boxes[n].box;
-        BoxDev &d = p.hb[n];
-        for (int i = 0; i < 3; ++i)
-        {
-            d.lo[i] = b->lo[i];
-            d.hi[i] = b->hi[i];
-            d.glo[i] = b->glo[i];
-            d.ghi[i] = b->ghi[i];
-            d.dlo[i] = b->dlo[i];
-            d.dhi[i] = b->dhi[i];
-            d.periodic[i] = b->periodic[i];
-            d.gn[i] = b->ghi[i] - b->glo[i] + 1;
-            d.in[i] = b->hi[i] - b->lo[i] + 1;
-            d.center[i] = b->center[i];
-        }
-        d.gnpts = (long long)d.gn[0] * d.gn[1] * d.gn[2];
-        d.inpts = (long long)d.in[0] * d.in[1] * d.in[2];
-        d.dx = b->dx;
-        d.off_u = boxes[n].state_offset;
-        d.off_g = p.tot_g;
-        d.off_i = p.tot_i;
-        for (int k = 0; k < 3; ++k)
-        {
-            d.off_f[k] = p.tot_f[k];
-            d.nface[k] = faces(d, k);
-            p.tot_f[k] += d.nface[k];
-        }
-        d.inv_dx = 1.0 / d.dx;
-        p.tot_g += d.gnpts;
-        p.tot_i += d.inpts;
-        for (int i = 0; i < 3; ++i)
-            if (b->periodic[i] || b->lo[i] != b->dlo[i] ||
-                b->hi[i] != b->dhi[i])
-                p.all_physical = false;
-    }
+        if (!add_boxdev(boxes[n], p.hb[n], p.tot_g, p.tot_i, p.tot_f))
+            p.all_physical = false;
     if (2 * p.tot_g >= (1ll << kKeyBits))
         return GRHD_ERR_INVALID_ARGUMENT;
     const int seg = march_segment(p.tot_i);
@@ -469,10 +488,10 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
         if (is_seg_class(c))
             continue;
         const std::vector<int2> items = class_table(p.hb, c);
-        p.nblocks[c] = (int)items.size();
         const int rc = upload_table(items, &p.d_items[c]);
         if (rc)
             return rc;
+        p.nblocks[c] = (int)items.size();
     }
     {
         const int rc = build_seg_tables(p, p.seg_rule);
@@ -526,7 +545,6 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
                     }
         }
     }
-    p.nexchange = (long long)xl.size();
     if (!xl.empty())
     {
         if (hipMalloc((void **)&p.d_exchange, sizeof(int4) * xl.size()) !=
@@ -535,6 +553,7 @@ int build_plan(Plan &p, const grhd_level_box *boxes, int nbox, int ncomp)
                       hipMemcpyHostToDevice) != hipSuccess)
             return GRHD_ERR_DEVICE;
     }
+    p.nexchange = (long long)xl.size();
     return GRHD_OK;
 }
 
@@ -651,7 +670,12 @@ int prepare(grhd_ctx *c, const grhd_level_box *boxes, int nbox, int ncomp)
         return GRHD_OK;
     const int rc = build_plan(c->plan, boxes, nbox, ncomp);
     if (rc)
+    {
+        /* no half-built plan survives: the next call with the same boxes
+         * builds it again (and fails the same way) */
+        free_plan(c->plan);
         return rc;
+    }
     /* a level tuned before keeps what was measured on it */
     Plan &pl = c->plan;
     for (const TuneRecord &t : c->tuned_levels)
@@ -661,6 +685,7 @@ int prepare(grhd_ctx *c, const grhd_level_box *boxes, int nbox, int ncomp)
             for (int q = 0; q < 2; ++q)
             {
                 pl.tuned[q] = t.tuned[q];
+                pl.tune_tried[q] = t.tuned[q];
                 pl.seg_pick[q] = t.seg_pick[q];
                 pl.fused_pick[q] = t.fused_pick[q];
                 pl.tuned_ms[q] = t.tuned_ms[q];
@@ -861,7 +886,7 @@ void tune_faces(grhd_ctx *c, const Options &o, const Background &bg,
 {
     Plan &pl = c->plan;
     const int prec = sizeof(T) < sizeof(double) ? 1 : 0;
-    pl.tuned[prec] = true;
+    pl.tune_tried[prec] = true;
     int best_k = pl.seg_rule;
     bool best_f = pl.tot_i < kFusedFacesCells;
     float best = 0.0f;
@@ -899,9 +924,12 @@ void tune_faces(grhd_ctx *c, const Options &o, const Background &bg,
             }
         }
     }
+    if (!have)
+        return;
     pl.seg_pick[prec] = best_k;
     pl.fused_pick[prec] = best_f;
-    pl.tuned_ms[prec] = have ? best / kTuneReps : 0.0f;
+    pl.tuned_ms[prec] = best / kTuneReps;
+    pl.tuned[prec] = true;
     remember_tuning(c);
 }
 
@@ -917,7 +945,7 @@ void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
 {
     Plan &pl = c->plan;
     const int prec = sizeof(T) < sizeof(double) ? 1 : 0;
-    if (c->autotune && !pl.tuned[prec])
+    if (c->autotune && !pl.tune_tried[prec])
         tune_faces<T>(c, o, bg, p, s);
     bool fused = pl.tot_i < kFusedFacesCells;
     int k = pl.seg_rule;
@@ -1768,15 +1796,6 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     return GRHD_OK;
 }
 
-inline grhd_level_box one_box(const grhd_box *b)
-{
-    grhd_level_box lb;
-    std::memset(&lb, 0, sizeof(lb));
-    if (b)
-        lb.box = *b;
-    lb.state_offset = 0;
-    return lb;
-}
 
 } // namespace
 

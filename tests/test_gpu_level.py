@@ -589,3 +589,29 @@ def test_tuning_is_remembered_per_level(hip_context):
         assert all(f["tuned"] and f["ms"] > 0.0 for f in first)
     finally:
         ctx.close()
+
+
+def test_failed_plan_is_not_reused(hip_context):
+    """A level whose plan cannot be built (here: more storage cells than
+    the failure key can number, refused before any allocation or launch)
+    fails the same way on the next call with the same boxes; a half-built
+    plan is never taken for a good one."""
+    import ctypes
+    from grdzhadzha_amd import abi
+    from grdzhadzha_amd.options import BoxSpec, KerrSchildParams, \
+        OperatorOptions
+    big = BoxSpec((0, 0, 0), (16383, 16383, 16383), 3, 0.5, (0.0,) * 3)
+    lb = (abi.GrhdLevelBox * 1)()
+    lb[0].box = big.to_c()
+    lb[0].state_offset = 0
+    params = OperatorOptions().to_c()
+    bg = KerrSchildParams(1.0, 0.0).to_c()
+    dummy = torch.zeros(16, dtype=torch.float64, device="cuda")
+    diag = abi.GrhdDiagnostics()
+    for _ in range(2):
+        rc = hip_context.lib.grhd_level_recover_primitives(
+            hip_context.handle, lb, 1, ctypes.byref(params),
+            ctypes.byref(bg), ctypes.c_double(0.0),
+            ctypes.c_void_p(dummy.data_ptr()), 11, 0, None,
+            ctypes.byref(diag), None)
+        assert rc == abi.GRHD_ERR_INVALID_ARGUMENT
