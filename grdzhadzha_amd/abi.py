@@ -30,6 +30,8 @@ GRHD_MASK_FLOORED = 2
 GRHD_MASK_RESET = 4
 GRHD_MASK_SOURCE_ATMOSPHERE = 8
 
+GRHD_NOALLOC_SCRATCH_BYTES = 128 << 20
+
 _I3 = ctypes.c_int * 3
 _D3 = ctypes.c_double * 3
 
@@ -122,6 +124,11 @@ SIGNATURES = {
                                   _P(GrhdBackground), _d, _d, _i, _vp, _i, _i,
                                   _vp, _P(GrhdDiagnostics), _P(GrhdError),
                                   _vp]),
+    "grhd_advance_rk4_box_noalloc": (_i, [_P(GrhdBox), _P(GrhdParams),
+                                          _P(GrhdBackground), _d, _d, _i,
+                                          _vp, _i, _i, _vp,
+                                          _P(GrhdDiagnostics), _P(GrhdError),
+                                          _vp]),
     "grhd_cell_geometry_box": (_i, [_vp, _P(GrhdBox), _P(GrhdBackground), _d,
                                     _vp, _vp]),
     "grhd_level_compute_rhs": (_i, [_vp, _P(GrhdLevelBox), _i,

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/grhd_abi.h"
@@ -19,6 +20,13 @@
 #include "grhd_adjoint.hpp"
 
 using namespace grhd;
+
+/* Scratch of grhd_advance_rk4_box_noalloc: one array per GPU, held from the
+ * library's load until the process ends.  Host code never uses its name
+ * as an address (on the host it is only a stand-in); it uses what
+ * hipGetSymbolAddress returns for the current device. */
+static __device__ __align__(256) unsigned char
+    scratch[GRHD_NOALLOC_SCRATCH_BYTES];
 
 namespace
 {
@@ -1441,6 +1449,191 @@ int level_rk4(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
     return finish(c, s, err, diag, false);
 }
 
+/* ---------------- RK4 of one box without allocation ---------------- */
+
+constexpr int kMaxScratchDevices = 256;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxScratchDevices] = {};
+
+/* The device address of `scratch` on the current device, looked up on
+ * the first call on that device and kept. */
+int scratch_on_current_device(unsigned char **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 ||
+        dev >= kMaxScratchDevices)
+        return GRHD_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev])
+    {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess ||
+            !ptr)
+            return GRHD_ERR_DEVICE;
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return GRHD_OK;
+}
+
+/* Where each buffer of one box's RK4 sits in the scratch array (byte
+ * offsets, 256-aligned). */
+struct NoAllocLayout
+{
+    size_t status, box, items[NCLASS], W, wbad, F[3], UA, UB, acc, total;
+    int nblocks[NCLASS];
+};
+
+NoAllocLayout noalloc_layout(const BoxDev &b)
+{
+    NoAllocLayout L;
+    size_t at = 0;
+    auto take = [&at](size_t bytes) {
+        const size_t o = at;
+        at += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    L.status = take(sizeof(DevStatus));
+    L.box = take(sizeof(BoxDev));
+    for (int c = 0; c < NCLASS; ++c)
+    {
+        if (c == C_FALL)
+            L.nblocks[c] =
+                L.nblocks[C_FX] + L.nblocks[C_FY] + L.nblocks[C_FZ];
+        else
+            L.nblocks[c] = (int)((class_items(b, c) + kBlock - 1) / kBlock);
+        L.items[c] = take(sizeof(int2) * (size_t)L.nblocks[c]);
+    }
+    const size_t G = (size_t)b.gnpts, I = (size_t)b.inpts;
+    L.W = take(sizeof(double) * 6 * G);
+    L.wbad = take(G);
+    for (int d = 0; d < 3; ++d)
+        L.F[d] = take(sizeof(double) * 5 * (size_t)b.nface[d]);
+    L.UA = take(sizeof(double) * 5 * G);
+    L.UB = take(sizeof(double) * 5 * G);
+    L.acc = take(sizeof(double) * 5 * I);
+    L.total = at;
+    return L;
+}
+
+struct NoAllocTables
+{
+    BoxDev *box;
+    int2 *items[NCLASS];
+    int nblocks[NCLASS];
+};
+
+/* The box table and block tables of one box, written on the device (block
+ * q of a class covers items 256 q .. 256 q + 255 of box 0; the fused face
+ * table is the x, y and z tables one after another). */
+__global__ void __launch_bounds__(kBlock)
+    k_noalloc_tables(BoxDev b, NoAllocTables t)
+{
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q == 0)
+        *t.box = b;
+    for (int c = 0; c < NCLASS; ++c)
+    {
+        if (q >= t.nblocks[c])
+            continue;
+        int first = q;
+        if (c == C_FALL)
+        {
+            const int nx = t.nblocks[C_FX], ny = t.nblocks[C_FY];
+            first = q < nx ? q : (q < nx + ny ? q - nx : q - nx - ny);
+        }
+        t.items[c][q] = make_int2(0, first * kBlock);
+    }
+}
+
+/* grhd_advance_rk4_box with every scratch buffer in `scratch`: a context
+ * on the host stack whose plan and pools point into the array, driven by
+ * the same launch sequence.  Nothing here allocates device memory: the
+ * context's own buffers are never grown (their capacities are set to what
+ * the box needs), the tables are written by a kernel, the status block is
+ * cleared by a kernel and read back into host stack memory. */
+int rk4_box_noalloc(const grhd_box *box, const grhd_params *params,
+                    const grhd_background *bg, double time, double dt,
+                    int nsteps, double *state, int ncomp, int cb,
+                    unsigned char *mask, grhd_diagnostics *diag,
+                    grhd_error *err, void *st)
+{
+    if (!box || !state || nsteps < 0)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    int rc = check_box(box);
+    if (rc || (rc = check_common(params, bg, ncomp, cb)))
+        return rc;
+    const grhd_level_box lb = one_box(box);
+    BoxDev b;
+    std::memset(&b, 0, sizeof(b));
+    long long tot_g = 0, tot_i = 0, tot_f[3] = {0, 0, 0};
+    const bool physical = add_boxdev(lb, b, tot_g, tot_i, tot_f);
+    /* as grhd_advance_rk4_box: every ghost an outflow ghost */
+    if (!physical)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    b.seg = march_segment(tot_i);
+    const NoAllocLayout L = noalloc_layout(b);
+    if (L.total > (size_t)GRHD_NOALLOC_SCRATCH_BYTES ||
+        tot_i >= kFusedFacesCells)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    unsigned char *base = nullptr;
+    if ((rc = scratch_on_current_device(&base)))
+        return rc;
+    hipStream_t s = (hipStream_t)st;
+
+    NoAllocTables t;
+    t.box = reinterpret_cast<BoxDev *>(base + L.box);
+    int most = 0;
+    for (int c = 0; c < NCLASS; ++c)
+    {
+        t.items[c] = reinterpret_cast<int2 *>(base + L.items[c]);
+        t.nblocks[c] = L.nblocks[c];
+        most = std::max(most, L.nblocks[c]);
+    }
+    hipLaunchKernelGGL(k_noalloc_tables, dim3(grid_of(std::max(most, 1))),
+                       dim3(kBlock), 0, s, b, t);
+
+    grhd_ctx c;
+    Plan &pl = c.plan;
+    pl.key.assign(&lb, &lb + 1);
+    pl.ncomp = ncomp;
+    pl.hb.assign(&b, &b + 1);
+    pl.tot_g = tot_g;
+    pl.tot_i = tot_i;
+    for (int d = 0; d < 3; ++d)
+        pl.tot_f[d] = tot_f[d];
+    pl.all_physical = true;
+    pl.d_boxes = t.box;
+    for (int q = 0; q < NCLASS; ++q)
+    {
+        pl.d_items[q] = t.items[q];
+        pl.nblocks[q] = t.nblocks[q];
+    }
+    for (int k = 0; k < kNumSegs; ++k)
+        if (kSegs[k] == b.seg)
+            pl.seg_rule = k;
+    pl.seg_cur = pl.seg_rule; /* the face tables above are for b.seg */
+    c.W = reinterpret_cast<double *>(base + L.W);
+    c.wbad = base + L.wbad;
+    c.cap_g = tot_g;
+    for (int d = 0; d < 3; ++d)
+    {
+        c.F[d] = reinterpret_cast<double *>(base + L.F[d]);
+        c.cap_f[d] = 5 * tot_f[d];
+    }
+    c.UA = reinterpret_cast<double *>(base + L.UA);
+    c.UB = reinterpret_cast<double *>(base + L.UB);
+    c.cap_u = 5 * tot_g;
+    c.acc = reinterpret_cast<double *>(base + L.acc);
+    c.cap_i = 5 * tot_i;
+    c.dstat = reinterpret_cast<DevStatus *>(base + L.status);
+    DevStatus host_status;
+    c.hstat = &host_status;
+    /* c owns no device memory: it goes out of scope without a free */
+    return level_rk4(&c, &lb, 1, params, bg, time, dt, nsteps, state, ncomp,
+                     cb, mask, false, diag, err, st);
+}
+
 /* ---------------- discrete adjoint of RK4 ---------------- */
 
 /* Device buffers of one adjoint call, freed on every return path. */
@@ -1981,6 +2174,19 @@ int grhd_advance_rk4_box(grhd_ctx *ctx, const grhd_box *box,
     const grhd_level_box lb = one_box(box);
     return level_rk4(ctx, &lb, 1, params, background, time, dt, nsteps, state,
                      ncomp, comp_base, mask, false, diagnostics, err, stream);
+}
+
+int grhd_advance_rk4_box_noalloc(const grhd_box *box,
+                                 const grhd_params *params,
+                                 const grhd_background *background,
+                                 double time, double dt, int nsteps,
+                                 double *state, int ncomp, int comp_base,
+                                 unsigned char *mask,
+                                 grhd_diagnostics *diagnostics,
+                                 grhd_error *err, void *stream)
+{
+    return rk4_box_noalloc(box, params, background, time, dt, nsteps, state,
+                           ncomp, comp_base, mask, diagnostics, err, stream);
 }
 
 int grhd_cell_geometry_box(grhd_ctx *ctx, const grhd_box *box,

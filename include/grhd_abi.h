@@ -283,6 +283,35 @@ int grhd_advance_rk4_box(grhd_ctx *ctx, const grhd_box *box,
                          grhd_diagnostics *diagnostics, grhd_error *err,
                          void *stream);
 
+/* Bytes of device scratch grhd_advance_rk4_box_noalloc may use per GPU:
+ * enough for a box of up to 72^3 interior cells with 3 ghost layers
+ * (64^3 needs 83 MiB). */
+#define GRHD_NOALLOC_SCRATCH_BYTES (128ull << 20)
+
+/* grhd_advance_rk4_box without a context and without allocating: no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Arguments, results and errors are those
+ * of grhd_advance_rk4_box (bit for bit the same state, mask and counts).
+ * Every scratch buffer (primitives, face fluxes, stage states, the RK
+ * accumulator, the status block and the launch tables) lives in one static
+ * device array of GRHD_NOALLOC_SCRATCH_BYTES, which each GPU the library
+ * runs on holds until the process ends; the call makes no hipMalloc,
+ * hipFree or other allocating call.  A box whose scratch would not fit
+ * returns GRHD_ERR_INVALID_ARGUMENT, as does a box that is not its own
+ * non-periodic ProblemDomain.  All calls on one GPU share the array and it
+ * keeps the last call's contents: calls that overlap on one GPU (from two
+ * streams or two host threads) race and must be serialised by the
+ * caller. */
+int grhd_advance_rk4_box_noalloc(const grhd_box *box,
+                                 const grhd_params *params,
+                                 const grhd_background *background,
+                                 double time, double dt, int nsteps,
+                                 double *state, int ncomp, int comp_base,
+                                 unsigned char *mask,
+                                 grhd_diagnostics *diagnostics,
+                                 grhd_error *err, void *stream);
+
 /* Analytic cell geometry over the FAB storage box, for checking the device
  * metric: out holds 16 components [g11 g12 g13 g22 g23 g33, gu11 gu12 gu13
  * gu22 gu23 gu33, lapse, beta1 beta2 beta3] in the FAB layout of the
