@@ -39,7 +39,8 @@ _D3 = ctypes.c_double * 3
 class GrhdBox(ctypes.Structure):
     _fields_ = [("lo", _I3), ("hi", _I3), ("glo", _I3), ("ghi", _I3),
                 ("dlo", _I3), ("dhi", _I3), ("periodic", _I3),
-                ("dx", ctypes.c_double), ("center", _D3)]
+                ("dx", ctypes.c_double), ("center", _D3),
+                ("inverse_dx", _D3)]
 
 
 class GrhdParams(ctypes.Structure):
@@ -135,6 +136,10 @@ SIGNATURES = {
                                     _P(GrhdParams), _P(GrhdBackground), _d,
                                     _vp, _i, _i, _vp, _vp, _P(GrhdError),
                                     _vp]),
+    "grhd_level_add_conserved_rhs": (_i, [_vp, _P(GrhdLevelBox), _i,
+                                          _P(GrhdParams), _P(GrhdBackground),
+                                          _d, _vp, _i, _i, _vp,
+                                          _P(GrhdError), _vp]),
     "grhd_level_recover_primitives": (_i, [_vp, _P(GrhdLevelBox), _i,
                                            _P(GrhdParams), _P(GrhdBackground),
                                            _d, _vp, _i, _i, _vp,

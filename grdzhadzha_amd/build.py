@@ -35,6 +35,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST_PROGRAMS = {
     os.path.join("tests", "cpp", "_build", "level_driver"):
         [os.path.join("tests", "cpp", "level_driver.cpp")],
+    os.path.join("tests", "cpp", "_build", "hooks_driver"):
+        [os.path.join("tests", "cpp", "hooks_driver.cpp")],
 }
 
 

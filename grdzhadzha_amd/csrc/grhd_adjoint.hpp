@@ -959,7 +959,7 @@ __global__ void __launch_bounds__(kBlock)
         {
             const double lr = Rin ? lam_k[m * ni + ir] : 0.0;
             const double ll = Lin ? lam_k[m * ni + il] : 0.0;
-            lam[m] = (lr - ll) * b.inv_dx;
+            lam[m] = (lr - ll) * b.inv_dx[D];
             any = any || lam[m] != 0.0;
         }
         if (any)

@@ -103,6 +103,12 @@ int check_box(const grhd_box *b)
 {
     if (!(b->dx > 0.0))
         return GRHD_ERR_INVALID_ARGUMENT;
+    /* inverse_dx: all zero (1/dx), or positive and finite */
+    if (b->inverse_dx[0] != 0.0 || b->inverse_dx[1] != 0.0 ||
+        b->inverse_dx[2] != 0.0)
+        for (int d = 0; d < 3; ++d)
+            if (!(b->inverse_dx[d] > 0.0) || !std::isfinite(b->inverse_dx[d]))
+                return GRHD_ERR_INVALID_ARGUMENT;
     for (int d = 0; d < 3; ++d)
     {
         if (b->lo[d] > b->hi[d])
@@ -465,7 +471,11 @@ bool add_boxdev(const grhd_level_box &lb, BoxDev &d, long long &tot_g,
         d.nface[k] = faces(d, k);
         tot_f[k] += d.nface[k];
     }
-    d.inv_dx = 1.0 / d.dx;
+    const bool default_inv = b->inverse_dx[0] == 0.0 &&
+                             b->inverse_dx[1] == 0.0 &&
+                             b->inverse_dx[2] == 0.0;
+    for (int k = 0; k < 3; ++k)
+        d.inv_dx[k] = default_inv ? 1.0 / d.dx : b->inverse_dx[k];
     tot_g += d.gnpts;
     tot_i += d.inpts;
     return physical;
@@ -1152,7 +1162,8 @@ int finish(grhd_ctx *c, hipStream_t s, grhd_error *err,
 int level_rhs(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
               const grhd_params *params, const grhd_background *bg,
               double time, double *state, int ncomp, int cb, double *rhs,
-              unsigned char *mask, bool exchange, grhd_error *err, void *st)
+              unsigned char *mask, bool exchange, grhd_error *err, void *st,
+              bool accumulate = false)
 {
     if (!state || !rhs)
         return GRHD_ERR_INVALID_ARGUMENT;
@@ -1168,12 +1179,16 @@ int level_rhs(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
     p.rhs_ncomp = ncomp;
     p.mask = mask;
     launch_status_init(c, s);
-    GRHD_LAUNCH(c, C_INT, k_zero_rhs, s, p);
-    tick(c, GRHD_KERNEL_OTHER, s);
+    if (!accumulate)
+    {
+        GRHD_LAUNCH(c, C_INT, k_zero_rhs, s, p);
+        tick(c, GRHD_KERNEL_OTHER, s);
+    }
     CellArgs a;
     std::memset(&a, 0, sizeof(a));
     a.us = POOL_STATE;
     a.write_mask = mask != nullptr;
+    a.accumulate = accumulate ? 1 : 0;
     rhs_sequence(c, MODE_RHS, params, bg, time, time, p, a, false, true,
                  exchange, ncomp, 0, s);
     return finish(c, s, err, nullptr, false);
@@ -2218,6 +2233,18 @@ int grhd_level_compute_rhs(grhd_ctx *ctx, const grhd_level_box *boxes,
     return level_rhs(ctx, boxes, nbox, params, background, time, state_pool,
                      ncomp, comp_base, rhs_pool, source_mask, true, err,
                      stream);
+}
+
+int grhd_level_add_conserved_rhs(grhd_ctx *ctx, const grhd_level_box *boxes,
+                                 int nbox, const grhd_params *params,
+                                 const grhd_background *background,
+                                 double time, double *state_pool, int ncomp,
+                                 int comp_base, double *rhs_pool,
+                                 grhd_error *err, void *stream)
+{
+    return level_rhs(ctx, boxes, nbox, params, background, time, state_pool,
+                     ncomp, comp_base, rhs_pool, nullptr, true, err, stream,
+                     true);
 }
 
 int grhd_level_recover_primitives(grhd_ctx *ctx, const grhd_level_box *boxes,

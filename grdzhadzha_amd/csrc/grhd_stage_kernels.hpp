@@ -97,7 +97,8 @@ struct BoxDev
     long long off_i;    /* interior cells of the boxes before it */
     long long off_f[3]; /* faces of direction d of the boxes before it */
     long long nface[3]; /* faces of direction d of this box */
-    double inv_dx;      /* 1 / dx, rounded once on the host */
+    double inv_dx[3];   /* inverse_dx_U: 1 / dx (rounded once on the
+                           host) unless the caller gives its own */
     int seg;            /* faces per walker of the marching y/z kernels */
 };
 
@@ -820,6 +821,7 @@ struct CellArgs
     int unext;       /* pool receiving the stage result */
     int store_prims; /* write floored primitives into the state FAB */
     int write_mask;
+    int accumulate;  /* MODE_RHS: add the conserved rhs to the rhs FAB */
     int count;       /* add recovery counts */
     double c_stage;  /* U0 coefficient of k */
     double c_acc;    /* accumulator coefficient of k */
@@ -857,10 +859,10 @@ __global__ void __launch_bounds__(kBlock)
         rhs.D = 0.0;
         rhs.tau = 0.0;
         rhs.S[0] = rhs.S[1] = rhs.S[2] = 0.0;
-        const double mi = -b.inv_dx;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
         {
+            const double mi = -b.inv_dx[d];
             const FT *F = face_pool<const FT>(p, b, d);
             const long long nf = b.nface[d];
             const int fn0 = b.in[0] + (d == 0), fn1 = b.in[1] + (d == 1);
@@ -938,11 +940,23 @@ __global__ void __launch_bounds__(kBlock)
                 const long long ni = b.inpts;
                 double *r =
                     p.rhs + p.rhs_ncomp * b.off_i + (long long)p.cb * ni;
-                r[it] = rhs.D;
-                r[ni + it] = rhs.S[0];
-                r[2 * ni + it] = rhs.S[1];
-                r[3 * ni + it] = rhs.S[2];
-                r[4 * ni + it] = rhs.tau;
+                if (a.accumulate)
+                {
+                    /* add_conserved_rhs (FArrayBoxFiniteVolume.hpp:912-925) */
+                    r[it] += rhs.D;
+                    r[ni + it] += rhs.S[0];
+                    r[2 * ni + it] += rhs.S[1];
+                    r[3 * ni + it] += rhs.S[2];
+                    r[4 * ni + it] += rhs.tau;
+                }
+                else
+                {
+                    r[it] = rhs.D;
+                    r[ni + it] = rhs.S[0];
+                    r[2 * ni + it] = rhs.S[1];
+                    r[3 * ni + it] = rhs.S[2];
+                    r[4 * ni + it] = rhs.tau;
+                }
                 if (a.write_mask)
                     p.mask[b.off_i + it] = mbits;
             }
@@ -1198,7 +1212,6 @@ __global__ void __launch_bounds__(kBlock)
             floors(pr, o, g.g);
             if (!is_atmosphere(pr, o, 1.0 + 1.0e-12))
             {
-                const double inv_dx = b.inv_dx;
                 double cell = 0.0;
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
@@ -1206,7 +1219,7 @@ __global__ void __launch_bounds__(kBlock)
                     double lo, hi;
                     char_speeds(pr, o.gamma, g, d, lo, hi);
                     const double ms = std_max(fabs(lo), fabs(hi));
-                    cell += ms * inv_dx;
+                    cell += ms * b.inv_dx[d];
                 }
                 best = std_max(best, cell);
             }

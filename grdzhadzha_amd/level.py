@@ -111,6 +111,18 @@ class LevelOperator:
             _stream(stream))
         _raise(rc, err, "level compute_rhs")
 
+    def add_conserved_rhs_to(self, rhs, pool, time=0.0, stream=None):
+        """add_conserved_rhs_to (FixedBGFArrayBoxFiniteVolume.hpp:484-496):
+        rhs[D, S_i, tau] += this level's rhs; other components untouched."""
+        self._check(pool)
+        if rhs.numel() != self.ncomp * self.num_interior:
+            raise ValueError("rhs pool must hold ncomp x interior cells")
+        err = abi.GrhdError()
+        rc = self.lib.grhd_level_add_conserved_rhs(
+            *self._head(time), _ptr(pool), self.ncomp, self.comp_base,
+            _ptr(rhs), ctypes.byref(err), _stream(stream))
+        _raise(rc, err, "level add_conserved_rhs_to")
+
     def recover_primitives(self, pool, time=0.0, mask=None, stream=None):
         self._check(pool)
         diag = abi.GrhdDiagnostics()

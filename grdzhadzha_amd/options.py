@@ -124,6 +124,9 @@ class BoxSpec:
     domain_lo: Tuple[int, int, int] = None
     domain_hi: Tuple[int, int, int] = None
     periodic: Tuple[int, int, int] = (0, 0, 0)
+    # inverse_dx_U of the reference's operator overloads: the divergence
+    # and CFL scale of each direction (None: 1/dx in every direction)
+    inverse_dx: Tuple[float, float, float] = None
 
     def __post_init__(self):
         self.lo = tuple(int(v) for v in self.lo)
@@ -172,6 +175,8 @@ class BoxSpec:
             b.dhi[d] = self.domain_hi[d]
             b.periodic[d] = int(self.periodic[d])
             b.center[d] = self.center[d]
+            b.inverse_dx[d] = (0.0 if self.inverse_dx is None
+                               else float(self.inverse_dx[d]))
         b.dx = self.dx
         return b
 

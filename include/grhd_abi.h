@@ -100,6 +100,14 @@ typedef struct grhd_box
     int periodic[3];    /* ProblemDomain::isPeriodic(dir)       */
     double dx;
     double center[3];
+    /* inverse_dx_U of the reference's operator overloads
+     * (FixedBGFArrayBoxFiniteVolume.hpp:468-484, :535-589, :702-809):
+     * the flux divergence of direction d is scaled by inverse_dx[d]
+     * (:243) and the CFL speed sums max|lambda_d| inverse_dx[d] (:407);
+     * coordinates still use dx.  All three zero means 1/dx in every
+     * direction (the overloads without inverse_dx_U, make_inverse_dx);
+     * otherwise each must be positive and finite. */
+    double inverse_dx[3];
 } grhd_box;
 
 /* IdealGasEOS + AtmosphereOptions + RecoveryOptions + operator flags
@@ -353,6 +361,20 @@ int grhd_level_compute_rhs(grhd_ctx *ctx, const grhd_level_box *boxes,
                            double *state_pool, int ncomp, int comp_base,
                            double *rhs_pool, unsigned char *source_mask,
                            grhd_error *err, void *stream);
+
+/* add_conserved_rhs_to (FixedBGFArrayBoxFiniteVolume.hpp:484-496): the
+ * level rhs of grhd_level_compute_rhs (exchange, ghost fill -- which
+ * mutates the state as there -- fluxes and sources) added to the D, S_i
+ * and tau components of rhs_pool, rhs += hydro_rhs cell by cell
+ * (add_conserved_rhs, FArrayBoxFiniteVolume.hpp:912-925); the other
+ * components of rhs_pool are left as they are.  For a caller (a CCZ4 or
+ * matter rhs) that sums several right-hand sides into one. */
+int grhd_level_add_conserved_rhs(grhd_ctx *ctx, const grhd_level_box *boxes,
+                                 int nbox, const grhd_params *params,
+                                 const grhd_background *background,
+                                 double time, double *state_pool, int ncomp,
+                                 int comp_base, double *rhs_pool,
+                                 grhd_error *err, void *stream);
 
 /* recover_primitives over the level (FixedBGFArrayBoxFiniteVolume.hpp:506-520,
  * :728-743); diagnostics are level totals, as after the reference's

@@ -14,8 +14,19 @@
  * (component-major, i fastest) back to back.  Every method is one launch
  * per kernel for the whole level; the pools stay on the device.
  *
- * Header-only; link against libgrhd_hip.so.  Uniform dx only (the
- * reference's inverse_dx_U overloads are 1/dx in every direction).
+ * The overloads taking inverse_dx_U accept any indexable type (GRChombo's
+ * Tensor<1, double>, std::array<double, 3>, double[3]); those without it
+ * use 1/dx in every direction, as the reference's make_inverse_dx does.
+ * A Box argument is an IntBox; a ProblemDomain built from it is
+ * non-periodic.
+ *
+ * FixedBGAMRLevelHooks and TimeDependentFixedBGAMRLevelHooks
+ * (FixedBGAMRLevelHooks.hpp:22-312) and the driver views
+ * FixedBGLevelDataDriverView / TimeDependentFixedBGLevelDataDriverView
+ * (FixedBGLevelDataDriver.hpp:17-141, :229-395) delegate to the operators
+ * exactly as the reference's do.
+ *
+ * Header-only; link against libgrhd_hip.so.
  */
 #ifndef GRHD_OPERATOR_HPP_
 #define GRHD_OPERATOR_HPP_
@@ -28,6 +39,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace GRHD
@@ -150,17 +163,62 @@ struct ProblemDomain
         : box(b), periodic(p)
     {
     }
+    const IntBox &domainBox() const { return box; }
 };
 
+/* inverse_dx_U: three per-direction inverse spacings; all zero stands
+ * for 1/dx in every direction. */
+using InverseDx = std::array<double, 3>;
+
+/* Anything indexable by 0..2 into doubles (Tensor<1, double>, arrays). */
+template <class V, class = void> struct is_inverse_dx : std::false_type
+{
+};
+template <class V>
+struct is_inverse_dx<
+    V, typename std::enable_if<
+           !std::is_arithmetic<V>::value &&
+           std::is_convertible<decltype(std::declval<const V &>()[0]),
+                               double>::value>::type> : std::true_type
+{
+};
+
+template <class V> InverseDx to_inverse_dx(const V &v)
+{
+    return InverseDx{{double(v[0]), double(v[1]), double(v[2])}};
+}
+
+/* The bounding box of a box list (the domain of a layout given none). */
+inline IntBox bounding_box(const std::vector<IntBox> &grids)
+{
+    IntBox b = grids.empty() ? IntBox() : grids[0];
+    for (const IntBox &g : grids)
+        for (int d = 0; d < 3; ++d)
+        {
+            b.lo[d] = std::min(b.lo[d], g.lo[d]);
+            b.hi[d] = std::max(b.hi[d], g.hi[d]);
+        }
+    return b;
+}
+
 /* LevelData<FArrayBox> in HBM.  `pool` is a device pointer to
- * pool_size() doubles (the caller owns it). */
+ * pool_size() doubles (the caller owns it).  The layout's ProblemDomain
+ * (DisjointBoxLayout::physDomain) is given, or the grids' bounding box,
+ * non-periodic. */
 class DeviceLevelData
 {
   public:
     DeviceLevelData(std::vector<IntBox> grids, int ghosts, int ncomp,
                     double *pool)
-        : m_grids(std::move(grids)), m_ghosts(ghosts), m_ncomp(ncomp),
-          m_pool(pool)
+        : DeviceLevelData(grids, ProblemDomain(bounding_box(grids)), ghosts,
+                          ncomp, pool)
+    {
+    }
+
+    DeviceLevelData(std::vector<IntBox> grids, const ProblemDomain &domain,
+                    int ghosts, int ncomp, double *pool)
+        : m_grids(std::move(grids)), m_domain(domain), m_ghosts(ghosts),
+          m_ncomp(ncomp), m_pool(pool)
     {
         if (ghosts < 0 || ncomp <= 0 || m_grids.empty())
             throw std::invalid_argument("DeviceLevelData: bad layout");
@@ -174,6 +232,7 @@ class DeviceLevelData
     }
 
     const std::vector<IntBox> &grids() const { return m_grids; }
+    const ProblemDomain &problem_domain() const { return m_domain; }
     int ghosts() const { return m_ghosts; }
     int ncomp() const { return m_ncomp; }
     double *data() const { return m_pool; }
@@ -188,10 +247,10 @@ class DeviceLevelData
     }
 
     /* The C-ABI's box table for this layout. */
-    std::vector<grhd_level_box> level_boxes(const ProblemDomain &domain,
-                                            double dx,
-                                            const std::array<double, 3> &c)
-        const
+    std::vector<grhd_level_box> level_boxes(
+        const ProblemDomain &domain, double dx,
+        const std::array<double, 3> &c,
+        const InverseDx &inverse_dx = InverseDx{{0.0, 0.0, 0.0}}) const
     {
         std::vector<grhd_level_box> out(m_grids.size());
         for (size_t n = 0; n < m_grids.size(); ++n)
@@ -208,6 +267,7 @@ class DeviceLevelData
                 b.dhi[d] = domain.box.hi[d];
                 b.periodic[d] = domain.periodic[d] ? 1 : 0;
                 b.center[d] = c[d];
+                b.inverse_dx[d] = inverse_dx[d];
             }
             b.dx = dx;
             out[n].state_offset = m_offsets[n];
@@ -217,6 +277,7 @@ class DeviceLevelData
 
   private:
     std::vector<IntBox> m_grids;
+    ProblemDomain m_domain;
     int m_ghosts;
     int m_ncomp;
     double *m_pool;
@@ -345,6 +406,10 @@ class LevelOperatorBase
 
     const grhd_background &background() const { return m_background; }
     grhd_ctx *context() const { return m_ctx->get(); }
+    void *stream() const { return m_stream; }
+    int comp_base() const { return m_comp_base; }
+    double dx() const { return m_dx; }
+    const std::array<double, 3> &center() const { return m_center; }
 
     /* FP32 reconstruction, face metric and HLLE (see grhd_params); off by
      * default, which is the bit-exact FP64 path. */
@@ -361,33 +426,45 @@ class LevelOperatorBase
     }
 
   protected:
+    static constexpr InverseDx uniform() { return InverseDx{{0.0, 0.0, 0.0}}; }
+
     std::vector<grhd_level_box> table(const DeviceLevelData &l,
-                                      const ProblemDomain &domain) const
+                                      const ProblemDomain &domain,
+                                      const InverseDx &inv) const
     {
-        return l.level_boxes(domain, m_dx, m_center);
+        return l.level_boxes(domain, m_dx, m_center, inv);
     }
 
     void rhs_at(DeviceLevelData &rhs, DeviceLevelData &state,
-                const ProblemDomain &domain, double time) const
+                const ProblemDomain &domain, const InverseDx &inv,
+                double time, bool accumulate = false) const
     {
-        const auto t = table(state, domain);
+        const auto t = table(state, domain, inv);
         if (rhs.ncomp() != state.ncomp() ||
             rhs.pool_size() != rhs.ncomp() * state.num_interior())
             throw std::invalid_argument(
                 "rhs must hold ncomp components over the interior boxes");
         grhd_error err{};
-        check(grhd_level_compute_rhs(context(), t.data(), (int)t.size(),
-                                     &m_params, &m_background, time,
-                                     state.data(), state.ncomp(), m_comp_base,
-                                     rhs.data(), nullptr, &err, m_stream),
-              &err, "compute_rhs");
+        if (accumulate)
+            check(grhd_level_add_conserved_rhs(
+                      context(), t.data(), (int)t.size(), &m_params,
+                      &m_background, time, state.data(), state.ncomp(),
+                      m_comp_base, rhs.data(), &err, m_stream),
+                  &err, "add_conserved_rhs_to");
+        else
+            check(grhd_level_compute_rhs(context(), t.data(), (int)t.size(),
+                                         &m_params, &m_background, time,
+                                         state.data(), state.ncomp(),
+                                         m_comp_base, rhs.data(), nullptr,
+                                         &err, m_stream),
+                  &err, "compute_rhs");
     }
 
     AtmosphereResetDiagnostics recover_at(DeviceLevelData &state,
                                           const ProblemDomain &domain,
                                           double time) const
     {
-        const auto t = table(state, domain);
+        const auto t = table(state, domain, uniform());
         grhd_diagnostics d{};
         check(grhd_level_recover_primitives(
                   context(), t.data(), (int)t.size(), &m_params,
@@ -401,7 +478,7 @@ class LevelOperatorBase
                    const ProblemDomain &domain, double dt, bool recover,
                    double recovery_time) const
     {
-        const auto t = table(state, domain);
+        const auto t = table(state, domain, uniform());
         grhd_diagnostics d{};
         check(grhd_level_update_conserved(
                   context(), t.data(), (int)t.size(), &m_params,
@@ -411,10 +488,13 @@ class LevelOperatorBase
               nullptr, "update_conserved");
     }
 
+    /* The domain box does not enter the CFL estimate (the reference's level
+     * version ignores it, FixedBGFArrayBoxFiniteVolume.hpp:423); the
+     * state's own ProblemDomain keys the launch plan. */
     double max_inverse_dt_at(const DeviceLevelData &state,
-                             const ProblemDomain &domain, double time) const
+                             const InverseDx &inv, double time) const
     {
-        const auto t = table(state, domain);
+        const auto t = table(state, state.problem_domain(), inv);
         double out = 0.0;
         grhd_error err{};
         check(grhd_level_compute_max_inverse_dt(
@@ -425,10 +505,19 @@ class LevelOperatorBase
         return out;
     }
 
-    void ssprk2_at(DeviceLevelData &state, const ProblemDomain &domain,
-                   double dt, double time) const
+    /* compute_stable_dt (:544-555, :769-780) */
+    double stable_dt_at(const DeviceLevelData &state, const InverseDx &inv,
+                        double cfl, double time) const
     {
-        const auto t = table(state, domain);
+        if (cfl <= 0.0)
+            throw std::invalid_argument("GRHD CFL number must be positive");
+        return cfl / std::max(max_inverse_dt_at(state, inv, time), 1.0e-14);
+    }
+
+    void ssprk2_at(DeviceLevelData &state, const ProblemDomain &domain,
+                   const InverseDx &inv, double dt, double time) const
+    {
+        const auto t = table(state, domain, inv);
         grhd_diagnostics d{};
         grhd_error err{};
         check(grhd_level_advance_ssprk2(context(), t.data(), (int)t.size(),
@@ -440,10 +529,11 @@ class LevelOperatorBase
     }
 
     AtmosphereResetDiagnostics rk4_at(DeviceLevelData &state,
-                                      const ProblemDomain &domain, double dt,
+                                      const ProblemDomain &domain,
+                                      const InverseDx &inv, double dt,
                                       int nsteps, double time) const
     {
-        const auto t = table(state, domain);
+        const auto t = table(state, domain, inv);
         grhd_diagnostics d{};
         grhd_error err{};
         check(grhd_level_advance_rk4(context(), t.data(), (int)t.size(),
@@ -463,6 +553,10 @@ class LevelOperatorBase
     std::shared_ptr<Context> m_ctx;
     void *m_stream;
 };
+
+#define GRHD_IF_INVERSE_DX(V)                                                 \
+    template <class V,                                                        \
+              typename std::enable_if<is_inverse_dx<V>::value, int>::type = 0>
 
 /* FixedBGLevelDataFiniteVolumeOperator
  * (FixedBGFArrayBoxFiniteVolume.hpp:446-614): a static background. */
@@ -485,10 +579,42 @@ class FixedBGLevelDataFiniteVolumeOperator : public LevelOperatorBase
     {
     }
 
+    /* :468-477 */
+    GRHD_IF_INVERSE_DX(V)
+    void compute_rhs(DeviceLevelData &rhs, DeviceLevelData &state,
+                     const ProblemDomain &domain, const V &inverse_dx_U) const
+    {
+        rhs_at(rhs, state, domain, to_inverse_dx(inverse_dx_U), 0.0);
+    }
+
+    /* :479-484 */
+    GRHD_IF_INVERSE_DX(V)
+    void compute_rhs(DeviceLevelData &rhs, DeviceLevelData &state,
+                     const IntBox &domain_box, const V &inverse_dx_U) const
+    {
+        compute_rhs(rhs, state, ProblemDomain(domain_box), inverse_dx_U);
+    }
+
+    /* :486-490 */
     void compute_rhs(DeviceLevelData &rhs, DeviceLevelData &state,
                      const ProblemDomain &domain) const
     {
-        rhs_at(rhs, state, domain, 0.0);
+        rhs_at(rhs, state, domain, uniform(), 0.0);
+    }
+
+    /* :492-503: rhs[D, S_i, tau] += the hydro rhs */
+    GRHD_IF_INVERSE_DX(V)
+    void add_conserved_rhs_to(DeviceLevelData &rhs, DeviceLevelData &state,
+                              const ProblemDomain &domain,
+                              const V &inverse_dx_U) const
+    {
+        rhs_at(rhs, state, domain, to_inverse_dx(inverse_dx_U), 0.0, true);
+    }
+
+    /* :506-520 */
+    AtmosphereResetDiagnostics recover_primitives(DeviceLevelData &state) const
+    {
+        return recover_at(state, state.problem_domain(), 0.0);
     }
 
     AtmosphereResetDiagnostics recover_primitives(DeviceLevelData &state,
@@ -496,6 +622,15 @@ class FixedBGLevelDataFiniteVolumeOperator : public LevelOperatorBase
         const
     {
         return recover_at(state, domain, 0.0);
+    }
+
+    /* :522-533 */
+    void update_conserved(DeviceLevelData &state, const DeviceLevelData &rhs,
+                          double dt,
+                          bool recover_primitives_after_update = true) const
+    {
+        update_at(state, rhs, state.problem_domain(), dt,
+                  recover_primitives_after_update, 0.0);
     }
 
     void update_conserved(DeviceLevelData &state, const DeviceLevelData &rhs,
@@ -506,24 +641,61 @@ class FixedBGLevelDataFiniteVolumeOperator : public LevelOperatorBase
                   0.0);
     }
 
+    /* :535-542 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_max_inverse_dt(const DeviceLevelData &state,
+                                  const IntBox &domain_box,
+                                  const V &inverse_dx_U) const
+    {
+        (void)domain_box;
+        return max_inverse_dt_at(state, to_inverse_dx(inverse_dx_U), 0.0);
+    }
+
     double compute_max_inverse_dt(const DeviceLevelData &state,
                                   const ProblemDomain &domain) const
     {
-        return max_inverse_dt_at(state, domain, 0.0);
+        (void)domain;
+        return max_inverse_dt_at(state, uniform(), 0.0);
+    }
+
+    /* :544-555 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const IntBox &domain_box, const V &inverse_dx_U,
+                             double cfl) const
+    {
+        (void)domain_box;
+        return stable_dt_at(state, to_inverse_dx(inverse_dx_U), cfl, 0.0);
+    }
+
+    /* :557-561 */
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const IntBox &domain_box, double cfl) const
+    {
+        (void)domain_box;
+        return stable_dt_at(state, uniform(), cfl, 0.0);
     }
 
     double compute_stable_dt(const DeviceLevelData &state,
                              const ProblemDomain &domain, double cfl) const
     {
-        if (cfl <= 0.0)
-            throw std::invalid_argument("GRHD CFL number must be positive");
-        return cfl / std::max(compute_max_inverse_dt(state, domain), 1.0e-14);
+        (void)domain;
+        return stable_dt_at(state, uniform(), cfl, 0.0);
     }
 
+    /* :563-589 */
+    GRHD_IF_INVERSE_DX(V)
+    void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
+                        const V &inverse_dx_U, double dt) const
+    {
+        ssprk2_at(state, domain, to_inverse_dx(inverse_dx_U), dt, 0.0);
+    }
+
+    /* :591-595 */
     void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
                         double dt) const
     {
-        ssprk2_at(state, domain, dt, 0.0);
+        ssprk2_at(state, domain, uniform(), dt, 0.0);
     }
 
     /* Classical RK4, `nsteps` steps (no counterpart in the reference; see
@@ -532,7 +704,17 @@ class FixedBGLevelDataFiniteVolumeOperator : public LevelOperatorBase
                                            const ProblemDomain &domain,
                                            double dt, int nsteps = 1) const
     {
-        return rk4_at(state, domain, dt, nsteps, 0.0);
+        return rk4_at(state, domain, uniform(), dt, nsteps, 0.0);
+    }
+
+    GRHD_IF_INVERSE_DX(V)
+    AtmosphereResetDiagnostics advance_rk4(DeviceLevelData &state,
+                                           const ProblemDomain &domain,
+                                           const V &inverse_dx_U, double dt,
+                                           int nsteps = 1) const
+    {
+        return rk4_at(state, domain, to_inverse_dx(inverse_dx_U), dt, nsteps,
+                      0.0);
     }
 };
 
@@ -563,10 +745,27 @@ class TimeDependentFixedBGLevelDataFiniteVolumeOperator
     {
     }
 
+    /* :702-720 */
+    GRHD_IF_INVERSE_DX(V)
+    void compute_rhs(DeviceLevelData &rhs, DeviceLevelData &state,
+                     const ProblemDomain &domain, const V &inverse_dx_U,
+                     double time) const
+    {
+        rhs_at(rhs, state, domain, to_inverse_dx(inverse_dx_U), time);
+    }
+
+    /* :722-726 */
     void compute_rhs(DeviceLevelData &rhs, DeviceLevelData &state,
                      const ProblemDomain &domain, double time) const
     {
-        rhs_at(rhs, state, domain, time);
+        rhs_at(rhs, state, domain, uniform(), time);
+    }
+
+    /* :728-743 */
+    AtmosphereResetDiagnostics recover_primitives(DeviceLevelData &state,
+                                                  double time) const
+    {
+        return recover_at(state, state.problem_domain(), time);
     }
 
     AtmosphereResetDiagnostics recover_primitives(DeviceLevelData &state,
@@ -574,6 +773,15 @@ class TimeDependentFixedBGLevelDataFiniteVolumeOperator
                                                   double time) const
     {
         return recover_at(state, domain, time);
+    }
+
+    /* :745-757 */
+    void update_conserved(DeviceLevelData &state, const DeviceLevelData &rhs,
+                          double dt, double recovery_time,
+                          bool recover_primitives_after_update = true) const
+    {
+        update_at(state, rhs, state.problem_domain(), dt,
+                  recover_primitives_after_update, recovery_time);
     }
 
     void update_conserved(DeviceLevelData &state, const DeviceLevelData &rhs,
@@ -585,27 +793,55 @@ class TimeDependentFixedBGLevelDataFiniteVolumeOperator
                   recovery_time);
     }
 
+    /* :759-767 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_max_inverse_dt(const DeviceLevelData &state,
+                                  const IntBox &domain_box,
+                                  const V &inverse_dx_U, double time) const
+    {
+        (void)domain_box;
+        return max_inverse_dt_at(state, to_inverse_dx(inverse_dx_U), time);
+    }
+
     double compute_max_inverse_dt(const DeviceLevelData &state,
                                   const ProblemDomain &domain,
                                   double time) const
     {
-        return max_inverse_dt_at(state, domain, time);
+        (void)domain;
+        return max_inverse_dt_at(state, uniform(), time);
+    }
+
+    /* :769-780 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const IntBox &domain_box, const V &inverse_dx_U,
+                             double cfl, double time) const
+    {
+        (void)domain_box;
+        return stable_dt_at(state, to_inverse_dx(inverse_dx_U), cfl, time);
     }
 
     double compute_stable_dt(const DeviceLevelData &state,
                              const ProblemDomain &domain, double cfl,
                              double time) const
     {
-        if (cfl <= 0.0)
-            throw std::invalid_argument("GRHD CFL number must be positive");
-        return cfl /
-               std::max(compute_max_inverse_dt(state, domain, time), 1.0e-14);
+        (void)domain;
+        return stable_dt_at(state, uniform(), cfl, time);
     }
 
+    /* :782-809 */
+    GRHD_IF_INVERSE_DX(V)
     void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
-                        double dt, double time) const
+                        const V &inverse_dx_U, double time, double dt) const
     {
-        ssprk2_at(state, domain, dt, time);
+        ssprk2_at(state, domain, to_inverse_dx(inverse_dx_U), dt, time);
+    }
+
+    /* the same with 1/dx in every direction */
+    void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
+                        double time, double dt) const
+    {
+        ssprk2_at(state, domain, uniform(), dt, time);
     }
 
     AtmosphereResetDiagnostics advance_rk4(DeviceLevelData &state,
@@ -613,8 +849,327 @@ class TimeDependentFixedBGLevelDataFiniteVolumeOperator
                                            double dt, int nsteps,
                                            double time) const
     {
-        return rk4_at(state, domain, dt, nsteps, time);
+        return rk4_at(state, domain, uniform(), dt, nsteps, time);
     }
+};
+
+/* FixedBGAMRLevelHooks (FixedBGAMRLevelHooks.hpp:22-167): what a GRAMRLevel
+ * delegates to on a static background -- RHS, post-update recovery, CFL and
+ * a standalone SSPRK2 step -- forwarded to the level operator. */
+class FixedBGAMRLevelHooks
+{
+  public:
+    FixedBGAMRLevelHooks(
+        const IdealGasEOS &a_eos, const grhd_background &a_background,
+        double a_dx, const std::array<double, 3> &a_center,
+        const AtmosphereOptions &a_atmosphere = AtmosphereOptions(),
+        double a_limiter_theta = 1.5, bool a_use_reconstruction = true,
+        const RecoveryOptions &a_recovery_options = RecoveryOptions(),
+        bool a_use_static_metric_sources = true, int a_comp_base = 0,
+        std::shared_ptr<Context> a_context = nullptr, void *a_stream = nullptr)
+        : m_level_operator(a_eos, a_background, a_dx, a_center, a_atmosphere,
+                           a_limiter_theta, a_use_reconstruction,
+                           a_recovery_options, a_use_static_metric_sources,
+                           a_comp_base, std::move(a_context), a_stream),
+          m_dx(a_dx)
+    {
+    }
+
+    const FixedBGLevelDataFiniteVolumeOperator &level_operator() const
+    {
+        return m_level_operator;
+    }
+
+    /* :40-46 */
+    GRHD_IF_INVERSE_DX(V)
+    void specific_eval_rhs(DeviceLevelData &soln, DeviceLevelData &rhs,
+                           const ProblemDomain &domain,
+                           const V &inverse_dx_U) const
+    {
+        m_level_operator.compute_rhs(rhs, soln, domain, inverse_dx_U);
+    }
+
+    /* :48-53 */
+    void specific_eval_rhs(DeviceLevelData &soln, DeviceLevelData &rhs,
+                           const ProblemDomain &domain, double dx) const
+    {
+        specific_eval_rhs(soln, rhs, domain, make_inverse_dx(dx));
+    }
+
+    /* :55-63 (the background is static: time is ignored) */
+    GRHD_IF_INVERSE_DX(V)
+    void specific_eval_rhs(DeviceLevelData &soln, DeviceLevelData &rhs,
+                           const ProblemDomain &domain, const V &inverse_dx_U,
+                           double time) const
+    {
+        (void)time;
+        specific_eval_rhs(soln, rhs, domain, inverse_dx_U);
+    }
+
+    /* :65-71 */
+    void specific_eval_rhs(DeviceLevelData &soln, DeviceLevelData &rhs,
+                           const ProblemDomain &domain, double dx,
+                           double time) const
+    {
+        specific_eval_rhs(soln, rhs, domain, make_inverse_dx(dx), time);
+    }
+
+    /* :73-77 */
+    AtmosphereResetDiagnostics specific_update_ode(DeviceLevelData &soln) const
+    {
+        return m_level_operator.recover_primitives(soln);
+    }
+
+    /* :79-86 */
+    AtmosphereResetDiagnostics specific_update_ode(DeviceLevelData &soln,
+                                                   const DeviceLevelData &rhs,
+                                                   double dt) const
+    {
+        (void)rhs;
+        (void)dt;
+        return specific_update_ode(soln);
+    }
+
+    /* :88-95 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const IntBox &domain_box, const V &inverse_dx_U,
+                             double cfl) const
+    {
+        return m_level_operator.compute_stable_dt(state, domain_box,
+                                                  inverse_dx_U, cfl);
+    }
+
+    /* :97-104 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const ProblemDomain &domain,
+                             const V &inverse_dx_U, double cfl) const
+    {
+        return compute_stable_dt(state, domain.domainBox(), inverse_dx_U,
+                                 cfl);
+    }
+
+    /* :106-111 */
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const ProblemDomain &domain, double dx,
+                             double cfl) const
+    {
+        return compute_stable_dt(state, domain, make_inverse_dx(dx), cfl);
+    }
+
+    /* :113-124 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_coupled_dt(const DeviceLevelData &state,
+                              const ProblemDomain &domain,
+                              const V &inverse_dx_U, double cfl,
+                              double metric_dt) const
+    {
+        if (metric_dt <= 0.0)
+            throw std::invalid_argument(
+                "GRHD fixed-background coupled timestep must be positive");
+        const double hydro_dt =
+            compute_stable_dt(state, domain, inverse_dx_U, cfl);
+        return std::min(metric_dt, hydro_dt);
+    }
+
+    /* :126-132 */
+    double compute_coupled_dt(const DeviceLevelData &state,
+                              const ProblemDomain &domain, double dx,
+                              double cfl, double metric_dt) const
+    {
+        return compute_coupled_dt(state, domain, make_inverse_dx(dx), cfl,
+                                  metric_dt);
+    }
+
+    /* :134-140 */
+    void update_conserved(DeviceLevelData &state, const DeviceLevelData &rhs,
+                          double dt,
+                          bool recover_primitives_after_update = true) const
+    {
+        m_level_operator.update_conserved(state, rhs, dt,
+                                          recover_primitives_after_update);
+    }
+
+    /* :142-148 */
+    GRHD_IF_INVERSE_DX(V)
+    void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
+                        const V &inverse_dx_U, double dt) const
+    {
+        m_level_operator.advance_ssprk2(state, domain, inverse_dx_U, dt);
+    }
+
+    /* :150-154 */
+    void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
+                        double dt) const
+    {
+        advance_ssprk2(state, domain, make_inverse_dx(m_dx), dt);
+    }
+
+  private:
+    static InverseDx make_inverse_dx(double dx)
+    {
+        return InverseDx{{1.0 / dx, 1.0 / dx, 1.0 / dx}};
+    }
+
+    FixedBGLevelDataFiniteVolumeOperator m_level_operator;
+    double m_dx;
+};
+
+/* TimeDependentFixedBGAMRLevelHooks (FixedBGAMRLevelHooks.hpp:170-310). */
+class TimeDependentFixedBGAMRLevelHooks
+{
+  public:
+    TimeDependentFixedBGAMRLevelHooks(
+        const IdealGasEOS &a_eos, const grhd_background &a_background,
+        double a_dx, const std::array<double, 3> &a_center,
+        const AtmosphereOptions &a_atmosphere = AtmosphereOptions(),
+        double a_limiter_theta = 1.5, bool a_use_reconstruction = true,
+        const RecoveryOptions &a_recovery_options = RecoveryOptions(),
+        bool a_use_static_metric_sources = true,
+        bool a_use_metric_volume_time_sources = true,
+        double a_metric_time_derivative_step = 1.0e-3, int a_comp_base = 0,
+        std::shared_ptr<Context> a_context = nullptr, void *a_stream = nullptr)
+        : m_level_operator(a_eos, a_background, a_dx, a_center, a_atmosphere,
+                           a_limiter_theta, a_use_reconstruction,
+                           a_recovery_options, a_use_static_metric_sources,
+                           a_use_metric_volume_time_sources,
+                           a_metric_time_derivative_step, a_comp_base,
+                           std::move(a_context), a_stream),
+          m_dx(a_dx)
+    {
+    }
+
+    const TimeDependentFixedBGLevelDataFiniteVolumeOperator &
+    level_operator() const
+    {
+        return m_level_operator;
+    }
+
+    /* :194-201 */
+    GRHD_IF_INVERSE_DX(V)
+    void specific_eval_rhs(DeviceLevelData &soln, DeviceLevelData &rhs,
+                           const ProblemDomain &domain, const V &inverse_dx_U,
+                           double time) const
+    {
+        m_level_operator.compute_rhs(rhs, soln, domain, inverse_dx_U, time);
+    }
+
+    /* :203-209 */
+    void specific_eval_rhs(DeviceLevelData &soln, DeviceLevelData &rhs,
+                           const ProblemDomain &domain, double dx,
+                           double time) const
+    {
+        specific_eval_rhs(soln, rhs, domain, make_inverse_dx(dx), time);
+    }
+
+    /* :211-215 */
+    AtmosphereResetDiagnostics specific_update_ode(DeviceLevelData &soln,
+                                                   double time) const
+    {
+        return m_level_operator.recover_primitives(soln, time);
+    }
+
+    /* :217-224 */
+    AtmosphereResetDiagnostics specific_update_ode(DeviceLevelData &soln,
+                                                   const DeviceLevelData &rhs,
+                                                   double dt,
+                                                   double time) const
+    {
+        (void)rhs;
+        (void)dt;
+        return specific_update_ode(soln, time);
+    }
+
+    /* :226-233 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const IntBox &domain_box, const V &inverse_dx_U,
+                             double cfl, double time) const
+    {
+        return m_level_operator.compute_stable_dt(state, domain_box,
+                                                  inverse_dx_U, cfl, time);
+    }
+
+    /* :235-242 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const ProblemDomain &domain,
+                             const V &inverse_dx_U, double cfl,
+                             double time) const
+    {
+        return compute_stable_dt(state, domain.domainBox(), inverse_dx_U, cfl,
+                                 time);
+    }
+
+    /* :244-250 */
+    double compute_stable_dt(const DeviceLevelData &state,
+                             const ProblemDomain &domain, double dx,
+                             double cfl, double time) const
+    {
+        return compute_stable_dt(state, domain, make_inverse_dx(dx), cfl,
+                                 time);
+    }
+
+    /* :252-264 */
+    GRHD_IF_INVERSE_DX(V)
+    double compute_coupled_dt(const DeviceLevelData &state,
+                              const ProblemDomain &domain,
+                              const V &inverse_dx_U, double cfl,
+                              double metric_dt, double time) const
+    {
+        if (metric_dt <= 0.0)
+            throw std::invalid_argument(
+                "GRHD time-dependent fixed-background coupled timestep must "
+                "be positive");
+        const double hydro_dt =
+            compute_stable_dt(state, domain, inverse_dx_U, cfl, time);
+        return std::min(metric_dt, hydro_dt);
+    }
+
+    /* :266-273 */
+    double compute_coupled_dt(const DeviceLevelData &state,
+                              const ProblemDomain &domain, double dx,
+                              double cfl, double metric_dt,
+                              double time) const
+    {
+        return compute_coupled_dt(state, domain, make_inverse_dx(dx), cfl,
+                                  metric_dt, time);
+    }
+
+    /* :275-282 */
+    void update_conserved(DeviceLevelData &state, const DeviceLevelData &rhs,
+                          double dt, double recovery_time,
+                          bool recover_primitives_after_update = true) const
+    {
+        m_level_operator.update_conserved(state, rhs, dt, recovery_time,
+                                          recover_primitives_after_update);
+    }
+
+    /* :284-291 */
+    GRHD_IF_INVERSE_DX(V)
+    void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
+                        const V &inverse_dx_U, double time, double dt) const
+    {
+        m_level_operator.advance_ssprk2(state, domain, inverse_dx_U, time,
+                                        dt);
+    }
+
+    /* :293-298 */
+    void advance_ssprk2(DeviceLevelData &state, const ProblemDomain &domain,
+                        double time, double dt) const
+    {
+        advance_ssprk2(state, domain, make_inverse_dx(m_dx), time, dt);
+    }
+
+  private:
+    static InverseDx make_inverse_dx(double dx)
+    {
+        return InverseDx{{1.0 / dx, 1.0 / dx, 1.0 / dx}};
+    }
+
+    TimeDependentFixedBGLevelDataFiniteVolumeOperator m_level_operator;
+    double m_dx;
 };
 
 /* compute_leveldata_conserved_sum (LevelDataDriver.hpp:22-57) */
@@ -649,6 +1204,285 @@ inline bool leveldata_conserved_state_is_finite(
           nullptr, "conserved_state_is_finite");
     return ok != 0;
 }
+
+/* FixedBGLevelDataDriverView (FixedBGLevelDataDriver.hpp:17-141): a level
+ * state, its domain, 1/dx in every direction, the level operator, and a
+ * time and step count advanced by each update. */
+class FixedBGLevelDataDriverView
+{
+  public:
+    FixedBGLevelDataDriverView(
+        DeviceLevelData &a_state, const ProblemDomain &a_domain, double a_dx,
+        const std::array<double, 3> &a_center, const IdealGasEOS &a_eos,
+        const grhd_background &a_background,
+        const AtmosphereOptions &a_atmosphere = AtmosphereOptions(),
+        double a_limiter_theta = 1.5, bool a_use_reconstruction = true,
+        const RecoveryOptions &a_recovery_options = RecoveryOptions(),
+        bool a_use_static_metric_sources = true, int a_comp_base = 0,
+        std::shared_ptr<Context> a_context = nullptr, void *a_stream = nullptr)
+        : m_state(a_state), m_domain(a_domain), m_dx(a_dx),
+          m_center(a_center),
+          m_inverse_dx_U{{1.0 / a_dx, 1.0 / a_dx, 1.0 / a_dx}},
+          m_operator(a_eos, a_background, a_dx, a_center, a_atmosphere,
+                     a_limiter_theta, a_use_reconstruction,
+                     a_recovery_options, a_use_static_metric_sources,
+                     a_comp_base, std::move(a_context), a_stream)
+    {
+        if (a_dx <= 0.0)
+            throw std::invalid_argument(
+                "GRHD fixed-background driver dx must be positive");
+    }
+
+    FixedBGLevelDataDriverView(
+        DeviceLevelData &a_state, const IntBox &a_domain_box, double a_dx,
+        const std::array<double, 3> &a_center, const IdealGasEOS &a_eos,
+        const grhd_background &a_background,
+        const AtmosphereOptions &a_atmosphere = AtmosphereOptions(),
+        double a_limiter_theta = 1.5, bool a_use_reconstruction = true,
+        const RecoveryOptions &a_recovery_options = RecoveryOptions(),
+        bool a_use_static_metric_sources = true, int a_comp_base = 0,
+        std::shared_ptr<Context> a_context = nullptr, void *a_stream = nullptr)
+        : FixedBGLevelDataDriverView(
+              a_state, ProblemDomain(a_domain_box), a_dx, a_center, a_eos,
+              a_background, a_atmosphere, a_limiter_theta,
+              a_use_reconstruction, a_recovery_options,
+              a_use_static_metric_sources, a_comp_base, std::move(a_context),
+              a_stream)
+    {
+    }
+
+    DeviceLevelData &state() { return m_state; }
+    const DeviceLevelData &state() const { return m_state; }
+    const std::vector<IntBox> &grids() const { return m_state.grids(); }
+    const IntBox &domain_box() const { return m_domain.domainBox(); }
+    const ProblemDomain &problem_domain() const { return m_domain; }
+    double dx() const { return m_dx; }
+    const std::array<double, 3> &center() const { return m_center; }
+    const InverseDx &inverse_dx_U() const { return m_inverse_dx_U; }
+    double time() const { return m_time; }
+    int step() const { return m_step; }
+
+    void compute_rhs(DeviceLevelData &rhs)
+    {
+        m_operator.compute_rhs(rhs, m_state, m_domain, m_inverse_dx_U);
+    }
+
+    void add_conserved_rhs_to(DeviceLevelData &rhs)
+    {
+        m_operator.add_conserved_rhs_to(rhs, m_state, m_domain,
+                                        m_inverse_dx_U);
+    }
+
+    double compute_stable_dt(double cfl) const
+    {
+        return m_operator.compute_stable_dt(m_state, m_domain.domainBox(),
+                                            m_inverse_dx_U, cfl);
+    }
+
+    double compute_coupled_dt(double cfl, double metric_dt) const
+    {
+        if (metric_dt <= 0.0)
+            throw std::invalid_argument(
+                "GRHD fixed-background coupled timestep must be positive");
+        return std::min(metric_dt, compute_stable_dt(cfl));
+    }
+
+    AtmosphereResetDiagnostics recover_primitives()
+    {
+        return m_operator.recover_primitives(m_state, m_domain);
+    }
+
+    void update_conserved(const DeviceLevelData &rhs, double dt,
+                          bool recover_primitives_after_update = true)
+    {
+        m_operator.update_conserved(m_state, rhs, m_domain, dt,
+                                    recover_primitives_after_update);
+        m_time += dt;
+        ++m_step;
+    }
+
+    void advance_ssprk2(double dt)
+    {
+        m_operator.advance_ssprk2(m_state, m_domain, m_inverse_dx_U, dt);
+        m_time += dt;
+        ++m_step;
+    }
+
+    ConservedSum conserved_sum() const
+    {
+        return compute_leveldata_conserved_sum(
+            m_state, m_domain, m_dx, m_center, m_operator.comp_base(),
+            m_operator.context(), m_operator.stream());
+    }
+
+    bool conserved_state_is_finite() const
+    {
+        return leveldata_conserved_state_is_finite(
+            m_state, m_domain, m_dx, m_center, m_operator.comp_base(),
+            m_operator.context(), m_operator.stream());
+    }
+
+  private:
+    DeviceLevelData &m_state;
+    ProblemDomain m_domain;
+    double m_dx;
+    std::array<double, 3> m_center;
+    InverseDx m_inverse_dx_U;
+    FixedBGLevelDataFiniteVolumeOperator m_operator;
+    double m_time = 0.0;
+    int m_step = 0;
+};
+
+/* TimeDependentFixedBGLevelDataDriverView
+ * (FixedBGLevelDataDriver.hpp:232-382): as above with the background at the
+ * view's time; an update recovers at time + dt. */
+class TimeDependentFixedBGLevelDataDriverView
+{
+  public:
+    TimeDependentFixedBGLevelDataDriverView(
+        DeviceLevelData &a_state, const ProblemDomain &a_domain, double a_dx,
+        const std::array<double, 3> &a_center, const IdealGasEOS &a_eos,
+        const grhd_background &a_background,
+        const AtmosphereOptions &a_atmosphere = AtmosphereOptions(),
+        double a_limiter_theta = 1.5, bool a_use_reconstruction = true,
+        const RecoveryOptions &a_recovery_options = RecoveryOptions(),
+        bool a_use_static_metric_sources = true,
+        bool a_use_metric_volume_time_sources = true,
+        double a_metric_time_derivative_step = 1.0e-3, int a_comp_base = 0,
+        std::shared_ptr<Context> a_context = nullptr, void *a_stream = nullptr)
+        : m_state(a_state), m_domain(a_domain), m_dx(a_dx),
+          m_center(a_center),
+          m_inverse_dx_U{{1.0 / a_dx, 1.0 / a_dx, 1.0 / a_dx}},
+          m_operator(a_eos, a_background, a_dx, a_center, a_atmosphere,
+                     a_limiter_theta, a_use_reconstruction,
+                     a_recovery_options, a_use_static_metric_sources,
+                     a_use_metric_volume_time_sources,
+                     a_metric_time_derivative_step, a_comp_base,
+                     std::move(a_context), a_stream)
+    {
+        if (a_dx <= 0.0)
+            throw std::invalid_argument(
+                "GRHD fixed-background driver dx must be positive");
+    }
+
+    TimeDependentFixedBGLevelDataDriverView(
+        DeviceLevelData &a_state, const IntBox &a_domain_box, double a_dx,
+        const std::array<double, 3> &a_center, const IdealGasEOS &a_eos,
+        const grhd_background &a_background,
+        const AtmosphereOptions &a_atmosphere = AtmosphereOptions(),
+        double a_limiter_theta = 1.5, bool a_use_reconstruction = true,
+        const RecoveryOptions &a_recovery_options = RecoveryOptions(),
+        bool a_use_static_metric_sources = true,
+        bool a_use_metric_volume_time_sources = true,
+        double a_metric_time_derivative_step = 1.0e-3, int a_comp_base = 0,
+        std::shared_ptr<Context> a_context = nullptr, void *a_stream = nullptr)
+        : TimeDependentFixedBGLevelDataDriverView(
+              a_state, ProblemDomain(a_domain_box), a_dx, a_center, a_eos,
+              a_background, a_atmosphere, a_limiter_theta,
+              a_use_reconstruction, a_recovery_options,
+              a_use_static_metric_sources, a_use_metric_volume_time_sources,
+              a_metric_time_derivative_step, a_comp_base,
+              std::move(a_context), a_stream)
+    {
+    }
+
+    DeviceLevelData &state() { return m_state; }
+    const DeviceLevelData &state() const { return m_state; }
+    const std::vector<IntBox> &grids() const { return m_state.grids(); }
+    const IntBox &domain_box() const { return m_domain.domainBox(); }
+    const ProblemDomain &problem_domain() const { return m_domain; }
+    double dx() const { return m_dx; }
+    const std::array<double, 3> &center() const { return m_center; }
+    const InverseDx &inverse_dx_U() const { return m_inverse_dx_U; }
+    double time() const { return m_time; }
+    int step() const { return m_step; }
+    void set_time(double time) { m_time = time; }
+
+    void compute_rhs(DeviceLevelData &rhs)
+    {
+        m_operator.compute_rhs(rhs, m_state, m_domain, m_inverse_dx_U,
+                               m_time);
+    }
+
+    void compute_rhs(DeviceLevelData &rhs, double time)
+    {
+        m_operator.compute_rhs(rhs, m_state, m_domain, m_inverse_dx_U, time);
+    }
+
+    double compute_stable_dt(double cfl) const
+    {
+        return m_operator.compute_stable_dt(m_state, m_domain.domainBox(),
+                                            m_inverse_dx_U, cfl, m_time);
+    }
+
+    double compute_stable_dt(double cfl, double time) const
+    {
+        return m_operator.compute_stable_dt(m_state, m_domain.domainBox(),
+                                            m_inverse_dx_U, cfl, time);
+    }
+
+    double compute_coupled_dt(double cfl, double metric_dt) const
+    {
+        if (metric_dt <= 0.0)
+            throw std::invalid_argument(
+                "GRHD fixed-background coupled timestep must be positive");
+        return std::min(metric_dt, compute_stable_dt(cfl));
+    }
+
+    AtmosphereResetDiagnostics recover_primitives()
+    {
+        return m_operator.recover_primitives(m_state, m_domain, m_time);
+    }
+
+    AtmosphereResetDiagnostics recover_primitives(double time)
+    {
+        return m_operator.recover_primitives(m_state, m_domain, time);
+    }
+
+    void update_conserved(const DeviceLevelData &rhs, double dt,
+                          bool recover_primitives_after_update = true)
+    {
+        const double recovery_time = m_time + dt;
+        m_operator.update_conserved(m_state, rhs, m_domain, dt, recovery_time,
+                                    recover_primitives_after_update);
+        m_time = recovery_time;
+        ++m_step;
+    }
+
+    void advance_ssprk2(double dt)
+    {
+        m_operator.advance_ssprk2(m_state, m_domain, m_inverse_dx_U, m_time,
+                                  dt);
+        m_time += dt;
+        ++m_step;
+    }
+
+    ConservedSum conserved_sum() const
+    {
+        return compute_leveldata_conserved_sum(
+            m_state, m_domain, m_dx, m_center, m_operator.comp_base(),
+            m_operator.context(), m_operator.stream());
+    }
+
+    bool conserved_state_is_finite() const
+    {
+        return leveldata_conserved_state_is_finite(
+            m_state, m_domain, m_dx, m_center, m_operator.comp_base(),
+            m_operator.context(), m_operator.stream());
+    }
+
+  private:
+    DeviceLevelData &m_state;
+    ProblemDomain m_domain;
+    double m_dx;
+    std::array<double, 3> m_center;
+    InverseDx m_inverse_dx_U;
+    TimeDependentFixedBGLevelDataFiniteVolumeOperator m_operator;
+    double m_time = 0.0;
+    int m_step = 0;
+};
+
+#undef GRHD_IF_INVERSE_DX
 
 } // namespace amd
 } // namespace GRHD

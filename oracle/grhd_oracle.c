@@ -1557,6 +1557,18 @@ static void excise_rhs(double *rhs, const fab_shape_t *rs, int cb,
 /* ------------------------------------------------------------------ */
 /* Argument checks                                                    */
 /* ------------------------------------------------------------------ */
+/* inverse_dx_U[d] of the reference's operator overloads
+ * (FixedBGFArrayBoxFiniteVolume.hpp:468-484, :535-589): the caller's, or
+ * 1/dx in every direction when all three are zero (make_inverse_dx,
+ * :598-603). */
+static double inverse_dx_of(const grhd_box *b, int d)
+{
+    if (b->inverse_dx[0] == 0.0 && b->inverse_dx[1] == 0.0 &&
+        b->inverse_dx[2] == 0.0)
+        return 1.0 / b->dx;
+    return b->inverse_dx[d];
+}
+
 static int check_args(const grhd_box *b, const grhd_params *o,
                       const grhd_background *bg, int ncomp, int cb)
 {
@@ -1568,6 +1580,11 @@ static int check_args(const grhd_box *b, const grhd_params *o,
         return GRHD_ERR_INVALID_ARGUMENT;
     if (!(b->dx > 0.0))
         return GRHD_ERR_INVALID_ARGUMENT;
+    if (b->inverse_dx[0] != 0.0 || b->inverse_dx[1] != 0.0 ||
+        b->inverse_dx[2] != 0.0)
+        for (int d = 0; d < 3; ++d)
+            if (!(b->inverse_dx[d] > 0.0) || !isfinite(b->inverse_dx[d]))
+                return GRHD_ERR_INVALID_ARGUMENT;
     for (int d = 0; d < 3; ++d)
     {
         if (b->lo[d] > b->hi[d])
@@ -1627,8 +1644,9 @@ int grhd_oracle_compute_rhs(const grhd_box *b, const grhd_params *o,
     for (int d = 0; d < 3 && rc == GRHD_OK; ++d)
     {
         memset(rhs_dir, 0, sizeof(double) * 5 * (size_t)is.npts);
+        /* inverse_dx_U[direction] (:341-343) */
         rc = directional_flux_rhs(rhs_dir, state, &ss, cb, b, &bg, o, d,
-                                  1.0 / b->dx, prims, slopes, err);
+                                  inverse_dx_of(b, d), prims, slopes, err);
         if (rc)
             break;
         /* add_conserved_rhs (FArrayBoxFiniteVolume.hpp:912-925) */
@@ -1736,7 +1754,8 @@ int grhd_oracle_compute_max_inverse_dt(const grhd_box *b,
     background_at(bgp, time, &bg);
     const fab_shape_t ss = shape_of(b->glo, b->ghi);
     const double gamma = o->adiabatic_index;
-    const double inverse_dx = 1.0 / b->dx;
+    const double inverse_dx_U[3] = {inverse_dx_of(b, 0), inverse_dx_of(b, 1),
+                                    inverse_dx_of(b, 2)};
     const int nx = b->hi[0] - b->lo[0] + 1, ny = b->hi[1] - b->lo[1] + 1;
     fail_t fail;
     fail_init(&fail);
@@ -1773,7 +1792,7 @@ int grhd_oracle_compute_max_inverse_dt(const grhd_box *b,
                     characteristic_speeds(&p, gamma, geo.g, geo.gu, d,
                                           geo.lapse, geo.shift, &lo, &hi);
                     const double ms = std_max(fabs(lo), fabs(hi));
-                    cell += ms * inverse_dx;
+                    cell += ms * inverse_dx_U[d];
                 }
                 max_inverse_dt = std_max(max_inverse_dt, cell);
             }

@@ -156,6 +156,24 @@ def test_rhs_reference_leveldata_box_binary(hip_context):
     _rhs_case(w, hip_context, time=0.35, time_dependent=True)
 
 
+def test_anisotropic_inverse_dx_rhs_cfl_and_rk4(hip_context):
+    """inverse_dx_U with a different spacing per direction (the reference's
+    overloads, FixedBGFArrayBoxFiniteVolume.hpp:468-484, :535-589): the
+    rhs, the CFL rate and an RK4 step equal the oracle's bit for bit."""
+    w = small_torus()
+    inv = 1.0 / w.box.dx
+    w.box = dataclasses.replace(w.box,
+                                inverse_dx=(inv, 1.25 * inv, 0.8 * inv))
+    _rhs_case(w, hip_context)
+    op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, w.options,
+                                              context=hip_context)
+    O = oracle.Oracle(w.background, w.box, w.options)
+    st = np.ascontiguousarray(w.state)
+    assert op.compute_max_inverse_dt(to_dev(st)) == \
+        O.compute_max_inverse_dt(st.copy())
+    _step_case(w, hip_context, "rk4", 1)
+
+
 # ---------------------------------------------------------------- recovery
 
 
