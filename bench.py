@@ -14,9 +14,14 @@ launch stream during the timed steps) and a CPU baseline (the oracle -- a C
 restatement of the reference's CPU path with its call structure -- timed on
 this host's cores on one RK4 step of the same box).
 
-The stage kernels are FP64 vector-ALU bound (no dense contraction, so no
-MFMA), so the roofline is FP64 VALU FLOP/s against the MI355X FP64 vector
-peak; the HBM fraction of the same launch is reported beside it.
+Roofline units are algorithmic, not issued: FLOPs per cell from the
+de-duplicated operation count of the oracle (oracle/opcount.cpp, frozen in
+profiles/algorithmic_counts.json), bytes from the values each kernel must
+move once.  The bound of the dominant kernel is the roof its algorithmic
+intensity (FLOP/B) meets first against the MI355X ridge of
+78.6 TFLOP/s / 8 TB/s = 9.8 FLOP/B; the other roof is reported beside it,
+and so is the whole step's (F_alg and 689 B per cell-update).  Nothing here
+is a dense contraction, so MFMA peaks do not apply.
 """
 import argparse
 import json
@@ -32,6 +37,16 @@ METRIC = ("GRHD cell-updates/s per RK4 step, plain FP64 vs mixed/batched "
           "at fixed error")
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+RIDGE_FLOP_PER_BYTE = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+
+# Which parts of the de-duplicated per-cell count (profiles/
+# algorithmic_counts.json, flop_per_cell_stage_by_part) each kernel class
+# performs: k_cells evaluates the cell's metric with its derivatives, the
+# divergence, the sources, the stage update and the recovery; the face
+# launch the slopes and the faces.
+KERNEL_PARTS = {"cells": ("metric", "divergence", "sources", "update",
+                          "recovery"),
+                "faces_x": ("slopes", "faces")}
 
 # Algorithmic HBM bytes per unit of each kernel class: every value the
 # kernel must read or write once, in FP64.
@@ -84,6 +99,26 @@ def measured_traffic(cls, n):
         parts = [per.get(k) for k in FACE_KERNELS]
         return None if None in parts else sum(parts)
     return per.get(cls)
+
+
+def algorithmic_counts(config):
+    """The frozen algorithmic count of one workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "algorithmic_counts.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["configs"].get(config)
+
+
+def measured_step_bytes(n):
+    """Counter HBM bytes of one RK4 step of the n^3 box (4 stages of each
+    kernel class, summed from the committed FETCH/WRITE summary); None
+    when the summary does not cover this size."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if n != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_rk4_step")
 
 
 def measured_flops(cls, n):
@@ -383,7 +418,20 @@ def bench_batched(args):
                 "speedup_vs_plain_per_box": mixed / plain,
                 "speedup_vs_batched_fp64": mixed / batched,
                 "error_vs_batched_fp64": err_m,
+                "gate_1e-6": gate_1e6(err_m),
                 "l1_rel_after_one_step": err_one}}
+
+
+def gate_1e6(err):
+    """The precision target: relative L1 error against FP64 <= 1e-6 for
+    every conserved variable (and the accretion rate, when reported).  A
+    speedup only counts as "at fixed error" where this holds."""
+    failing = {k: e["l1_rel"] for k, e in err.items()
+               if k != "mdot" and not e["l1_rel"] <= 1e-6}
+    if "mdot" in err and not err["mdot"]["rel"] <= 1e-6:
+        failing["mdot"] = err["mdot"]["rel"]
+    return {"meets": not failing, "failing": failing,
+            "speedup_is_at_fixed_error": not failing}
 
 
 def bench_mixed(args):
@@ -469,6 +517,7 @@ def bench_mixed(args):
                         "%d steps" % (w.name, steps),
             "interior_cells": cells, "steps": steps,
             "value": v32, "fp64_value": v64, "speedup_vs_fp64": v32 / v64,
+            "gate_1e-6": gate_1e6(err),
             "ms_per_step": t32 / steps * 1e3,
             "fp64_ms_per_step": t64 / steps * 1e3,
             "error_vs_fp64": err,
@@ -694,6 +743,7 @@ def main():
             print("bench:   %-8s %9.3f ms total, %5d launches, %.4f ms avg"
                   % (k, ms, n, ms / n), file=sys.stderr, flush=True)
 
+    counts = algorithmic_counts("c2" if args.n == 128 else None)
     dom = max((k for k in timing if k in ALG_BYTES),
               key=lambda k: timing[k][0])
     ms, nl = timing[dom]
@@ -702,18 +752,53 @@ def main():
     gbs = ALG_BYTES[dom] * units / avg_s / 1e9
     hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": gbs / HBM_PEAK_GBS, "alg_bytes_per_unit": ALG_BYTES[dom]}
-    flops = measured_flops(dom, args.n)
     common = {"kernel": dom, "traffic": measured_traffic(dom, args.n),
-              "avg_launch_ms": ms / nl, "units_per_launch": units}
-    if flops:
+              "avg_launch_ms": ms / nl, "units_per_launch": units,
+              "ridge_flop_per_byte": RIDGE_FLOP_PER_BYTE}
+    issued = measured_flops(dom, args.n)
+    if issued:
+        common["issued_fp64_flop_per_unit"] = issued / units
+    if counts and dom in KERNEL_PARTS:
+        parts = counts["flop_per_cell_stage_by_part"]
+        # faces: one face per direction per cell, so 3 faces per cell
+        per_cell = sum(parts[p] for p in KERNEL_PARTS[dom])
+        cells_per_launch = args.n ** 3
+        flops = per_cell * cells_per_launch
         tf = flops / avg_s / 1e12
-        roofline = dict(bound="valu", achieved=tf, peak=FP64_PEAK_TFLOPS,
-                        unit="TFLOP/s", frac=tf / FP64_PEAK_TFLOPS,
-                        fp64_flops_per_unit=flops / units, hbm=hbm, **common)
+        fp64 = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / FP64_PEAK_TFLOPS,
+                "alg_flop_per_cell": per_cell}
+        intensity = flops / (ALG_BYTES[dom] * units)
+        common["alg_intensity_flop_per_byte"] = intensity
+        if intensity < RIDGE_FLOP_PER_BYTE:
+            roofline = dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS,
+                            unit="GB/s", frac=gbs / HBM_PEAK_GBS,
+                            alg_bytes_per_unit=ALG_BYTES[dom], fp64=fp64,
+                            **common)
+        else:
+            roofline = dict(bound="fp64", achieved=tf,
+                            peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                            frac=tf / FP64_PEAK_TFLOPS,
+                            alg_flop_per_cell=per_cell, hbm=hbm, **common)
     else:
         roofline = dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS,
                         unit="GB/s", frac=gbs / HBM_PEAK_GBS,
                         alg_bytes_per_unit=ALG_BYTES[dom], **common)
+
+    step = {"alg_bytes_per_cell_update": STEP_ALG_BYTES,
+            "achieved_gbs": value * STEP_ALG_BYTES / 1e9,
+            "hbm_frac": value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS}
+    if counts:
+        f_alg = counts["flop_per_cell_update"]
+        step.update(alg_flop_per_cell_update=f_alg,
+                    achieved_tflops=value * f_alg / 1e12,
+                    fp64_frac=value * f_alg / 1e12 / FP64_PEAK_TFLOPS,
+                    alg_intensity_flop_per_byte=f_alg / STEP_ALG_BYTES,
+                    bound=("fp64" if f_alg / STEP_ALG_BYTES >=
+                           RIDGE_FLOP_PER_BYTE else "hbm"))
+    sb = measured_step_bytes(args.n)
+    if sb:
+        step["measured_bytes_per_cell_update"] = sb / cells
 
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -726,9 +811,7 @@ def main():
                       "dt": w.dt, "parallelism": "single GPU"},
            "roofline": roofline,
            "face_launch": op.ctx.face_launch(),
-           "step_hbm": {"alg_bytes_per_cell_update": STEP_ALG_BYTES,
-                        "achieved_gbs": value * STEP_ALG_BYTES / 1e9,
-                        "frac": value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS}}
+           "step": step}
     if not args.no_batched:
         del op, st, init
         torch.cuda.empty_cache()

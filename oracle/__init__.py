@@ -368,3 +368,52 @@ def checkpoint_unpack(boxes, fabs, values, domain_lo, domain_hi, ghosts):
     lib().grhd_oracle_checkpoint_unpack(cb, len(boxes), ptrs,
                                         fabs[0].shape[0], I3(*domain_lo),
                                         I3(*domain_hi), ghosts, _p(values))
+
+
+# ------------------------------------------------------------ op counting
+OPCOUNT_LIB = os.path.join(HERE, "libgrhd_opcount.so")
+OPCOUNT_PARTS = ("metric", "slopes", "faces", "divergence", "sources",
+                 "update", "recovery")
+OPCOUNT_KINDS = ("add", "mul", "div", "sqrt", "other")
+_opc = None
+
+
+def _opcount_lib():
+    global _opc
+    if _opc is None:
+        if not os.path.exists(OPCOUNT_LIB):
+            build()
+        L = ctypes.CDLL(OPCOUNT_LIB)
+        P = ctypes.POINTER
+        L.grhd_opcount_rk4_stage.restype = ctypes.c_int
+        L.grhd_opcount_rk4_stage.argtypes = [
+            P(abi.GrhdBox), P(abi.GrhdParams), P(abi.GrhdBackground),
+            ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
+            ctypes.c_int, P(ctypes.c_ulonglong)]
+        _opc = L
+    return _opc
+
+
+def opcount_rk4_stage(background, box, options, state, dt, time=0.0,
+                      comp_base=0):
+    """Operation counts of one de-duplicated RK4 stage over the interior of
+    `box` (oracle/opcount.cpp): {part: {kind: count}} plus 'cells',
+    'newton_iterations', 'source_atmosphere_cells', 'reset_cells'."""
+    out = (ctypes.c_ulonglong * (len(OPCOUNT_PARTS) * 5 + 4))()
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    rc = _opcount_lib().grhd_opcount_rk4_stage(
+        ctypes.byref(box.to_c()), ctypes.byref(options.to_c()),
+        ctypes.byref(background.to_c()), time, dt, _p(st), st.shape[0],
+        comp_base, out)
+    if rc != 0:
+        raise OracleError(rc, None)
+    res = {}
+    for p, name in enumerate(OPCOUNT_PARTS):
+        res[name] = {k: int(out[p * 5 + q]) for q, k in
+                     enumerate(OPCOUNT_KINDS)}
+    base = len(OPCOUNT_PARTS) * 5
+    res["cells"] = int(out[base])
+    res["newton_iterations"] = int(out[base + 1])
+    res["source_atmosphere_cells"] = int(out[base + 2])
+    res["reset_cells"] = int(out[base + 3])
+    return res
