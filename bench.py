@@ -46,7 +46,7 @@ RIDGE_FLOP_PER_BYTE = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
 # launch the slopes and the faces.
 KERNEL_PARTS = {"cells": ("metric", "divergence", "sources", "update",
                           "recovery"),
-                "faces_x": ("slopes", "faces")}
+                "faces": ("slopes", "faces")}
 
 # Algorithmic HBM bytes per unit of each kernel class: every value the
 # kernel must read or write once, in FP64.
@@ -57,7 +57,7 @@ KERNEL_PARTS = {"cells": ("metric", "divergence", "sources", "update",
 #            accumulator read (3 x 5 x 8), U_next, the accumulator and W of
 #            the next stage written (5 x 8 + 5 x 8 + 6 x 8).
 #   prims:   per exterior cell: one state read, state + W written.
-ALG_BYTES = {"faces_x": 88, "faces_y": 88, "faces_z": 88,
+ALG_BYTES = {"faces": 88,
              "cells": 120 + 48 + 120 + 40 + 40 + 48, "prims": 40 + 40 + 48}
 # Whole RK4 step, per cell-update: FP64 state traffic of a fused RK4 (stage
 # 1: U0 read, U1 and acc written, 120 B; stages 2 and 3: U_s, U0, acc read,
@@ -66,15 +66,13 @@ ALG_BYTES = {"faces_x": 88, "faces_y": 88, "faces_z": 88,
 STEP_ALG_BYTES = 120 + 200 + 200 + 120 + 48 + 1
 
 
-# The "faces_x" timing class covers the three face directions of a stage
+# The "faces" timing class covers the three face directions of a stage
 # (one fused launch, or three concurrent ones), so its unit count, FLOPs and
-# traffic are those of all three: the "faces" entry of the committed
-# summaries (the fused launch), else the sum of the per-direction entries.
-FACE_KERNELS = ("faces_x", "faces_y", "faces_z")
+# traffic are those of all three.
 
 
 def units_per_launch(cls, n, g):
-    if cls == "faces_x":
+    if cls == "faces":
         return 3 * (n + 1) * n * n
     if cls == "cells":
         return n ** 3
@@ -92,13 +90,7 @@ def measured_traffic(cls, n):
         return None
     with open(path) as f:
         data = json.load(f)
-    per = data.get("bytes_per_launch", {})
-    if cls == "faces_x":
-        if "faces" in per:
-            return per["faces"]
-        parts = [per.get(k) for k in FACE_KERNELS]
-        return None if None in parts else sum(parts)
-    return per.get(cls)
+    return data.get("bytes_per_launch", {}).get(cls)
 
 
 def algorithmic_counts(config):
@@ -130,13 +122,7 @@ def measured_flops(cls, n):
         return None
     with open(path) as f:
         data = json.load(f)
-    per = data.get("flops_per_launch", {})
-    if cls == "faces_x":
-        if "faces" in per:
-            return per["faces"]
-        parts = [per.get(k) for k in FACE_KERNELS]
-        return None if None in parts else sum(parts)
-    return per.get(cls)
+    return data.get("flops_per_launch", {}).get(cls)
 
 
 def cpu_baseline(w, steps=3):
@@ -630,7 +616,14 @@ def bench_adjoint(args):
                        "%d RK4 steps, J = mass flux through r = 4M over the "
                        "last %d steps, horizon excised" % (n, spin0, steps,
                                                           window),
-           "checkpoint_every": every, "recovery_tolerance": 1e-16}
+           "checkpoint_every": every, "recovery_tolerance": 1e-16,
+           "forward_model": "horizon excision on (KerrSchild::"
+                            "check_if_excised, zero rhs inside) and recovery "
+                            "tolerance 1e-16, not the reference's default "
+                            "1e-12 without excision: with those, recoveries "
+                            "that fail inside the horizon reset cells on "
+                            "either side of any perturbation and J is not "
+                            "differentiable (see without_excision)"}
     if args.no_adjoint_checks:
         rec.update(run(True, 0, ()))
         return rec

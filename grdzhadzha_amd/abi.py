@@ -182,7 +182,7 @@ SIGNATURES = {
                                       _P(_d)]),
 }
 
-KERNEL_CLASSES = ("prims", "faces_x", "faces_y", "faces_z", "cells", "other")
+KERNEL_CLASSES = ("prims", "faces", "cells", "other")
 
 _lib = None
 

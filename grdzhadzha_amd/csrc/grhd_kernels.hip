@@ -975,7 +975,7 @@ void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
     }
     select_seg(pl, k);
     faces_launches<T>(c, o, bg, p, s, fused);
-    tick(c, GRHD_KERNEL_FACES_X, s);
+    tick(c, GRHD_KERNEL_FACES, s);
 }
 
 /* FP64 faces, or FP32 faces when the options ask for mixed precision. */

@@ -73,8 +73,8 @@ class Context:
 
     def timing(self):
         """{class: (total_ms, launches)} accumulated since set_timing."""
-        ms = (ctypes.c_double * 6)()
-        n = (ctypes.c_longlong * 6)()
+        ms = (ctypes.c_double * len(abi.KERNEL_CLASSES))()
+        n = (ctypes.c_longlong * len(abi.KERNEL_CLASSES))()
         self.lib.grhd_ctx_get_timing(self.handle, ms, n)
         return {name: (ms[i], n[i])
                 for i, name in enumerate(abi.KERNEL_CLASSES)}

@@ -512,15 +512,13 @@ int grhd_level_advance_rk4(grhd_ctx *ctx, const grhd_level_box *boxes,
 /* ------------------------------------------------------------------ */
 enum
 {
-    GRHD_KERNEL_PRIMS = 0,   /* ghost fill + con2prim over the storage box */
-    GRHD_KERNEL_FACES_X = 1, /* MUSCL-HLLE faces: the three directions run
-                                as one launch or concurrently and are
-                                timed together here                       */
-    GRHD_KERNEL_FACES_Y = 2, /* (no longer used)                          */
-    GRHD_KERNEL_FACES_Z = 3, /* (no longer used)                          */
-    GRHD_KERNEL_CELLS = 4,   /* divergence + sources + stage update       */
-    GRHD_KERNEL_OTHER = 5,
-    GRHD_NUM_KERNEL_CLASSES = 6
+    GRHD_KERNEL_PRIMS = 0, /* ghost fill + con2prim over the storage box */
+    GRHD_KERNEL_FACES = 1, /* MUSCL-HLLE faces: the three directions run as
+                              one launch or concurrently and are timed
+                              together                                   */
+    GRHD_KERNEL_CELLS = 2, /* divergence + sources + stage update         */
+    GRHD_KERNEL_OTHER = 3,
+    GRHD_NUM_KERNEL_CLASSES = 4
 };
 
 /* Enable (1) or disable (0) timing; either resets the accumulators. */
