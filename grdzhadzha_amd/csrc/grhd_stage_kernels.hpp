@@ -39,117 +39,11 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/grhd_abi.h"
+#include "grhd_box_table.hpp"
 #include "grhd_physics.hpp"
 
 namespace grhd
 {
-
-constexpr int kBlock = 256;
-constexpr unsigned long long kNoFail = ~0ull;
-constexpr int kKeyBits = 43; /* 2 x (cells of a level) < 2^43 */
-
-/* Faces per walker of the marching y/z face kernels on a level of `cells`
- * interior cells.  A walk shares each cell's metric and slopes between the
- * two faces it touches, but it is sequential: walks of 8 faces on a 32^3
- * box leave 80 waves for 1024 SIMDs.  Walks get shorter until the level
- * yields at least 2^16 walkers (one wave per SIMD).  Measured per RK4 step
- * (ms, walk length 1 / 2 / 4 / 8): 32^3 0.19 / 0.21 / 0.26 / 0.36,
- * 64^3 0.48 / 0.45 / 0.43 / 0.52, 96^3 1.19 / 1.13 / 1.12 / 1.09; the
- * fluxes do not depend on it. */
-__host__ inline int march_segment(long long cells)
-{
-    int seg = 8;
-    while (seg > 1 && cells / seg < (1ll << 16))
-        seg /= 2;
-    return seg;
-}
-
-/* Device-side status block of one call.  The failure key, which every
- * wavefront of the stage kernels reads, has a 128-byte line of its own,
- * apart from the counters that every block updates with atomics. */
-struct DevStatus
-{
-    /* first RHS recovery failure: (evaluation << 43) | position, where a
-     * box owns positions [2 off_g, 2 (off_g + gnpts)): its ghost-fill
-     * failures first, then its recovery failures, each in BoxIterator
-     * order -- the cell the reference would throw at first */
-    alignas(128) unsigned long long fail_key;
-    alignas(128) unsigned long long cfl_key; /* off_i + interior index */
-    unsigned long long counts[4];
-    unsigned long long max_bits;
-    double sums[5];              /* level conserved sums */
-    unsigned long long nonfinite; /* interior cells with a non-finite value */
-};
-
-/* One box of a level as the kernels see it. */
-struct BoxDev
-{
-    int lo[3], hi[3];
-    int glo[3], ghi[3];
-    int dlo[3], dhi[3];
-    int periodic[3];
-    int gn[3], in[3];
-    long long gnpts, inpts;
-    double dx;
-    double center[3];
-    long long off_u;    /* this box's FAB in the state pool (doubles) */
-    long long off_g;    /* storage cells of the boxes before it */
-    long long off_i;    /* interior cells of the boxes before it */
-    long long off_f[3]; /* faces of direction d of the boxes before it */
-    long long nface[3]; /* faces of direction d of this box */
-    double inv_dx[3];   /* inverse_dx_U: 1 / dx (rounded once on the
-                           host) unless the caller gives its own */
-    int seg;            /* faces per walker of the marching y/z kernels */
-};
-
-/* Kinds of work item, one block table each. */
-enum ItemClass
-{
-    C_ALL = 0, /* storage cells                          */
-    C_EXT = 1, /* storage cells outside the interior     */
-    C_INT = 2, /* interior cells                         */
-    C_FX = 3,  /* x-face lanes: 64 per 63 cells of a row */
-    C_FY = 4,  /* marching y-face walkers                */
-    C_FZ = 5,  /* marching z-face walkers                */
-    C_FALL = 6, /* the C_FX, C_FY and C_FZ tables, in that order */
-    NCLASS = 7
-};
-
-__host__ __device__ inline int face_segments(const BoxDev &b, int d)
-{
-    return (b.in[d] + 1 + b.seg - 1) / b.seg;
-}
-
-__host__ __device__ inline long long fx_cells(const BoxDev &b)
-{
-    return (long long)(b.in[0] + 2) * b.in[1] * b.in[2];
-}
-
-__host__ __device__ inline long long class_items(const BoxDev &b, int c)
-{
-    switch (c)
-    {
-    case C_ALL:
-        return b.gnpts;
-    case C_EXT:
-        return b.gnpts - b.inpts;
-    case C_INT:
-        return b.inpts;
-    case C_FX:
-        return (fx_cells(b) - 1 + 62) / 63 * 64;
-    case C_FY:
-        return (long long)b.in[0] * b.in[2] * face_segments(b, 1);
-    case C_FZ:
-        return (long long)b.in[0] * b.in[1] * face_segments(b, 2);
-    default: /* C_FALL is made of the three face tables */
-        return 0;
-    }
-}
-
-__host__ __device__ inline long long faces(const BoxDev &b, int d)
-{
-    return b.inpts / b.in[d] * (b.in[d] + 1);
-}
 
 /* Pools shared by every box of the level. */
 enum PoolId

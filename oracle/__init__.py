@@ -15,7 +15,10 @@ from grdzhadzha_amd import abi
 from grdzhadzha_amd.options import OperatorOptions
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "libgrhd_oracle.so")
+# GRHD_ORACLE_LIB names another build of the same source (the sanitizer
+# build, oracle/_san/libgrhd_oracle.so, made by `make sanitize`)
+LIB = os.environ.get("GRHD_ORACLE_LIB") or os.path.join(HERE,
+                                                        "libgrhd_oracle.so")
 
 
 def build():
