@@ -422,11 +422,15 @@ def gate_1e6(err):
 
 def bench_mixed(args):
     """Mixed precision on the 256^3 torus (the 128^3 recipe scaled 2x in
-    radius: [-64, 64]^3, R = 24, sigma = 6, seed 4), 200 RK4 steps: FP32
-    reconstruction, face metric and HLLE against the FP64 path (which is
-    bit-exact with the oracle), both on this GPU from the same state.
-    Returns the JSON record: throughput of both, relative L1 / Linf error
-    per conserved variable and the accretion rate through |x| = 2 r_+."""
+    radius: [-64, 64]^3, R = 24, sigma = 6, seed 4), 200 RK4 steps, against
+    the FP64 path (which is bit-exact with the oracle), all on this GPU from
+    the same state:
+    - plain mixed: FP32 reconstruction, face metric and HLLE on every face;
+    - guarded mixed: the same, except FP64 faces beside cells whose shift
+      has |beta| >= args.mixed_shift (grhd_params.mixed_fp64_shift).
+    Returns the JSON record: throughput, relative L1 / Linf error per
+    conserved variable, the accretion rate through |x| = 2 r_+, and the
+    cells whose final recovery mask differs from FP64's."""
     import dataclasses
 
     import numpy as np
@@ -441,10 +445,14 @@ def bench_mixed(args):
                             sigma=3.0 * n / 128, nsteps=steps)
     op64 = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box,
                                                 w.options)
-    o32 = dataclasses.replace(w.options, mixed_precision=True)
-    op32 = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, o32,
-                                                context=op64.ctx)
     op64.ctx.set_autotune(args.autotune)
+    variants = {
+        "mixed": dataclasses.replace(w.options, mixed_precision=True),
+        "guarded": dataclasses.replace(w.options, mixed_precision=True,
+                                       mixed_fp64_shift=args.mixed_shift)}
+    ops = {k: FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, o,
+                                                   context=op64.ctx)
+           for k, o in variants.items()}
     init = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
     cells = w.box.num_interior
     print("bench: mixed precision, %s, %d interior cells, %d steps"
@@ -452,11 +460,13 @@ def bench_mixed(args):
 
     def run(op, k):
         st = init.clone()
+        mask = torch.zeros(w.box.interior_shape, dtype=torch.uint8,
+                           device="cuda")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        op.advance_rk4(st, w.dt, k)
+        op.advance_rk4(st, w.dt, k, mask=mask)
         torch.cuda.synchronize()
-        return st, time.perf_counter() - t0
+        return st, mask, time.perf_counter() - t0
 
     sl = (slice(None),) + w.box.interior_slices()
     names = ("D", "S1", "S2", "S3", "tau")
@@ -466,49 +476,62 @@ def bench_mixed(args):
                             np.abs(b64[c][sl[1:]]).sum())
                 for c, name in enumerate(names)}
 
-    # one step from the same state: the FP32 rounding of a single step,
-    # before branch decisions (resets, floors, atmosphere) can diverge
-    one64, _ = run(op64, 1)
-    one32, _ = run(op32, 1)
-    err_one = l1(one64.cpu().numpy(), one32.cpu().numpy())
-    del one64, one32
-    s64, t64 = run(op64, steps)
-    s32, t32 = run(op32, steps)
-    a64 = s64.cpu().numpy()
-    a32 = s32.cpu().numpy()
-    del s64, s32
-    err = {}
-    smag = float(np.sqrt(sum(a64[c][sl[1:]] ** 2 for c in (1, 2, 3))).sum())
-    for c, name in enumerate(names):
-        ref = a64[c][sl[1:]]
-        diff = np.abs(a32[c][sl[1:]] - ref)
-        err[name] = {"l1_rel": float(diff.sum() / np.abs(ref).sum()),
-                     "linf_rel": float(diff.max() / np.abs(ref).max())}
-        if name.startswith("S"):
-            # normalised by the momentum magnitude sum |S|
-            err[name]["l1_rel_to_abs_S"] = float(diff.sum() / smag)
     spin = w.background.spin
     r_plus = 1.0 + (1.0 - spin * spin) ** 0.5
+    # one step from the same state: the FP32 rounding of a single step,
+    # before branch decisions (resets, floors, atmosphere) can diverge
+    one64 = run(op64, 1)[0].cpu().numpy()
+    s64, m64, t64 = run(op64, steps)
+    a64 = s64.cpu().numpy()
+    del s64
     md64 = workloads.mass_accretion_rate(w.box, a64, 1.0, spin, 2 * r_plus)
-    md32 = workloads.mass_accretion_rate(w.box, a32, 1.0, spin, 2 * r_plus)
-    err["mdot"] = {"fp64": md64, "mixed": md32,
-                   "rel": abs(md32 - md64) / max(abs(md64), 1e-300)}
-    v64, v32 = cells * steps / t64, cells * steps / t32
-    print("bench: FP64 %.3f s (%.3e cell-updates/s), mixed %.3f s (%.3e); "
-          "max L1 rel error %.3e" % (t64, v64, t32, v32,
-                                     max(e["l1_rel"] for k, e in err.items()
-                                         if k != "mdot")),
+    smag = float(np.sqrt(sum(a64[c][sl[1:]] ** 2 for c in (1, 2, 3))).sum())
+    v64 = cells * steps / t64
+    rec = {}
+    for key, op in ops.items():
+        err_one = l1(one64, run(op, 1)[0].cpu().numpy())
+        s32, m32, t32 = run(op, steps)
+        flips = int((m32 != m64).sum())
+        a32 = s32.cpu().numpy()
+        del s32, m32
+        err = {}
+        for c, name in enumerate(names):
+            ref = a64[c][sl[1:]]
+            diff = np.abs(a32[c][sl[1:]] - ref)
+            err[name] = {"l1_rel": float(diff.sum() / np.abs(ref).sum()),
+                         "linf_rel": float(diff.max() / np.abs(ref).max())}
+            if name.startswith("S"):
+                # normalised by the momentum magnitude sum |S|
+                err[name]["l1_rel_to_abs_S"] = float(diff.sum() / smag)
+        md32 = workloads.mass_accretion_rate(w.box, a32, 1.0, spin,
+                                             2 * r_plus)
+        del a32
+        err["mdot"] = {"fp64": md64, "mixed": md32,
+                       "rel": abs(md32 - md64) / max(abs(md64), 1e-300)}
+        v32 = cells * steps / t32
+        print("bench: %s %.3f s (%.3e cell-updates/s, %.3fx FP64); max L1 "
+              "rel error %.3e" % (key, t32, v32, v32 / v64,
+                                  max(e["l1_rel"] for k, e in err.items()
+                                      if k != "mdot")),
+              file=sys.stderr, flush=True)
+        rec[key] = {"value": v32, "speedup_vs_fp64": v32 / v64,
+                    "gate_1e-6": gate_1e6(err),
+                    "ms_per_step": t32 / steps * 1e3,
+                    "error_vs_fp64": err,
+                    "final_mask_cells_differing_from_fp64": flips,
+                    "l1_rel_after_one_step": err_one}
+    print("bench: FP64 %.3f s (%.3e cell-updates/s)" % (t64, v64),
           file=sys.stderr, flush=True)
-    return {"workload": "%s mixed FP32 faces / FP64 recovery+update RK4, "
-                        "%d steps" % (w.name, steps),
-            "interior_cells": cells, "steps": steps,
-            "value": v32, "fp64_value": v64, "speedup_vs_fp64": v32 / v64,
-            "gate_1e-6": gate_1e6(err),
-            "ms_per_step": t32 / steps * 1e3,
-            "fp64_ms_per_step": t64 / steps * 1e3,
-            "error_vs_fp64": err,
-            "l1_rel_after_one_step": err_one,
-            "reference": "the FP64 GPU path, bit-exact with the oracle"}
+    out = {"workload": "%s mixed FP32 faces / FP64 recovery+update RK4, "
+                       "%d steps" % (w.name, steps),
+           "interior_cells": cells, "steps": steps, "fp64_value": v64,
+           "fp64_ms_per_step": t64 / steps * 1e3,
+           "reference": "the FP64 GPU path, bit-exact with the oracle"}
+    out.update(rec["mixed"])
+    out["guarded"] = dict(rec["guarded"],
+                          fp64_faces="beside cells with |beta| >= %g"
+                          % args.mixed_shift)
+    return out
 
 
 def bench_adjoint(args):
@@ -659,6 +682,9 @@ def main():
                     help="skip the mixed-precision record of the default run")
     ap.add_argument("--mixed-n", type=int, default=256)
     ap.add_argument("--mixed-steps", type=int, default=200)
+    ap.add_argument("--mixed-shift", type=float, default=0.1,
+                    help="guarded mixed record: FP64 faces beside cells "
+                         "whose shift has |beta| at least this")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the binary-PN box-set record of the default "
                          "run")

@@ -61,7 +61,9 @@ class GrhdParams(ctypes.Structure):
                 ("mixed_precision", ctypes.c_int),
                 ("excise_horizon", ctypes.c_int),
                 ("excision_inner_buffer", ctypes.c_double),
-                ("excision_outer_buffer", ctypes.c_double)]
+                ("excision_outer_buffer", ctypes.c_double),
+                ("mixed_fp64_margin", ctypes.c_double),
+                ("mixed_fp64_shift", ctypes.c_double)]
 
 
 class GrhdBackground(ctypes.Structure):

@@ -46,6 +46,8 @@ struct DevStatus
     unsigned long long max_bits;
     double sums[5];              /* level conserved sums */
     unsigned long long nonfinite; /* interior cells with a non-finite value */
+    /* entries of the guarded mixed path's FP64 face list (Pools::list) */
+    alignas(128) unsigned long long nlist;
 };
 
 /* One box of a level as the kernels see it. */

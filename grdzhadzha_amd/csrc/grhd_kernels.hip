@@ -54,6 +54,8 @@ Options make_options(const grhd_params *p)
     o.use_static_sources = p->use_static_metric_sources;
     o.use_volume_sources = p->use_metric_volume_time_sources;
     o.mixed = p->mixed_precision != 0;
+    o.guard_margin = o.mixed ? p->mixed_fp64_margin : 0.0;
+    o.guard_shift = o.mixed ? p->mixed_fp64_shift : 0.0;
     o.excise = p->excise_horizon != 0;
     o.excise_inner = p->excision_inner_buffer;
     o.excise_outer = p->excision_outer_buffer;
@@ -123,6 +125,13 @@ int check_box(const grhd_box *b)
     return GRHD_OK;
 }
 
+/* Mixed precision with FP64 faces near floors, the cap or a large shift
+ * (grhd_params::mixed_fp64_margin, mixed_fp64_shift). */
+bool guarded(const Options &o)
+{
+    return o.mixed && (o.guard_margin > 0.0 || o.guard_shift > 0.0);
+}
+
 int check_common(const grhd_params *o, const grhd_background *bg, int ncomp,
                  int cb)
 {
@@ -131,6 +140,12 @@ int check_common(const grhd_params *o, const grhd_background *bg, int ncomp,
     if (cb < 0 || cb + GRHD_NUM_COMPS > ncomp)
         return GRHD_ERR_INVALID_ARGUMENT;
     if (o->max_iterations < 0)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (!(o->mixed_fp64_margin == 0.0 ||
+          (o->mixed_fp64_margin >= 1.0 &&
+           o->mixed_fp64_margin <= 1.0e300)))
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (!(o->mixed_fp64_shift >= 0.0 && o->mixed_fp64_shift <= 1.0e300))
         return GRHD_ERR_INVALID_ARGUMENT;
     if (o->excise_horizon && bg->kind != GRHD_BG_KERR_SCHILD)
         return GRHD_ERR_INVALID_ARGUMENT;
@@ -382,6 +397,8 @@ struct grhd_ctx
     long long cap_g = 0;
     double *F[3] = {nullptr, nullptr, nullptr};
     long long cap_f[3] = {0, 0, 0};
+    int2 *list = nullptr; /* FP64 face list of the guarded mixed path */
+    long long cap_list = 0;
     double *UA = nullptr, *UB = nullptr;
     long long cap_u = 0;
     double *acc = nullptr;
@@ -428,9 +445,16 @@ template <class T> int grow(T **p, long long *cap, long long need)
     return GRHD_OK;
 }
 
-int ensure_scratch(grhd_ctx *c, bool stages)
+int ensure_scratch(grhd_ctx *c, bool stages, const grhd_params *params)
 {
     const Plan &p = c->plan;
+    if (guarded(make_options(params)))
+    {
+        const int rc =
+            grow(&c->list, &c->cap_list, p.tot_f[0] + p.tot_f[1] + p.tot_f[2]);
+        if (rc)
+            return rc;
+    }
     if (c->cap_g < p.tot_g)
     {
         (void)hipFree(c->W);
@@ -584,6 +608,7 @@ Pools make_pools(grhd_ctx *c, double *state, int cb)
     p.wbad = c->wbad;
     for (int d = 0; d < 3; ++d)
         p.F[d] = c->F[d];
+    p.list = c->list;
     p.st = c->dstat;
     return p;
 }
@@ -642,14 +667,15 @@ constexpr long long kFusedFacesCells = 3ll << 20;
  * after everything already queued on s (the ghost fill and recovery of
  * this stage, the previous stage's cell kernel) and joined back into s
  * before the cell kernel. */
-template <class T>
+template <class T, int SEL = FACES_ALL>
 void faces_launches(grhd_ctx *c, const Options &o, const Background &bg,
                     const Pools &p, hipStream_t s, bool fused)
 {
     if (fused)
     {
         if (c->plan.nblocks[C_FALL] > 0)
-            hipLaunchKernelGGL(k_faces_all<T>, dim3(c->plan.nblocks[C_FALL]),
+            hipLaunchKernelGGL((k_faces_all<T, SEL>),
+                               dim3(c->plan.nblocks[C_FALL]),
                                dim3(kBlock), 0, s, c->plan.d_boxes,
                                c->plan.d_items[C_FALL],
                                c->plan.nblocks[C_FX], c->plan.nblocks[C_FY],
@@ -665,14 +691,14 @@ void faces_launches(grhd_ctx *c, const Options &o, const Background &bg,
                             hipSuccess;
     if (!forked)
     {
-        GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
-        GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), s, o, bg, p);
-        GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), s, o, bg, p);
+        GRHD_LAUNCH(c, C_FX, (k_faces_x<T, SEL>), s, o, bg, p);
+        GRHD_LAUNCH(c, C_FY, (k_faces<1, T, SEL>), s, o, bg, p);
+        GRHD_LAUNCH(c, C_FZ, (k_faces<2, T, SEL>), s, o, bg, p);
         return;
     }
-    GRHD_LAUNCH(c, C_FY, (k_faces<1, T>), c->side[0], o, bg, p);
-    GRHD_LAUNCH(c, C_FZ, (k_faces<2, T>), c->side[1], o, bg, p);
-    GRHD_LAUNCH(c, C_FX, (k_faces_x<T>), s, o, bg, p);
+    GRHD_LAUNCH(c, C_FY, (k_faces<1, T, SEL>), c->side[0], o, bg, p);
+    GRHD_LAUNCH(c, C_FZ, (k_faces<2, T, SEL>), c->side[1], o, bg, p);
+    GRHD_LAUNCH(c, C_FX, (k_faces_x<T, SEL>), s, o, bg, p);
     /* join: `s` waits for both side streams; if an event operation fails,
      * the host waits for the side stream instead, so k_cells never runs
      * before the y and z fluxes are written */
@@ -753,32 +779,53 @@ void tune_faces(grhd_ctx *c, const Options &o, const Background &bg,
  * faces less than the two launch boundaries it removes.  A tuning context
  * uses what tune_faces measured instead.  The faces timing class covers
  * all three directions. */
-template <class T>
+template <class T, int SEL = FACES_ALL>
 void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
                     const Pools &p, hipStream_t s)
 {
     Plan &pl = c->plan;
     const int prec = sizeof(T) < sizeof(double) ? 1 : 0;
-    if (c->autotune && !pl.tune_tried[prec])
+    /* the two passes of the guarded path keep the size rules */
+    const bool tunes = c->autotune && SEL == FACES_ALL;
+    if (tunes && !pl.tune_tried[prec])
         tune_faces<T>(c, o, bg, p, s);
     bool fused = pl.tot_i < kFusedFacesCells;
     int k = pl.seg_rule;
-    if (c->autotune && pl.tuned[prec] &&
+    if (tunes && pl.tuned[prec] &&
         build_seg_tables(pl, pl.seg_pick[prec]) == GRHD_OK)
     {
         k = pl.seg_pick[prec];
         fused = pl.fused_pick[prec];
     }
     select_seg(pl, k);
-    faces_launches<T>(c, o, bg, p, s, fused);
+    faces_launches<T, SEL>(c, o, bg, p, s, fused);
     tick(c, GRHD_KERNEL_FACES, s);
 }
 
-/* FP64 faces, or FP32 faces when the options ask for mixed precision. */
+/* Blocks of the grid-stride FP64 face-list kernel: the list's length is
+ * only known on the device, so the grid is sized for the level and idle
+ * blocks leave at once. */
+constexpr long long kListBlocksMax = 4096;
+
+/* Guarded mixed precision: the FP32 pass writes its faces and lists the
+ * others, which k_faces_list computes in FP64; both write the one double
+ * flux pool, disjoint faces each. */
 void launch_faces(grhd_ctx *c, const Options &o, const Background &bg,
                   const Pools &p, hipStream_t s)
 {
-    if (o.mixed)
+    if (guarded(o))
+    {
+        hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, p.st);
+        launch_faces_t<float, FACES_FP32>(c, o, bg, p, s);
+        const Plan &pl = c->plan;
+        const long long nf = pl.tot_f[0] + pl.tot_f[1] + pl.tot_f[2];
+        const long long blocks =
+            std::min(kListBlocksMax, std::max(1ll, (nf + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL(k_faces_list, dim3((unsigned)blocks), dim3(kBlock),
+                           0, s, pl.d_boxes, o, bg, p);
+        tick(c, GRHD_KERNEL_FACES, s);
+    }
+    else if (o.mixed)
         launch_faces_t<float>(c, o, bg, p, s);
     else
         launch_faces_t<double>(c, o, bg, p, s);
@@ -829,7 +876,7 @@ void launch_cells(grhd_ctx *c, int mode, const Options &o,
                   const Background &bp, const Background &bm, const Pools &p,
                   const CellArgs &a, hipStream_t s)
 {
-    if (o.mixed)
+    if (o.mixed && !guarded(o))
         launch_cells_t<float>(c, mode, o, bs, bn, bp, bm, p, a, s);
     else
         launch_cells_t<double>(c, mode, o, bs, bn, bp, bm, p, a, s);
@@ -965,7 +1012,7 @@ int level_rhs(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
     if (rc || (rc = prepare(c, boxes, nbox, ncomp)) ||
-        (rc = ensure_scratch(c, false)))
+        (rc = ensure_scratch(c, false, params)))
         return rc;
     if (exchange && !c->plan.exchange_ok)
         return GRHD_ERR_INVALID_ARGUMENT;
@@ -1019,7 +1066,7 @@ int level_max_inv_dt(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
     if (rc || (rc = prepare(c, boxes, nbox, ncomp)) ||
-        (rc = ensure_scratch(c, false)))
+        (rc = ensure_scratch(c, false, params)))
         return rc;
     hipStream_t s = (hipStream_t)st;
     Pools p = make_pools(c, const_cast<double *>(state), cb);
@@ -1168,7 +1215,7 @@ int level_ssprk2(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
     if (rc || (rc = prepare(c, boxes, nbox, ncomp)) ||
-        (rc = ensure_scratch(c, true)))
+        (rc = ensure_scratch(c, true, params)))
         return rc;
     /* stage states are internal: every ghost must be an outflow ghost or
      * be filled from another box of the level */
@@ -1215,7 +1262,7 @@ int level_rk4(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
     if (rc || (rc = prepare(c, boxes, nbox, ncomp)) ||
-        (rc = ensure_scratch(c, true)))
+        (rc = ensure_scratch(c, true, params)))
         return rc;
     if (!c->plan.all_physical && !(exchange && c->plan.exchange_ok))
         return GRHD_ERR_INVALID_ARGUMENT;
@@ -1291,11 +1338,13 @@ int scratch_on_current_device(unsigned char **out)
  * offsets, 256-aligned). */
 struct NoAllocLayout
 {
-    size_t status, box, items[NCLASS], W, wbad, F[3], UA, UB, acc, total;
+    size_t status, box, items[NCLASS], W, wbad, F[3], list, UA, UB, acc,
+        total;
     int nblocks[NCLASS];
 };
 
-NoAllocLayout noalloc_layout(const BoxDev &b)
+/* with_list: room for the guarded mixed path's FP64 face list */
+NoAllocLayout noalloc_layout(const BoxDev &b, bool with_list)
 {
     NoAllocLayout L;
     size_t at = 0;
@@ -1320,6 +1369,9 @@ NoAllocLayout noalloc_layout(const BoxDev &b)
     L.wbad = take(G);
     for (int d = 0; d < 3; ++d)
         L.F[d] = take(sizeof(double) * 5 * (size_t)b.nface[d]);
+    L.list = take(with_list ? sizeof(int2) * (size_t)(b.nface[0] + b.nface[1] +
+                                                      b.nface[2])
+                            : 0);
     L.UA = take(sizeof(double) * 5 * G);
     L.UB = take(sizeof(double) * 5 * G);
     L.acc = take(sizeof(double) * 5 * I);
@@ -1383,7 +1435,8 @@ int rk4_box_noalloc(const grhd_box *box, const grhd_params *params,
     if (!physical)
         return GRHD_ERR_INVALID_ARGUMENT;
     b.seg = march_segment(tot_i);
-    const NoAllocLayout L = noalloc_layout(b);
+    const bool with_list = guarded(make_options(params));
+    const NoAllocLayout L = noalloc_layout(b, with_list);
     if (L.total > (size_t)GRHD_NOALLOC_SCRATCH_BYTES ||
         tot_i >= kFusedFacesCells)
         return GRHD_ERR_INVALID_ARGUMENT;
@@ -1431,6 +1484,11 @@ int rk4_box_noalloc(const grhd_box *box, const grhd_params *params,
     {
         c.F[d] = reinterpret_cast<double *>(base + L.F[d]);
         c.cap_f[d] = 5 * tot_f[d];
+    }
+    if (with_list)
+    {
+        c.list = reinterpret_cast<int2 *>(base + L.list);
+        c.cap_list = tot_f[0] + tot_f[1] + tot_f[2];
     }
     c.UA = reinterpret_cast<double *>(base + L.UA);
     c.UB = reinterpret_cast<double *>(base + L.UB);
@@ -1598,7 +1656,7 @@ int rk4_adjoint_box(grhd_ctx *c, const grhd_box *box,
     std::memset(&lb, 0, sizeof(lb));
     lb.box = *box;
     if ((rc = prepare(c, &lb, 1, GRHD_NUM_COMPS)) ||
-        (rc = ensure_scratch(c, false)))
+        (rc = ensure_scratch(c, false, params)))
         return rc;
     AdjRun r;
     r.c = c;
@@ -1843,6 +1901,7 @@ void grhd_ctx_destroy(grhd_ctx *c)
     (void)hipFree(c->wbad);
     for (int d = 0; d < 3; ++d)
         (void)hipFree(c->F[d]);
+    (void)hipFree(c->list);
     (void)hipFree(c->UA);
     (void)hipFree(c->UB);
     (void)hipFree(c->acc);

@@ -72,6 +72,10 @@ struct Options
     double theta;
     int use_reconstruction, use_static_sources, use_volume_sources;
     int mixed; /* FP32 reconstruction, face metric and HLLE */
+    /* mixed only: FP64 faces beside cells near a floor or the v^2 cap
+     * (guard_margin > 0), or where the shift is large (guard_shift > 0);
+     * see fp64_cell.  Both 0: every face FP32 */
+    double guard_margin, guard_shift;
     int excise; /* zero rhs inside the buffered Kerr-Schild horizons */
     double excise_inner, excise_outer;
 };
@@ -905,6 +909,34 @@ __device__ __forceinline__ void atmosphere_prim(const Options &o, Prim &p)
     p.pressure = o.pressure_floor;
     p.eps = p.pressure / ((o.gamma - 1.0) * p.rho);
     p.v[0] = p.v[1] = p.v[2] = 0.0;
+}
+
+/* Mixed precision: whether the faces beside a cell are computed in FP64.
+ * - guard_margin m > 0: the recovered primitives lie within a factor m of
+ *   a floor or of the velocity cap, where the floor and reset decisions of
+ *   the next recoveries turn on rounding: rho <= m' rho_floor or
+ *   P <= m' P_floor with m' = max(m, the metric-source atmosphere factor),
+ *   or 1 - v^2 <= m (1 - max_v2);
+ * - guard_shift s > 0: the shift is at least s in size,
+ *   gamma_ij beta^i beta^j >= s^2.  In horizon-penetrating coordinates
+ *   infalling gas moves with the shift, so the flux speed alpha v^i -
+ *   beta^i is a small difference of large terms there, which FP32 loses. */
+__device__ __forceinline__ bool fp64_cell(const Prim &w, const Options &o,
+                                          const Geo &geo)
+{
+    bool near = false;
+    const double m = o.guard_margin;
+    if (m > 0.0)
+    {
+        const double mf = std_max(m, o.source_atmosphere_factor);
+        near = w.rho <= mf * o.rho_floor ||
+               w.pressure <= mf * o.pressure_floor ||
+               1.0 - vsq(w.v, geo.g) <= m * (1.0 - o.max_v2);
+    }
+    const double s = o.guard_shift;
+    if (s > 0.0 && vsq(geo.shift, geo.g) >= s * s)
+        near = true;
+    return near;
 }
 
 /* FArrayBoxFiniteVolume.hpp:118-126 */

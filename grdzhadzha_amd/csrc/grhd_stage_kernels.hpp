@@ -66,7 +66,32 @@ struct Pools
     int rhs_ncomp;
     unsigned char *mask;
     DevStatus *st;
+    /* guarded mixed path: the faces left to FP64, one entry each
+     * ((direction << 30) | box, face index), as many as DevStatus::nlist */
+    int2 *list;
 };
+
+/* Per storage cell flags beside W (Pools::wbad): kWFailed when the cell's
+ * recovery failed (W then holds the residual), kWGuard when, on the
+ * guarded mixed path, the faces beside the cell are computed in FP64. */
+constexpr unsigned char kWFailed = 1;
+constexpr unsigned char kWGuard = 2;
+
+__device__ __forceinline__ bool guarded_mixed(const Options &o)
+{
+    return o.guard_margin > 0.0 || o.guard_shift > 0.0;
+}
+
+__device__ __forceinline__ unsigned char failed_flags(const Options &o)
+{
+    return guarded_mixed(o) ? (unsigned char)(kWFailed | kWGuard) : kWFailed;
+}
+
+__device__ __forceinline__ unsigned char
+recovered_flags(const Prim &w, const Options &o, const Geo &geo)
+{
+    return (guarded_mixed(o) && fp64_cell(w, o, geo)) ? kWGuard : 0;
+}
 
 __device__ __forceinline__ double *pool_cons(const Pools &p, int which,
                                              const BoxDev &b)
@@ -365,7 +390,7 @@ __global__ void __launch_bounds__(kBlock)
         if (EXTERIOR && src_interior)
         {
             /* con2prim of the source was done by the previous stage */
-            src_ok = wbad[so] == 0;
+            src_ok = (wbad[so] & kWFailed) == 0;
             if (src_ok)
                 pr = load_prim(W, n, so);
             else
@@ -386,7 +411,7 @@ __global__ void __launch_bounds__(kBlock)
             {
                 atomicMin(&p.st->fail_key, fail_key(eval, 0, b, idx));
                 W[idx] = res;
-                wbad[idx] = 1;
+                wbad[idx] = failed_flags(o);
             }
             return;
         }
@@ -412,11 +437,11 @@ __global__ void __launch_bounds__(kBlock)
     {
         atomicMin(&p.st->fail_key, fail_key(eval, 1, b, idx));
         W[idx] = res;
-        wbad[idx] = 1;
+        wbad[idx] = failed_flags(o);
         return;
     }
     store_prim(W, n, idx, w);
-    wbad[idx] = 0;
+    wbad[idx] = recovered_flags(w, o, gt);
 }
 
 /* The face kernels compute in T: double on the bit-exact path, float on
@@ -482,15 +507,103 @@ __device__ __forceinline__ void slope_step(const PrimT<T> &l,
     }
 }
 
-template <class T>
-__device__ __forceinline__ void store_flux(T *F, long long nf, long long fo,
+template <class T, class O>
+__device__ __forceinline__ void store_flux(O *F, long long nf, long long fo,
                                            const ConsT<T> &f)
 {
-    F[fo] = f.D;
-    F[nf + fo] = f.S[0];
-    F[2 * nf + fo] = f.S[1];
-    F[3 * nf + fo] = f.S[2];
-    F[4 * nf + fo] = f.tau;
+    F[fo] = O(f.D);
+    F[nf + fo] = O(f.S[0]);
+    F[2 * nf + fo] = O(f.S[1]);
+    F[3 * nf + fo] = O(f.S[2]);
+    F[4 * nf + fo] = O(f.tau);
+}
+
+/* Which faces a face kernel writes, and as what.  FACES_ALL: every face,
+ * stored as the compute type.  FACES_FP32, the first pass of the guarded
+ * mixed path: the faces neither of whose cells carries kWGuard, stored as
+ * double; the others go to the FP64 face list, which k_faces_list then
+ * computes into the same double pool.  k_cells<double> reads that pool as
+ * on the FP64 path. */
+enum FaceSel
+{
+    FACES_ALL = 0,
+    FACES_FP32 = 1
+};
+
+template <class T, int SEL> struct FluxOut
+{
+    using type = double;
+};
+template <class T> struct FluxOut<T, FACES_ALL>
+{
+    using type = T;
+};
+
+/* Whether the face between two cells with these flags is computed in
+ * FP64 on the guarded path. */
+__device__ __forceinline__ bool fp64_face(int left, int right)
+{
+    return ((left | right) & kWGuard) != 0;
+}
+
+__device__ __forceinline__ int2 list_entry(int box, int d, long long fo)
+{
+    return make_int2(box | (d << 30), (int)fo);
+}
+
+/* Append one entry per lane that takes one: a single atomic per
+ * wavefront, entries in lane order from there.  Every lane of the
+ * wavefront must reach it. */
+__device__ __forceinline__ void list_append_lane(const Pools &p, bool take,
+                                                 int2 entry)
+{
+    const unsigned long long bal = __ballot(take);
+    if (bal == 0ull)
+        return;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)bal) - 1;
+    unsigned long long base = 0ull;
+    if (lane == leader)
+        base = atomicAdd(&p.st->nlist, (unsigned long long)__popcll(bal));
+    base = __shfl(base, leader, 64);
+    if (take)
+        p.list[base + __popcll(bal & ((1ull << lane) - 1ull))] = entry;
+}
+
+/* Append the faces of a walk whose bits are set (bit q: face fo0 +
+ * q f_stride): one atomic per wavefront, after a scan of the counts.
+ * Every lane of the wavefront must reach it. */
+__device__ __forceinline__ void list_append_walk(const Pools &p,
+                                                 unsigned bits, int box,
+                                                 int d, long long fo0,
+                                                 long long f_stride)
+{
+    if (!__any(bits != 0u))
+        return;
+    const int lane = (int)__lane_id();
+    const int cnt = __popc(bits);
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += y;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (total == 0)
+        return;
+    unsigned long long base = 0ull;
+    if (lane == 63)
+        base = atomicAdd(&p.st->nlist, (unsigned long long)total);
+    base = __shfl(base, 63, 64);
+    unsigned long long at = base + (unsigned long long)(incl - cnt);
+    while (bits)
+    {
+        const int q = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        p.list[at++] = list_entry(box, d, fo0 + q * f_stride);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -506,12 +619,13 @@ template <class T> __device__ __forceinline__ T shfl_next(T v)
     return __shfl_down(v, 1, 64);
 }
 
-template <class T>
+template <class T, int SEL = FACES_ALL>
 __device__ __forceinline__ void faces_x_block(const BoxDev *__restrict__ boxes,
                                               const int2 bt, const Options &o,
                                               const Background &bg,
                                               const Pools &p)
 {
+    using O = typename FluxOut<T, SEL>::type;
     const BoxDev &b = boxes[bt.x];
     const int lane = threadIdx.x & 63;
     const long long wave = ((long long)bt.y + threadIdx.x) >> 6;
@@ -528,6 +642,27 @@ __device__ __forceinline__ void faces_x_block(const BoxDev *__restrict__ boxes,
     const long long n = b.gnpts;
     const double *W = p.W + 6 * b.off_g;
     const long long oc = goff(b, i, j, k);
+    const bool makes_face =
+        live && lane != 63 && x != E - 1 && e + 1 < next;
+    const long long fo =
+        ((long long)(k - b.lo[2]) * b.in[1] + (j - b.lo[1])) * (b.in[0] + 1) +
+        x;
+    bool mine = makes_face;
+    if (SEL == FACES_FP32)
+    {
+        /* FP64 faces go to the list; a wavefront with none of its own
+         * leaves at once (the exit is uniform, so the shuffles below see
+         * whole wavefronts) */
+        const int fc = p.wbad[b.off_g + oc];
+        /* every lane shuffles (a lane that skipped it would leave its
+         * neighbour an undefined value) */
+        const int fn = shfl_next(fc);
+        const bool to64 = makes_face && fp64_face(fc, fn);
+        list_append_lane(p, to64, list_entry(bt.x, 0, fo));
+        mine = makes_face && !to64;
+        if (!__any(mine))
+            return;
+    }
     const PrimT<T> wc = load_prim_as<T>(W, n, oc);
     PrimT<T> mh = wc, ph = wc;
     if (o.use_reconstruction)
@@ -562,25 +697,21 @@ __device__ __forceinline__ void faces_x_block(const BoxDev *__restrict__ boxes,
     for (int m = 0; m < 3; ++m)
         gn.shift[m] = shfl_next(gc.shift[m]);
 
-    if (!live || lane == 63 || x == E - 1 || e + 1 >= next)
+    if (!mine)
         return;
     GeoT<T> face;
     average_geo(gc, gn, face);
     ConsT<T> f;
     hlle_face(ph, rp, o, face, 0, f);
-    const long long nf = b.nface[0];
-    const long long fo =
-        ((long long)(k - b.lo[2]) * b.in[1] + (j - b.lo[1])) * (b.in[0] + 1) +
-        x;
-    store_flux(face_pool<T>(p, b, 0), nf, fo, f);
+    store_flux(face_pool<O>(p, b, 0), b.nface[0], fo, f);
 }
 
-template <class T>
+template <class T, int SEL = FACES_ALL>
 __global__ void __launch_bounds__(kBlock)
     k_faces_x(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
               Background bg, Pools p)
 {
-    faces_x_block<T>(boxes, items[blockIdx.x], o, bg, p);
+    faces_x_block<T, SEL>(boxes, items[blockIdx.x], o, bg, p);
 }
 
 /* ------------------------------------------------------------------ */
@@ -589,15 +720,19 @@ __global__ void __launch_bounds__(kBlock)
 /* state and the right cell's metric, so a cell's slope and metric are */
 /* computed once per walk; lanes are consecutive along i.              */
 /* ------------------------------------------------------------------ */
-template <int D, class T>
+template <int D, class T, int SEL = FACES_ALL>
 __device__ __forceinline__ void faces_march_block(
     const BoxDev *__restrict__ boxes, const int2 bt, const Options &o,
     const Background &bg, const Pools &p)
 {
+    using O = typename FluxOut<T, SEL>::type;
     const BoxDev &b = boxes[bt.x];
-    const long long it = (long long)bt.y + threadIdx.x;
-    if (it >= class_items(b, D == 1 ? C_FY : C_FZ))
+    const long long it0 = (long long)bt.y + threadIdx.x;
+    /* the FP32 pass keeps every lane to the list append below */
+    const bool valid = it0 < class_items(b, D == 1 ? C_FY : C_FZ);
+    if (SEL == FACES_ALL && !valid)
         return;
+    const long long it = valid ? it0 : 0;
     constexpr int B = (D == 2) ? 1 : 2; /* the other slow axis */
     const int na = b.in[0], nb = b.in[B];
     const int ia = (int)(it % na);
@@ -621,8 +756,32 @@ __device__ __forceinline__ void faces_march_block(
     long long fo = ((long long)(c[2] - b.lo[2]) * fn1 + (c[1] - b.lo[1])) *
                        fn0 + (c[0] - b.lo[0]);
     const long long nf = b.nface[D];
-    T *F = face_pool<T>(p, b, D);
+    O *F = face_pool<O>(p, b, D);
     const T theta = T(o.theta);
+    /* bit q - p0: face q is written here (every face when SEL is
+     * FACES_ALL); the FP32 pass lists its FP64 faces, and a walk with none
+     * of its own leaves at once */
+    unsigned mine = ~0u;
+    if (SEL == FACES_FP32)
+    {
+        unsigned to64 = 0u;
+        if (valid)
+        {
+            const unsigned char *fl = p.wbad + b.off_g;
+            int left = fl[oR - stride];
+            for (int q = p0; q < p1; ++q)
+            {
+                const int right = fl[oR + (long long)(q - p0) * stride];
+                if (fp64_face(left, right))
+                    to64 |= 1u << (q - p0);
+                left = right;
+            }
+        }
+        list_append_walk(p, to64, bt.x, D, fo, f_stride);
+        mine = valid ? ~to64 & ((1u << (p1 - p0)) - 1u) : 0u;
+        if (mine == 0u)
+            return;
+    }
 
     PrimT<T> wL = load_prim_as<T>(W, n, oR - stride);
     PrimT<T> wR = load_prim_as<T>(W, n, oR);
@@ -654,7 +813,8 @@ __device__ __forceinline__ void faces_march_block(
         average_geo(gL, gR, face);
         ConsT<T> f;
         hlle_face(lp, rp, o, face, D, f);
-        store_flux(F, nf, fo, f);
+        if (SEL == FACES_ALL || ((mine >> (q - p0)) & 1u))
+            store_flux(F, nf, fo, f);
         if (q + 1 < p1)
         {
             lp = lp_next;
@@ -668,12 +828,12 @@ __device__ __forceinline__ void faces_march_block(
     }
 }
 
-template <int D, class T>
+template <int D, class T, int SEL = FACES_ALL>
 __global__ void __launch_bounds__(kBlock)
     k_faces(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o, Background bg,
             Pools p)
 {
-    faces_march_block<D, T>(boxes, items[blockIdx.x], o, bg, p);
+    faces_march_block<D, T, SEL>(boxes, items[blockIdx.x], o, bg, p);
 }
 
 /* All three face directions of a call in one launch, for levels too small
@@ -683,18 +843,94 @@ __global__ void __launch_bounds__(kBlock)
  * registers of the marching kernels, which costs the x blocks occupancy;
  * on small levels that matters less than the two launch boundaries it
  * removes. */
-template <class T>
+template <class T, int SEL = FACES_ALL>
 __global__ void __launch_bounds__(kBlock)
     k_faces_all(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, int nbx, int nby,
                 Options o, Background bg, Pools p)
 {
     const int2 bt = items[blockIdx.x];
     if ((int)blockIdx.x < nbx)
-        faces_x_block<T>(boxes, bt, o, bg, p);
+        faces_x_block<T, SEL>(boxes, bt, o, bg, p);
     else if ((int)blockIdx.x < nbx + nby)
-        faces_march_block<1, T>(boxes, bt, o, bg, p);
+        faces_march_block<1, T, SEL>(boxes, bt, o, bg, p);
     else
-        faces_march_block<2, T>(boxes, bt, o, bg, p);
+        faces_march_block<2, T, SEL>(boxes, bt, o, bg, p);
+}
+
+/* The FP64 faces of the guarded mixed path, one face per list entry: the
+ * same loads, slopes, metric evaluations, average and HLLE as a walk of
+ * one face (or the x kernel's lane pair), so each flux equals the FP64
+ * path's bit for bit for the same W. */
+template <int D>
+__device__ __forceinline__ void face_one(const BoxDev &b, long long fo,
+                                         const Options &o,
+                                         const Background &bg,
+                                         const Pools &p)
+{
+    const int fn0 = b.in[0] + (D == 0), fn1 = b.in[1] + (D == 1);
+    int c[3]; /* the face's right cell */
+    c[0] = b.lo[0] + (int)(fo % fn0);
+    c[1] = b.lo[1] + (int)((fo / fn0) % fn1);
+    c[2] = b.lo[2] + (int)(fo / ((long long)fn0 * fn1));
+    const long long n = b.gnpts;
+    const double *W = p.W + 6 * b.off_g;
+    const long long stride =
+        (D == 0) ? 1ll
+                 : ((D == 1) ? (long long)b.gn[0]
+                             : (long long)b.gn[0] * b.gn[1]);
+    const long long oR = goff(b, c[0], c[1], c[2]);
+    const PrimT<double> wL = load_prim_as<double>(W, n, oR - stride);
+    const PrimT<double> wR = load_prim_as<double>(W, n, oR);
+    PrimT<double> lp = wL, rp = wR;
+    if (o.use_reconstruction)
+    {
+        const PrimT<double> wLL = load_prim_as<double>(W, n, oR - 2 * stride);
+        const PrimT<double> wRR = load_prim_as<double>(W, n, oR + stride);
+        PrimT<double> unused;
+        slope_step(wLL, wL, wR, o.theta, unused, lp);
+        slope_step(wL, wR, wRR, o.theta, rp, unused);
+    }
+    double x[3];
+    GeoT<double> gL, gR, face;
+    c[D] -= 1;
+    coords_as(c[0], c[1], c[2], b, x);
+    cell_geometry(bg, x, gL);
+    c[D] += 1;
+    coords_as(c[0], c[1], c[2], b, x);
+    cell_geometry(bg, x, gR);
+    average_geo(gL, gR, face);
+    ConsT<double> f;
+    hlle_face(lp, rp, o, face, D, f);
+    store_flux(face_pool<double>(p, b, D), b.nface[D], fo, f);
+}
+
+/* Grid-stride over the FP64 face list (DevStatus::nlist entries). */
+__global__ void __launch_bounds__(kBlock)
+    k_faces_list(const BoxDev *__restrict__ boxes, Options o, Background bg,
+                 Pools p)
+{
+    const unsigned long long cnt = p.st->nlist;
+    for (unsigned long long q = (unsigned long long)blockIdx.x * kBlock +
+                                threadIdx.x;
+         q < cnt; q += (unsigned long long)gridDim.x * kBlock)
+    {
+        const int2 e = p.list[q];
+        const int d = (int)((unsigned)e.x >> 30);
+        const BoxDev &b = boxes[e.x & ((1 << 30) - 1)];
+        if (d == 0)
+            face_one<0>(b, e.y, o, bg, p);
+        else if (d == 1)
+            face_one<1>(b, e.y, o, bg, p);
+        else
+            face_one<2>(b, e.y, o, bg, p);
+    }
+}
+
+/* Empty the FP64 face list before a stage's FP32 pass. */
+__global__ void k_list_reset(DevStatus *st)
+{
+    if (threadIdx.x == 0)
+        st->nlist = 0ull;
 }
 
 enum CellMode
@@ -960,13 +1196,13 @@ __global__ void __launch_bounds__(kBlock)
                 if (wok)
                 {
                     store_prim(W, n, so, wn);
-                    wbad[so] = 0;
+                    wbad[so] = recovered_flags(wn, o, g);
                 }
                 else
                 {
                     atomicMin(&p.st->fail_key, fail_key(a.next_eval, 1, b, so));
                     W[so] = res;
-                    wbad[so] = 1;
+                    wbad[so] = failed_flags(o);
                 }
                 if (a.write_mask)
                     p.mask[b.off_i + it] = m;

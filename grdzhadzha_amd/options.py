@@ -82,6 +82,15 @@ class OperatorOptions:
     # no counterpart in the reference: FP32 reconstruction, face metric and
     # HLLE, everything else FP64 (False: the bit-exact FP64 path)
     mixed_precision: bool = False
+    # no counterpart in the reference: with mixed_precision, a margin m >= 1
+    # computes in FP64 the faces beside cells with rho <= m rho_floor,
+    # P <= m pressure_floor, 1 - v^2 <= m (1 - max_v2) or a failed recovery
+    # (0: every face FP32)
+    mixed_fp64_margin: float = 0.0
+    # no counterpart in the reference: with mixed_precision, s > 0 also
+    # computes in FP64 the faces beside cells whose shift has
+    # gamma_ij beta^i beta^j >= s^2 (0: no shift test)
+    mixed_fp64_shift: float = 0.0
     # KerrSchild::check_if_excised (KerrSchild.hpp:247-280) with
     # ExcisionTest's zero right-hand side (ExcisionTest.hpp:40-63) inside
     # the buffered horizons; Kerr-Schild backgrounds only
@@ -111,6 +120,8 @@ class OperatorOptions:
         p.excise_horizon = int(self.excise_horizon)
         p.excision_inner_buffer = self.excision_inner_buffer
         p.excision_outer_buffer = self.excision_outer_buffer
+        p.mixed_fp64_margin = self.mixed_fp64_margin
+        p.mixed_fp64_shift = self.mixed_fp64_shift
         return p
 
 

@@ -149,6 +149,28 @@ typedef struct grhd_params
     int excise_horizon;
     double excision_inner_buffer; /* the reference's default: 1.05 */
     double excision_outer_buffer; /* the reference's default: 0.9 */
+    /* No counterpart in the reference.  With mixed_precision, a margin
+     * m >= 1 computes in FP64 every face adjacent to a cell whose recovered
+     * primitives lie within a factor m of a floor or of the velocity cap:
+     * rho <= m' rho_floor or P <= m' pressure_floor, with m' the larger of
+     * m and metric_source_atmosphere_factor (so the metric-source
+     * atmosphere test is covered), or 1 - v^2 <= m (1 -
+     * max_velocity_squared); faces of cells whose recovery failed are FP64
+     * as well.  The other faces stay FP32.  Such cells sit
+     * on the floor and cap thresholds, where a rounding difference flips a
+     * reset decision; with their faces in FP64, cells that only such cells
+     * reach evolve exactly as on the FP64 path.  Fluxes are then stored as
+     * double.  Zero (the default) keeps every face in FP32; other values
+     * below 1, and non-finite ones, are invalid. */
+    double mixed_fp64_margin;
+    /* No counterpart in the reference.  With mixed_precision, s > 0 also
+     * computes in FP64 every face adjacent to a cell whose shift has
+     * gamma_ij beta^i beta^j >= s^2: in horizon-penetrating coordinates
+     * infalling gas moves with the shift, and the flux speed
+     * alpha v^i - beta^i is a small difference of large terms that FP32
+     * loses.  Zero (the default) disables the test; negative or non-finite
+     * values are invalid. */
+    double mixed_fp64_shift;
 } grhd_params;
 
 /* Analytic background.  For the binary, the orbital time at a call is

@@ -415,6 +415,21 @@ class LevelOperatorBase
      * default, which is the bit-exact FP64 path. */
     void set_mixed_precision(bool on) { m_params.mixed_precision = on ? 1 : 0; }
 
+    /* With mixed precision: FP64 faces beside cells within a factor
+     * `margin` of a floor or the velocity cap (see grhd_params); 0 keeps
+     * every face FP32. */
+    void set_mixed_fp64_margin(double margin)
+    {
+        m_params.mixed_fp64_margin = margin;
+    }
+
+    /* With mixed precision: FP64 faces beside cells whose shift is at
+     * least `shift` in size (see grhd_params); 0 disables the test. */
+    void set_mixed_fp64_shift(double shift)
+    {
+        m_params.mixed_fp64_shift = shift;
+    }
+
     /* KerrSchild::check_if_excised with ExcisionTest's zero right-hand side
      * (see grhd_params); Kerr-Schild backgrounds only. */
     void set_horizon_excision(bool on, double inner_buffer = 1.05,

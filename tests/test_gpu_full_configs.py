@@ -8,8 +8,10 @@ C3  3 levels x 180 seeded binary-PN boxes (sides 8 .. 64): one RK4 step
 C4  the 256^3 Kerr a = 0.9 torus: one FP64 RK4 step equals the oracle bit
     for bit (masks and counts included); one mixed-precision step is
     within 1e-8 relative L1 of FP64; the 200-step mixed gate (1e-6 per
-    variable) is its own test, expected to fail until the mixed path holds
-    it (the measured shortfall is in its reason).
+    variable) holds with FP64 faces where the shift has |beta| >= 0.1
+    (guarded mixed precision), and the accretion rate through 2 r_+ is
+    within 1e-6.  Plain mixed precision (every face FP32) misses the gate;
+    that run is kept as an expected failure with its measured numbers.
 C5  64^3 Kerr a = 0.5 Michel-like inflow, 200 RK4 steps, J over the last
     50 through r = 4M, checkpoints every 20: the adjoint against central
     differences (h = 1e-6) of the GPU forward along 4 smooth rho0
@@ -123,26 +125,54 @@ def test_c4_fp64_step_equals_oracle_and_mixed_step_within_1e8(hip_context):
         assert e <= 1e-8
 
 
-@pytest.mark.xfail(strict=False, reason=(
-    "measured on one MI355X, 200 steps: relative L1 D 3.5e-7, S1 8.3e-7, "
-    "S2 8.2e-7, S3 3.7e-6, tau 1.7e-6; reset and floor decisions that come "
-    "out differently in FP32 faces carry the error past 1e-6"))
-def test_c4_mixed_200_steps_within_1e6(hip_context):
+def c4_200_steps(ctx, *variants):
+    """FP64 and each mixed variant over 200 steps of the C4 torus: relative
+    L1 per variable, and the relative error of the accretion rate through
+    2 r_+, against FP64."""
     w = c4_workload(200)
-    o32 = dataclasses.replace(w.options, mixed_precision=True)
-    ops = [FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, o,
-                                                context=hip_context)
-           for o in (w.options, o32)]
+    bg = w.background
+    r_plus = bg.mass + (bg.mass ** 2 - bg.spin ** 2) ** 0.5
+    sl = w.box.interior_slices()
     out = []
-    for op in ops:
+    for o in (w.options,) + variants:
+        op = FixedBGLevelDataFiniteVolumeOperator(bg, w.box, o, context=ctx)
         st = torch.from_numpy(np.ascontiguousarray(w.state)).cuda()
         op.advance_rk4(st, w.dt, 200)
         out.append(st.cpu().numpy())
         del st
-    sl = w.box.interior_slices()
-    errs = [float(np.abs(out[1][c][sl] - out[0][c][sl]).sum() /
-                  np.abs(out[0][c][sl]).sum()) for c in range(5)]
-    print("C4 mixed after 200 steps, L1 rel per variable:", errs, flush=True)
+    mdot = [workloads.mass_accretion_rate(w.box, s, bg.mass, bg.spin,
+                                          2.0 * r_plus) for s in out]
+    res = []
+    for s, md in zip(out[1:], mdot[1:]):
+        errs = [float(np.abs(s[c][sl] - out[0][c][sl]).sum() /
+                      np.abs(out[0][c][sl]).sum()) for c in range(5)]
+        res.append((errs, rel(md, mdot[0])))
+    return res
+
+
+def test_c4_guarded_mixed_200_steps_within_1e6(hip_context):
+    w = c4_workload()
+    og = dataclasses.replace(w.options, mixed_precision=True,
+                             mixed_fp64_shift=0.1)
+    [(errs, mdot_err)] = c4_200_steps(hip_context, og)
+    print("C4 guarded mixed (|beta| >= 0.1 FP64) after 200 steps, L1 rel "
+          "per variable:", errs, "accretion rate rel err %.2e" % mdot_err,
+          flush=True)
+    assert max(errs) <= 1e-6, errs
+    assert mdot_err <= 1e-6
+
+
+@pytest.mark.xfail(strict=False, reason=(
+    "measured on one MI355X, 200 steps, every face FP32: relative L1 D "
+    "3.5e-7, S1 8.3e-7, S2 8.2e-7, S3 3.7e-6, tau 1.7e-6, accretion rate "
+    "2.0e-4; nine tenths of the error sits in the inflow at r < 12 M, "
+    "where alpha v - beta is a small difference of large terms"))
+def test_c4_mixed_200_steps_within_1e6(hip_context):
+    w = c4_workload()
+    o32 = dataclasses.replace(w.options, mixed_precision=True)
+    [(errs, mdot_err)] = c4_200_steps(hip_context, o32)
+    print("C4 mixed after 200 steps, L1 rel per variable:", errs,
+          "accretion rate rel err %.2e" % mdot_err, flush=True)
     assert max(errs) <= 1e-6, errs
 
 

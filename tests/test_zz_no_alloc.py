@@ -8,6 +8,7 @@ scratch exceeds GRHD_NOALLOC_SCRATCH_BYTES, or that is not its own
 non-periodic domain, is rejected with GRHD_ERR_INVALID_ARGUMENT.
 """
 import ctypes
+import dataclasses
 
 import numpy as np
 import pytest
@@ -36,11 +37,21 @@ def noalloc_rk4(w, state, nsteps, mask):
     return rc, diag.as_dict(), err
 
 
-@pytest.mark.parametrize("first,second", [(48, 16), (32, 8)])
-def test_noalloc_rk4_equals_context_rk4(hip_context, first, second):
+@pytest.mark.parametrize("first,second,guarded", [(48, 16, False),
+                                                  (32, 8, False),
+                                                  (48, 16, True)])
+def test_noalloc_rk4_equals_context_rk4(hip_context, first, second,
+                                        guarded):
+    """guarded: mixed precision with FP64 faces near floors and where the
+    shift is large, whose face list also lives in the static array."""
     for n, nsteps in ((first, 2), (second, 3)):
         w = workloads.torus_box(n=n, R0=0.3 * n, sigma=0.08 * n, nsteps=1)
-        print("noalloc case n", n, "steps", nsteps, flush=True)
+        if guarded:
+            w.options = dataclasses.replace(w.options, mixed_precision=True,
+                                            mixed_fp64_margin=2.0,
+                                            mixed_fp64_shift=0.2)
+        print("noalloc case n", n, "steps", nsteps, "guarded", guarded,
+              flush=True)
         op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box,
                                                   w.options,
                                                   context=hip_context)
