@@ -351,10 +351,23 @@ def bench_batched(args):
     p64, _ = run_levels(ops, steps)
     mixed = cells * steps / tm
     err_m = level_errors(p64, p32)
-    del p32, p64
+    del p32
+    # guarded mixed: FP64 faces beside cells whose shift has |beta| >=
+    # args.batched_shift (the binary's shift is small: the default 0.05
+    # covers the neighbourhood of each mass)
+    og = dataclasses.replace(opts, mixed_precision=True,
+                             mixed_fp64_shift=args.batched_shift)
+    opsg = [LevelOperator(bg, boxes, og, time_dependent=True, context=ctx)
+            for boxes, _, _ in levels]
+    run_levels(opsg, 1)
+    pg, tg = run_levels(opsg, steps)
+    guarded = cells * steps / tg
+    err_g = level_errors(p64, pg)
+    del pg, p64
     print("bench: batched mixed %.3f s (%.3e cell-updates/s), max L1 rel "
-          "error %.3e" % (tm, mixed, max(e["l1_rel"] for e in
-                                         err_m.values())),
+          "error %.3e; guarded %.3f s (%.3e), max L1 rel error %.3e"
+          % (tm, mixed, max(e["l1_rel"] for e in err_m.values()), tg,
+             guarded, max(e["l1_rel"] for e in err_g.values())),
           file=sys.stderr, flush=True)
 
     # plain path: one call per box, the same boxes and steps; each box has
@@ -405,7 +418,14 @@ def bench_batched(args):
                 "speedup_vs_batched_fp64": mixed / batched,
                 "error_vs_batched_fp64": err_m,
                 "gate_1e-6": gate_1e6(err_m),
-                "l1_rel_after_one_step": err_one}}
+                "l1_rel_after_one_step": err_one},
+            "batched_guarded_mixed": {
+                "fp64_faces": "beside cells with |beta| >= %g"
+                              % args.batched_shift,
+                "value": guarded, "ms_per_step": tg / steps * 1e3,
+                "speedup_vs_batched_fp64": guarded / batched,
+                "error_vs_batched_fp64": err_g,
+                "gate_1e-6": gate_1e6(err_g)}}
 
 
 def gate_1e6(err):
@@ -682,6 +702,9 @@ def main():
                     help="skip the mixed-precision record of the default run")
     ap.add_argument("--mixed-n", type=int, default=256)
     ap.add_argument("--mixed-steps", type=int, default=200)
+    ap.add_argument("--batched-shift", type=float, default=0.05,
+                    help="guarded batched mixed record: FP64 faces beside "
+                         "cells whose shift has |beta| at least this")
     ap.add_argument("--mixed-shift", type=float, default=0.1,
                     help="guarded mixed record: FP64 faces beside cells "
                          "whose shift has |beta| at least this")

@@ -4,7 +4,9 @@ the same code paths on shapes the oracle finishes in a moment).
 
 C3  3 levels x 180 seeded binary-PN boxes (sides 8 .. 64): one RK4 step
     of every level in one level call equals every box stepped alone bit
-    for bit, and the oracle on the largest box of each level.
+    for bit, and the oracle on the largest box of each level; after 50
+    steps guarded mixed precision (FP64 faces where |beta| >= 0.05) is
+    within 1e-6 relative L1 of batched FP64 on every variable.
 C4  the 256^3 Kerr a = 0.9 torus: one FP64 RK4 step equals the oracle bit
     for bit (masks and counts included); one mixed-precision step is
     within 1e-8 relative L1 of FP64; the 200-step mixed gate (1e-6 per
@@ -82,6 +84,32 @@ def test_c3_full_box_set_level_step_equals_boxes_and_oracle(hip_context):
         assert_bitwise(got[big][sl], ref[sl], "C3 level %d box %d" %
                        (lev, big))
         assert np.array_equal(masks[big], mref)
+
+
+def test_c3_guarded_mixed_50_steps_within_1e6(hip_context):
+    bg, opts, levels, _ = workloads.binary_pn_levels()
+    og = dataclasses.replace(opts, mixed_precision=True,
+                             mixed_fp64_shift=0.05)
+    num = np.zeros(5)
+    den = np.zeros(5)
+    for boxes, fabs, dt in levels:
+        res = []
+        for o in (opts, og):
+            L = LevelOperator(bg, boxes, o, time_dependent=True,
+                              context=hip_context)
+            pool = L.pack(fabs)
+            L.advance_rk4(pool, dt, nsteps=50, time=0.0)
+            res.append(L.unpack(pool))
+            del pool
+        for b, f64, fg in zip(boxes, *res):
+            sl = b.interior_slices()
+            for c in range(5):
+                num[c] += np.abs(fg[c][sl] - f64[c][sl]).sum()
+                den[c] += np.abs(f64[c][sl]).sum()
+    errs = (num / den).tolist()
+    print("C3 guarded mixed after 50 steps, L1 rel per variable:", errs,
+          flush=True)
+    assert max(errs) <= 1e-6, errs
 
 
 # ----------------------------------------------------------------- C4
