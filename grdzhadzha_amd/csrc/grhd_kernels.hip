@@ -56,6 +56,7 @@ Options make_options(const grhd_params *p)
     o.mixed = p->mixed_precision != 0;
     o.guard_margin = o.mixed ? p->mixed_fp64_margin : 0.0;
     o.guard_shift = o.mixed ? p->mixed_fp64_shift : 0.0;
+    o.fp32_sources = o.mixed;
     o.excise = p->excise_horizon != 0;
     o.excise_inner = p->excision_inner_buffer;
     o.excise_outer = p->excision_outer_buffer;
@@ -831,7 +832,7 @@ void launch_faces(grhd_ctx *c, const Options &o, const Background &bg,
         launch_faces_t<double>(c, o, bg, p, s);
 }
 
-template <class FT>
+template <class FT, bool S32 = (sizeof(FT) < sizeof(double))>
 void launch_cells_t(grhd_ctx *c, int mode, const Options &o,
                     const Background &bs, const Background &bn,
                     const Background &bp, const Background &bm,
@@ -840,31 +841,31 @@ void launch_cells_t(grhd_ctx *c, int mode, const Options &o,
     switch (mode)
     {
     case MODE_RHS:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RHS, FT>), s, o, bs, bn, bp, bm,
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RHS, FT, S32>), s, o, bs, bn, bp, bm,
                     p, a);
         break;
     case MODE_RK4_1:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_1, FT>), s, o, bs, bn, bp,
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_1, FT, S32>), s, o, bs, bn, bp,
                     bm, p, a);
         break;
     case MODE_RK4_2:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_2, FT>), s, o, bs, bn, bp,
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_2, FT, S32>), s, o, bs, bn, bp,
                     bm, p, a);
         break;
     case MODE_RK4_3:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_3, FT>), s, o, bs, bn, bp,
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_3, FT, S32>), s, o, bs, bn, bp,
                     bm, p, a);
         break;
     case MODE_RK4_4:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_4, FT>), s, o, bs, bn, bp,
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_4, FT, S32>), s, o, bs, bn, bp,
                     bm, p, a);
         break;
     case MODE_SSP_1:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_1, FT>), s, o, bs, bn, bp,
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_1, FT, S32>), s, o, bs, bn, bp,
                     bm, p, a);
         break;
     default:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_2, FT>), s, o, bs, bn, bp,
+        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_2, FT, S32>), s, o, bs, bn, bp,
                     bm, p, a);
         break;
     }
@@ -876,7 +877,9 @@ void launch_cells(grhd_ctx *c, int mode, const Options &o,
                   const Background &bp, const Background &bm, const Pools &p,
                   const CellArgs &a, hipStream_t s)
 {
-    if (o.mixed && !guarded(o))
+    if (guarded(o))
+        launch_cells_t<double, true>(c, mode, o, bs, bn, bp, bm, p, a, s);
+    else if (o.mixed)
         launch_cells_t<float>(c, mode, o, bs, bn, bp, bm, p, a, s);
     else
         launch_cells_t<double>(c, mode, o, bs, bn, bp, bm, p, a, s);

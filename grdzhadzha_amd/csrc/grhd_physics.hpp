@@ -76,6 +76,9 @@ struct Options
      * (guard_margin > 0), or where the shift is large (guard_shift > 0);
      * see fp64_cell.  Both 0: every face FP32 */
     double guard_margin, guard_shift;
+    /* mixed only: Kerr-Schild static-metric sources in FP32 (cells
+     * without kWGuard) */
+    int fp32_sources;
     int excise; /* zero rhs inside the buffered Kerr-Schild horizons */
     double excise_inner, excise_outer;
 };
@@ -295,7 +298,7 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
     (void)M;
     const T a2 = a * a;
     const T z = xc[2];
-    const T el_t = 1.0;
+    const T el_t = T(1.0);
     const T H = c.H;
     const T r = c.r, r2 = c.r2, rho = c.rho, rho2 = c.rho2;
 
@@ -306,34 +309,34 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
     for (int i = 0; i < 3; ++i)
     {
         const T drhodx = xc[i] / rho;
-        drdx[i] = 0.5 / r *
+        drdx[i] = T(0.5) / r *
                   (rho * drhodx +
-                   0.5 / sqrt(0.25 * (rho2 - a2) * (rho2 - a2) + a2 * z * z) *
+                   T(0.5) / sqrt(T(0.25) * (rho2 - a2) * (rho2 - a2) + a2 * z * z) *
                        (drhodx * rho * (rho2 - a2) +
-                        kdelta(i, 2) * 2.0 * a2 * z));
+                        kdelta<T>(i, 2) * T(2.0) * a2 * z));
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        const T dcosthetadx = -z / r2 * drdx[i] + kdelta(i, 2) / r;
+        const T dcosthetadx = -z / r2 * drdx[i] + kdelta<T>(i, 2) / r;
         dHdx[i] = H * (drdx[i] / r -
-                       2.0 / (r2 + a2 * cos_theta2) *
+                       T(2.0) / (r2 + a2 * cos_theta2) *
                            (r * drdx[i] + a2 * c.cos_theta * dcosthetadx));
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
         dldx[0][i] =
-            (xc[0] * drdx[i] + r * kdelta(i, 0) + a * kdelta(i, 1) -
-             2.0 * r * drdx[i] * (r * xc[0] + a * xc[1]) / (r2 + a2)) /
+            (xc[0] * drdx[i] + r * kdelta<T>(i, 0) + a * kdelta<T>(i, 1) -
+             T(2.0) * r * drdx[i] * (r * xc[0] + a * xc[1]) / (r2 + a2)) /
             (r2 + a2);
         dldx[1][i] =
-            (xc[1] * drdx[i] + r * kdelta(i, 1) - a * kdelta(i, 0) -
-             2.0 * r * drdx[i] * (r * xc[1] - a * xc[0]) / (r2 + a2)) /
+            (xc[1] * drdx[i] + r * kdelta<T>(i, 1) - a * kdelta<T>(i, 0) -
+             T(2.0) * r * drdx[i] * (r * xc[1] - a * xc[0]) / (r2 + a2)) /
             (r2 + a2);
-        dldx[2][i] = -xc[2] * drdx[i] / r2 + kdelta(i, 2) / r;
+        dldx[2][i] = -xc[2] * drdx[i] / r2 + kdelta<T>(i, 2) / r;
     }
-    const T dltdx = 0.0;
+    const T dltdx = T(0.0);
 
     f.geo = gin;
     const GeoT<T> &g = f.geo;
@@ -346,23 +349,23 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 f.dg[k][i][j] =
-                    2.0 * (c.el[i] * c.el[j] * dHdx[k] +
+                    T(2.0) * (c.el[i] * c.el[j] * dHdx[k] +
                            H * c.el[i] * dldx[j][k] + H * c.el[j] * dldx[i][k]);
 
     const T lapse2 = g.lapse * g.lapse;
     const T lapse3 = g.lapse * g.lapse * g.lapse;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        f.dlapse[i] = -lapse3 * el_t * (el_t * dHdx[i] + 2.0 * H * dltdx);
+        f.dlapse[i] = -lapse3 * el_t * (el_t * dHdx[i] + T(2.0) * H * dltdx);
 
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            f.dshift[i][j] = 2.0 * el_t * dHdx[j] * lapse2 * c.el[i] +
-                             4.0 * el_t * H * g.lapse * f.dlapse[j] * c.el[i] +
-                             2.0 * el_t * H * lapse2 * dldx[i][j] +
-                             2.0 * dltdx * H * lapse2 * c.el[i];
+            f.dshift[i][j] = T(2.0) * el_t * dHdx[j] * lapse2 * c.el[i] +
+                             T(4.0) * el_t * H * g.lapse * f.dlapse[j] * c.el[i] +
+                             T(2.0) * el_t * H * lapse2 * dldx[i][j] +
+                             T(2.0) * dltdx * H * lapse2 * c.el[i];
 
     /* K_ij with the Christoffel symbols formed per (i, j)
      * (GRChombo compute_christoffel, KerrSchild.hpp:146-164) */
@@ -374,8 +377,8 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
             T LLL[3];
 #pragma unroll
             for (int l = 0; l < 3; ++l)
-                LLL[l] = 0.5 * (f.dg[j][i][l] + f.dg[i][j][l] - f.dg[l][i][j]);
-            T Kij = 0.0;
+                LLL[l] = T(0.5) * (f.dg[j][i][l] + f.dg[i][j][l] - f.dg[l][i][j]);
+            T Kij = T(0.0);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
             {
@@ -387,9 +390,9 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
                        (f.dg[j][k][i] + f.dg[i][k][j]) * g.shift[k];
 #pragma unroll
                 for (int m = 0; m < 3; ++m)
-                    Kij += -2.0 * ULL * g.g[k][m] * g.shift[m];
+                    Kij += -T(2.0) * ULL * g.g[k][m] * g.shift[m];
             }
-            Kij *= 0.5 / g.lapse;
+            Kij *= T(0.5) / g.lapse;
             f.K[i][j] = Kij;
         }
 
@@ -397,12 +400,12 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        T s = 0.0;
+        T s = T(0.0);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                s += 0.5 * g.gu[i][j] * f.dg[d][i][j];
+                s += T(0.5) * g.gu[i][j] * f.dg[d][i][j];
         f.dlogsqrtdet[d] = s;
     }
 }
@@ -1074,8 +1077,8 @@ __device__ __forceinline__ void static_sources(const PrimT<T> &p,
 {
     (void)gamma; /* the EOS argument of the reference's signature */
     const GeoT<G> &g = f.geo;
-    const T W = 1.0 / sqrt(1.0 - vsq(p.v, g.g));
-    const T h = 1.0 + p.eps + p.pressure / p.rho;
+    const T W = T(1.0) / sqrt(T(1.0) - vsq(p.v, g.g));
+    const T h = T(1.0) + p.eps + p.pressure / p.rho;
     const T rhW2 = p.rho * h * W * W;
     T Tuu[3][3];
 #pragma unroll
@@ -1087,7 +1090,7 @@ __device__ __forceinline__ void static_sources(const PrimT<T> &p,
     T mU[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        mU[i] = 0.0;
+        mU[i] = T(0.0);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -1097,9 +1100,9 @@ __device__ __forceinline__ void static_sources(const PrimT<T> &p,
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         flux_of(p, c, d, g.lapse, g.shift, cf[d]);
-    src.D = 0.0;
-    src.tau = 0.0;
-    src.S[0] = src.S[1] = src.S[2] = 0.0;
+    src.D = T(0.0);
+    src.tau = T(0.0);
+    src.S[0] = src.S[1] = src.S[2] = T(0.0);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
@@ -1120,7 +1123,7 @@ __device__ __forceinline__ void static_sources(const PrimT<T> &p,
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                ms += 0.5 * g.lapse * Tuu[i][j] * f.dg[m][i][j];
+                ms += T(0.5) * g.lapse * Tuu[i][j] * f.dg[m][i][j];
 #pragma unroll
         for (int s = 0; s < 3; ++s)
             ms += c.S[s] * f.dshift[s][m];

@@ -933,6 +933,50 @@ __global__ void k_list_reset(DevStatus *st)
         st->nlist = 0ull;
 }
 
+/* Mixed precision: the static-metric sources of a Kerr-Schild cell in
+ * FP32 (metric core, derivatives, K_ij and the source sums), from the
+ * FP64 floored primitives and state rounded once; the result is added to
+ * the FP64 right-hand side.  Guarded cells keep FP64 sources, like their
+ * faces; the binary's metric keeps FP64 sources throughout. */
+template <bool S32>
+__device__ __forceinline__ bool sources_in_fp32(const Options &o,
+                                                const Background &bg,
+                                                unsigned char wflags)
+{
+    if constexpr (S32)
+        return o.fp32_sources && bg.kind == 0 && !(wflags & kWGuard);
+    else
+        return false;
+}
+
+__device__ __forceinline__ ConsT<float>
+static_sources_fp32(const Background &bg, const double x[3], const Prim &pw,
+                    const Cons &us, double gamma)
+{
+    const float xf[3] = {float(x[0]), float(x[1]), float(x[2])};
+    GeoT<float> g;
+    KSCoreT<float> core;
+    ks_geometry_core(float(bg.M), float(bg.a), xf, g, core);
+    FullGeoT<float> f;
+    ks_derivs(float(bg.M), float(bg.a), xf, core, g, f);
+    PrimT<float> pf;
+    pf.rho = float(pw.rho);
+    pf.eps = float(pw.eps);
+    pf.pressure = float(pw.pressure);
+    ConsT<float> uf;
+    uf.D = float(us.D);
+    uf.tau = float(us.tau);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+    {
+        pf.v[m] = float(pw.v[m]);
+        uf.S[m] = float(us.S[m]);
+    }
+    ConsT<float> src;
+    static_sources(pf, uf, float(gamma), f, src);
+    return src;
+}
+
 enum CellMode
 {
     MODE_RHS = 0,
@@ -963,7 +1007,10 @@ struct CellArgs
 /* ------------------------------------------------------------------ */
 /* k_cells<MODE>: one thread per interior cell.                        */
 /* ------------------------------------------------------------------ */
-template <int MODE, class FT>
+/* FT: the type of the face-flux pool.  S32: the mixed paths' FP32
+ * sources may run (sources_in_fp32); false on the FP64 path, whose
+ * instantiations carry no FP32 code. */
+template <int MODE, class FT, bool S32 = (sizeof(FT) < sizeof(double))>
 __global__ void __launch_bounds__(kBlock)
     k_cells(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
             Background bg_s, Background bg_next, Background bg_plus,
@@ -1027,6 +1074,16 @@ __global__ void __launch_bounds__(kBlock)
             if (is_atmosphere(pw, o, o.source_atmosphere_factor))
             {
                 mbits |= GRHD_MASK_SOURCE_ATMOSPHERE;
+            }
+            else if (sources_in_fp32<S32>(o, bg_s, p.wbad[b.off_g + so]))
+            {
+                const ConsT<float> src = static_sources_fp32(bg_s, x, pw, us,
+                                                             o.gamma);
+                rhs.D += (double)src.D;
+                rhs.tau += (double)src.tau;
+                rhs.S[0] += (double)src.S[0];
+                rhs.S[1] += (double)src.S[1];
+                rhs.S[2] += (double)src.S[2];
             }
             else
             {
