@@ -526,67 +526,68 @@ __device__ __forceinline__ void pn_geometry(const Background &bg,
 
 /* The derivative half of the binary's ADM evaluation given its geometry
  * (CircularBinaryPN.hpp:132-188 + FixedBGMetric.hpp:32-55). */
-__device__ __forceinline__ void pn_derivs(const Potentials &p, const Geo &gin,
-                                          FullGeo &f)
+template <class T>
+__device__ __forceinline__ void pn_derivs(const PotentialsT<T> &p,
+                                          const GeoT<T> &gin, FullGeoT<T> &f)
 {
     f.geo = gin;
-    const Geo &g = f.geo;
-    const double U = p.scalar;
-    const double dcf_dt = 2.0 * p.scalar_dt;
-    const double cf = 1.0 + 2.0 * U;
-    double vsq = 0.0;
+    const GeoT<T> &g = f.geo;
+    const T U = p.scalar;
+    const T dcf_dt = T(2.0) * p.scalar_dt;
+    const T cf = T(1.0) + T(2.0) * U;
+    T vsq = T(0.0);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         vsq += p.vec[i] * p.vec[i];
-    const double lapse_squared = 1.0 - 2.0 * U + 2.0 * U * U + 16.0 * vsq / cf;
+    const T lapse_squared = T(1.0) - T(2.0) * U + T(2.0) * U * U + T(16.0) * vsq / cf;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        f.dlapse[i] = 0.0;
+        f.dlapse[i] = T(0.0);
 #pragma unroll
         for (int j = 0; j < 3; ++j)
         {
-            f.dshift[i][j] = 0.0;
+            f.dshift[i][j] = T(0.0);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                f.dg[k][i][j] = 0.0;
+                f.dg[k][i][j] = T(0.0);
         }
     }
     (void)lapse_squared;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        const double dU = p.dscalar[d];
-        const double dcf = 2.0 * dU;
+        const T dU = p.dscalar[d];
+        const T dcf = T(2.0) * dU;
 #pragma unroll
         for (int m = 0; m < 3; ++m)
             f.dg[d][m][m] = dcf;
-        double dvsq = 0.0;
+        T dvsq = T(0.0);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
         {
-            dvsq += 2.0 * p.vec[c] * p.dvec[c][d];
+            dvsq += T(2.0) * p.vec[c] * p.dvec[c][d];
             f.dshift[c][d] =
-                -4.0 * (p.dvec[c][d] / cf - p.vec[c] * dcf / (cf * cf));
+                -T(4.0) * (p.dvec[c][d] / cf - p.vec[c] * dcf / (cf * cf));
         }
-        const double dls = (-2.0 + 4.0 * U) * dU +
-                           16.0 * (dvsq / cf - vsq * dcf / (cf * cf));
-        f.dlapse[d] = 0.5 * dls / g.lapse;
+        const T dls = (-T(2.0) + T(4.0) * U) * dU +
+                           T(16.0) * (dvsq / cf - vsq * dcf / (cf * cf));
+        f.dlapse[d] = T(0.5) * dls / g.lapse;
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
         {
-            double LLL[3];
+            T LLL[3];
 #pragma unroll
             for (int l = 0; l < 3; ++l)
-                LLL[l] = 0.5 * (f.dg[j][i][l] + f.dg[i][j][l] - f.dg[l][i][j]);
-            double Kij = 0.0;
+                LLL[l] = T(0.5) * (f.dg[j][i][l] + f.dg[i][j][l] - f.dg[l][i][j]);
+            T Kij = T(0.0);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
             {
-                double ULL = 0;
+                T ULL = T(0);
 #pragma unroll
                 for (int l = 0; l < 3; ++l)
                     ULL += g.gu[k][l] * LLL[l];
@@ -594,22 +595,22 @@ __device__ __forceinline__ void pn_derivs(const Potentials &p, const Geo &gin,
                        (f.dg[j][k][i] + f.dg[i][k][j]) * g.shift[k];
 #pragma unroll
                 for (int m = 0; m < 3; ++m)
-                    Kij += -2.0 * ULL * g.g[k][m] * g.shift[m];
+                    Kij += -T(2.0) * ULL * g.g[k][m] * g.shift[m];
             }
             if (i == j)
                 Kij -= dcf_dt;
-            Kij *= 0.5 / g.lapse;
+            Kij *= T(0.5) / g.lapse;
             f.K[i][j] = Kij;
         }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        double s = 0.0;
+        T s = T(0.0);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                s += 0.5 * g.gu[i][j] * f.dg[d][i][j];
+                s += T(0.5) * g.gu[i][j] * f.dg[d][i][j];
         f.dlogsqrtdet[d] = s;
     }
 }

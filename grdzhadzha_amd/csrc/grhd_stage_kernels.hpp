@@ -933,18 +933,19 @@ __global__ void k_list_reset(DevStatus *st)
         st->nlist = 0ull;
 }
 
-/* Mixed precision: the static-metric sources of a Kerr-Schild cell in
- * FP32 (metric core, derivatives, K_ij and the source sums), from the
- * FP64 floored primitives and state rounded once; the result is added to
- * the FP64 right-hand side.  Guarded cells keep FP64 sources, like their
- * faces; the binary's metric keeps FP64 sources throughout. */
+/* Mixed precision: the static-metric sources of a cell in FP32 (metric,
+ * its derivatives, K_ij and the source sums), from the FP64 floored
+ * primitives and state rounded once; the result is added to the FP64
+ * right-hand side.  Guarded cells keep FP64 sources, like their faces.
+ * The binary's volume-time sources (a difference of log sqrt(gamma) at
+ * t +- h over 2h) stay FP64. */
 template <bool S32>
 __device__ __forceinline__ bool sources_in_fp32(const Options &o,
                                                 const Background &bg,
                                                 unsigned char wflags)
 {
     if constexpr (S32)
-        return o.fp32_sources && bg.kind == 0 && !(wflags & kWGuard);
+        return o.fp32_sources && !(wflags & kWGuard);
     else
         return false;
 }
@@ -955,10 +956,19 @@ static_sources_fp32(const Background &bg, const double x[3], const Prim &pw,
 {
     const float xf[3] = {float(x[0]), float(x[1]), float(x[2])};
     GeoT<float> g;
-    KSCoreT<float> core;
-    ks_geometry_core(float(bg.M), float(bg.a), xf, g, core);
     FullGeoT<float> f;
-    ks_derivs(float(bg.M), float(bg.a), xf, core, g, f);
+    if (bg.kind == 1)
+    {
+        PotentialsT<float> pot;
+        pn_geometry_pot(bg, xf, g, pot);
+        pn_derivs(pot, g, f);
+    }
+    else
+    {
+        KSCoreT<float> core;
+        ks_geometry_core(float(bg.M), float(bg.a), xf, g, core);
+        ks_derivs(float(bg.M), float(bg.a), xf, core, g, f);
+    }
     PrimT<float> pf;
     pf.rho = float(pw.rho);
     pf.eps = float(pw.eps);

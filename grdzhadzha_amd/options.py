@@ -80,8 +80,8 @@ class OperatorOptions:
     use_metric_volume_time_sources: bool = False
     metric_time_derivative_step: float = 1.0e-3
     # no counterpart in the reference: FP32 reconstruction, face metric and
-    # HLLE, and Kerr-Schild static-metric sources; recovery, the divergence
-    # sum and the update FP64 (False: the bit-exact FP64 path)
+    # HLLE, and static-metric sources; recovery, the divergence sum and the
+    # update FP64 (False: the bit-exact FP64 path)
     mixed_precision: bool = False
     # no counterpart in the reference: with mixed_precision, a margin m >= 1
     # computes in FP64 the faces beside cells with rho <= m rho_floor,

@@ -133,11 +133,11 @@ typedef struct grhd_params
     int use_metric_volume_time_sources;
     double metric_time_derivative_step;
     /* No counterpart in the reference: nonzero puts reconstruction, the
-     * face metric and HLLE in FP32 (face fluxes stored as float), and on
-     * Kerr-Schild backgrounds the static-metric sources (metric
-     * derivatives, K_ij and the source sums); con2prim, W, the
-     * flux-divergence sum, the addition of the sources and the RK update
-     * stay FP64.  Zero (the default) is the bit-exact FP64 path. */
+     * face metric and HLLE in FP32 (face fluxes stored as float), and the
+     * static-metric sources (metric derivatives, K_ij and the source
+     * sums); con2prim, W, the flux-divergence sum, the addition of the
+     * sources, the volume-time sources and the RK update stay FP64.  Zero
+     * (the default) is the bit-exact FP64 path. */
     int mixed_precision;
     /* KerrSchild::check_if_excised (KerrSchild.hpp:247-280) applied the way
      * ExcisionTest applies it to the matter right-hand side
