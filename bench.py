@@ -868,7 +868,9 @@ def main():
                           "speedup_vs_plain": b["speedup_vs_plain"],
                           "face_launch_per_level":
                               b["face_launch_per_level"],
-                          "batched_mixed": b["batched_mixed"]}
+                          "batched_mixed": b["batched_mixed"],
+                          "batched_guarded_mixed":
+                              b["batched_guarded_mixed"]}
     if not args.no_mixed:
         torch.cuda.empty_cache()
         out["mixed"] = bench_mixed(args)
