@@ -933,10 +933,10 @@ __global__ void k_list_reset(DevStatus *st)
         st->nlist = 0ull;
 }
 
-/* Mixed precision: the static-metric sources of a cell in FP32 (metric,
- * its derivatives, K_ij and the source sums), from the FP64 floored
- * primitives and state rounded once; the result is added to the FP64
- * right-hand side.  Guarded cells keep FP64 sources, like their faces.
+/* Mixed precision: the static-metric sources of a cell in FP32 (the
+ * metric's derivatives, K_ij and the source sums), from the FP64 metric,
+ * floored primitives and state rounded once; the result is added to the
+ * FP64 right-hand side.  Guarded cells keep FP64 sources, like their faces.
  * The binary's volume-time sources (a difference of log sqrt(gamma) at
  * t +- h over 2h) stay FP64. */
 template <bool S32>
@@ -950,24 +950,67 @@ __device__ __forceinline__ bool sources_in_fp32(const Options &o,
         return false;
 }
 
+/* The derivative half runs in FP32 on the cell's FP64 geometry half
+ * (metric, inverse, lapse, shift, and the Kerr-Schild radius, H and l_i
+ * or the binary's potentials), rounded once: the cell has evaluated that
+ * half in FP64 already for its floors, so it is not evaluated again. */
+__device__ __forceinline__ GeoT<float> round_geo(const Geo &g)
+{
+    GeoT<float> r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+        {
+            r.g[i][j] = float(g.g[i][j]);
+            r.gu[i][j] = float(g.gu[i][j]);
+        }
+        r.shift[i] = float(g.shift[i]);
+    }
+    r.lapse = float(g.lapse);
+    return r;
+}
+
 __device__ __forceinline__ ConsT<float>
-static_sources_fp32(const Background &bg, const double x[3], const Prim &pw,
+static_sources_fp32(const Background &bg, const double x[3],
+                    const GeoCore &core, const Geo &geo, const Prim &pw,
                     const Cons &us, double gamma)
 {
     const float xf[3] = {float(x[0]), float(x[1]), float(x[2])};
-    GeoT<float> g;
+    const GeoT<float> g = round_geo(geo);
     FullGeoT<float> f;
     if (bg.kind == 1)
     {
+        const Potentials &pd = core.pn;
         PotentialsT<float> pot;
-        pn_geometry_pot(bg, xf, g, pot);
+        pot.scalar = float(pd.scalar);
+        pot.scalar_dt = float(pd.scalar_dt);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            pot.dscalar[i] = float(pd.dscalar[i]);
+            pot.vec[i] = float(pd.vec[i]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                pot.dvec[i][j] = float(pd.dvec[i][j]);
+        }
         pn_derivs(pot, g, f);
     }
     else
     {
-        KSCoreT<float> core;
-        ks_geometry_core(float(bg.M), float(bg.a), xf, g, core);
-        ks_derivs(float(bg.M), float(bg.a), xf, core, g, f);
+        const KSCore &kd = core.ks;
+        KSCoreT<float> kc;
+        kc.rho = float(kd.rho);
+        kc.rho2 = float(kd.rho2);
+        kc.r2 = float(kd.r2);
+        kc.r = float(kd.r);
+        kc.cos_theta = float(kd.cos_theta);
+        kc.H = float(kd.H);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            kc.el[i] = float(kd.el[i]);
+        ks_derivs(float(bg.M), float(bg.a), xf, kc, g, f);
     }
     PrimT<float> pf;
     pf.rho = float(pw.rho);
@@ -1087,8 +1130,8 @@ __global__ void __launch_bounds__(kBlock)
             }
             else if (sources_in_fp32<S32>(o, bg_s, p.wbad[b.off_g + so]))
             {
-                const ConsT<float> src = static_sources_fp32(bg_s, x, pw, us,
-                                                             o.gamma);
+                const ConsT<float> src = static_sources_fp32(
+                    bg_s, x, core, geo_s, pw, us, o.gamma);
                 rhs.D += (double)src.D;
                 rhs.tau += (double)src.tau;
                 rhs.S[0] += (double)src.S[0];
