@@ -15,16 +15,31 @@ constexpr int kBlock = 256;
 constexpr unsigned long long kNoFail = ~0ull;
 constexpr int kKeyBits = 43; /* 2 x (cells of a level) < 2^43 */
 
-/* Faces per walker of the marching y/z face kernels on a level of `cells`
- * interior cells.  A walk shares each cell's metric and slopes between the
- * two faces it touches, but it is sequential: walks of 8 faces on a 32^3
- * box leave 80 waves for 1024 SIMDs.  Walks get shorter until the level
- * yields at least 2^16 walkers (one wave per SIMD).  Measured per RK4 step
- * (ms, walk length 1 / 2 / 4 / 8): 32^3 0.19 / 0.21 / 0.26 / 0.36,
- * 64^3 0.48 / 0.45 / 0.43 / 0.52, 96^3 1.19 / 1.13 / 1.12 / 1.09; the
- * fluxes do not depend on it. */
+/* Tiles of the transverse face kernels (k_faces_t, BoxDev::seg ==
+ * kSegTile): a wavefront holds kTileI cells along i by kTileD cells along
+ * the face direction d and forms the kTileD - 1 faces between them in
+ * every column; tiles along d overlap by one cell. */
+constexpr int kTileI = 8, kTileD = 8;
+constexpr int kSegTile = 0;
+
+/* The y/z face launch of a level of `cells` interior cells: the number of
+ * faces per walker of the marching kernels (k_faces), or kSegTile for the
+ * tile kernels (k_faces_t).  A walk shares each cell's metric and slopes
+ * between the two faces it touches, but it is sequential: walks of 8
+ * faces on a 32^3 box leave 80 waves for 1024 SIMDs.  Walks get shorter
+ * until the level yields at least 2^16 walkers (one wave per SIMD); where
+ * that would leave walks of one face (two metric evaluations per face),
+ * below 2^17 cells, the tiles take over (8 / 7 metric evaluations per
+ * face, every cell its own lane).  Measured per RK4 step (ms, walk length
+ * 1 / 2 / 4 / 8): 32^3 0.19 / 0.21 / 0.26 / 0.36, 64^3 0.48 / 0.45 / 0.43
+ * / 0.52, 96^3 1.19 / 1.13 / 1.12 / 1.09; on the 32^3 Michel box the
+ * tiles took 0.136 ms against 0.144 for walks of one face.  On larger
+ * levels the tiles were no faster in FP64 and 20% slower in FP32.  The
+ * fluxes do not depend on the choice. */
 __host__ inline int march_segment(long long cells)
 {
+    if (cells < (1ll << 17))
+        return kSegTile;
     int seg = 8;
     while (seg > 1 && cells / seg < (1ll << 16))
         seg /= 2;
@@ -68,7 +83,8 @@ struct BoxDev
     long long nface[3]; /* faces of direction d of this box */
     double inv_dx[3];   /* inverse_dx_U: 1 / dx (rounded once on the
                            host) unless the caller gives its own */
-    int seg;            /* faces per walker of the marching y/z kernels */
+    int seg;            /* faces per walker of the marching y/z kernels,
+                           or kSegTile for the tile kernels */
 };
 
 /* Kinds of work item, one block table each. */
@@ -78,11 +94,18 @@ enum ItemClass
     C_EXT = 1, /* storage cells outside the interior     */
     C_INT = 2, /* interior cells                         */
     C_FX = 3,  /* x-face lanes: 64 per 63 cells of a row */
-    C_FY = 4,  /* marching y-face walkers                */
-    C_FZ = 5,  /* marching z-face walkers                */
+    C_FY = 4,  /* marching y-face walkers, or tile lanes */
+    C_FZ = 5,  /* marching z-face walkers, or tile lanes */
     C_FALL = 6, /* the C_FX, C_FY and C_FZ tables, in that order */
     NCLASS = 7
 };
+
+__host__ __device__ inline long long face_tiles(const BoxDev &b, int d)
+{
+    const int other = (d == 2) ? 1 : 2;
+    return (long long)((b.in[0] + kTileI - 1) / kTileI) *
+           ((b.in[d] + 1 + kTileD - 2) / (kTileD - 1)) * b.in[other];
+}
 
 __host__ __device__ inline int face_segments(const BoxDev &b, int d)
 {
@@ -107,8 +130,12 @@ __host__ __device__ inline long long class_items(const BoxDev &b, int c)
     case C_FX:
         return (fx_cells(b) - 1 + 62) / 63 * 64;
     case C_FY:
+        if (b.seg == kSegTile)
+            return 64 * face_tiles(b, 1);
         return (long long)b.in[0] * b.in[2] * face_segments(b, 1);
     case C_FZ:
+        if (b.seg == kSegTile)
+            return 64 * face_tiles(b, 2);
         return (long long)b.in[0] * b.in[1] * face_segments(b, 2);
     default: /* C_FALL is made of the three face tables */
         return 0;

@@ -176,9 +176,10 @@ inline grhd_level_box one_box(const grhd_box *b)
     return lb;
 }
 
-/* Walk lengths of the marching y/z face kernels that the tuner times. */
-constexpr int kNumSegs = 4;
-constexpr int kSegs[kNumSegs] = {1, 2, 4, 8};
+/* Walk lengths of the marching y/z face kernels, and the tile kernels,
+ * that the tuner times. */
+constexpr int kNumSegs = 5;
+constexpr int kSegs[kNumSegs] = {1, 2, 4, 8, kSegTile};
 
 /* The device copies that depend on the walk length: the box table (each
  * BoxDev carries it) and the y, z and fused face block tables. */
@@ -672,15 +673,26 @@ template <class T, int SEL = FACES_ALL>
 void faces_launches(grhd_ctx *c, const Options &o, const Background &bg,
                     const Pools &p, hipStream_t s, bool fused)
 {
+    const bool tiles = kSegs[c->plan.seg_cur] == kSegTile;
     if (fused)
     {
         if (c->plan.nblocks[C_FALL] > 0)
-            hipLaunchKernelGGL((k_faces_all<T, SEL>),
-                               dim3(c->plan.nblocks[C_FALL]),
-                               dim3(kBlock), 0, s, c->plan.d_boxes,
-                               c->plan.d_items[C_FALL],
-                               c->plan.nblocks[C_FX], c->plan.nblocks[C_FY],
-                               o, bg, p);
+        {
+            if (tiles)
+                hipLaunchKernelGGL((k_faces_all_t<T, SEL>),
+                                   dim3(c->plan.nblocks[C_FALL]),
+                                   dim3(kBlock), 0, s, c->plan.d_boxes,
+                                   c->plan.d_items[C_FALL],
+                                   c->plan.nblocks[C_FX],
+                                   c->plan.nblocks[C_FY], o, bg, p);
+            else
+                hipLaunchKernelGGL((k_faces_all<T, SEL>),
+                                   dim3(c->plan.nblocks[C_FALL]),
+                                   dim3(kBlock), 0, s, c->plan.d_boxes,
+                                   c->plan.d_items[C_FALL],
+                                   c->plan.nblocks[C_FX],
+                                   c->plan.nblocks[C_FY], o, bg, p);
+        }
         return;
     }
     /* a failed fork runs the three kernels in order on `s`; the error
@@ -690,15 +702,22 @@ void faces_launches(grhd_ctx *c, const Options &o, const Background &bg,
                             hipSuccess &&
                         hipStreamWaitEvent(c->side[1], c->fork, 0) ==
                             hipSuccess;
+    hipStream_t sy = forked ? c->side[0] : s;
+    hipStream_t sz = forked ? c->side[1] : s;
     if (!forked)
-    {
         GRHD_LAUNCH(c, C_FX, (k_faces_x<T, SEL>), s, o, bg, p);
-        GRHD_LAUNCH(c, C_FY, (k_faces<1, T, SEL>), s, o, bg, p);
-        GRHD_LAUNCH(c, C_FZ, (k_faces<2, T, SEL>), s, o, bg, p);
-        return;
+    if (tiles)
+    {
+        GRHD_LAUNCH(c, C_FY, (k_faces_t<1, T, SEL>), sy, o, bg, p);
+        GRHD_LAUNCH(c, C_FZ, (k_faces_t<2, T, SEL>), sz, o, bg, p);
     }
-    GRHD_LAUNCH(c, C_FY, (k_faces<1, T, SEL>), c->side[0], o, bg, p);
-    GRHD_LAUNCH(c, C_FZ, (k_faces<2, T, SEL>), c->side[1], o, bg, p);
+    else
+    {
+        GRHD_LAUNCH(c, C_FY, (k_faces<1, T, SEL>), sy, o, bg, p);
+        GRHD_LAUNCH(c, C_FZ, (k_faces<2, T, SEL>), sz, o, bg, p);
+    }
+    if (!forked)
+        return;
     GRHD_LAUNCH(c, C_FX, (k_faces_x<T, SEL>), s, o, bg, p);
     /* join: `s` waits for both side streams; if an event operation fails,
      * the host waits for the side stream instead, so k_cells never runs
@@ -713,7 +732,8 @@ void faces_launches(grhd_ctx *c, const Options &o, const Background &bg,
 constexpr int kTuneReps = 3;
 
 /* Time every face launch of the level -- each walk length of the marching
- * kernels, fused and unfused -- on the call's own primitives, and keep the
+ * kernels and the tile kernels, fused and unfused -- on the call's own
+ * primitives, and keep the
  * fastest for faces of type T.  The faces only read W and write their
  * flux pools, and the fluxes do not depend on the walk length or the
  * launch form, so the extra launches change no result: the call's own

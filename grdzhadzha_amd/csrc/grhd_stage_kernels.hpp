@@ -19,8 +19,9 @@
  *                failure byte per cell
  *   k_faces_x,   MUSCL slopes (FArrayBoxFiniteVolume.hpp:573-589), face
  *   k_faces<1,2> metric = mean of the two cell metrics (:179-197), floors,
- *                prim2con, HLLE -> face fluxes F_d (5 x nfaces), each
- *                face once
+ *   (walkers) or prim2con, HLLE -> face fluxes F_d (5 x nfaces), each
+ *   k_faces_t<1,2> face once
+ *   (tiles)
  *   k_cells<M>   flux divergence in the reference's summation order,
  *                static-metric sources (:255-298), volume-time sources
  *                (:616-657), then per mode: write rhs, or an RK4 / SSPRK2
@@ -834,6 +835,135 @@ __global__ void __launch_bounds__(kBlock)
             Pools p)
 {
     faces_march_block<D, T, SEL>(boxes, items[blockIdx.x], o, bg, p);
+}
+
+/* ------------------------------------------------------------------ */
+/* k_faces_t<D> (D = 1, 2): fluxes of direction D by tiles, one lane   */
+/* per cell.  A wavefront holds kTileI consecutive cells along i (lane */
+/* bits 0-2) by kTileD consecutive cells along D (bits 3-5), starting  */
+/* one cell below the tile's first face; each lane evaluates its       */
+/* cell's metric and two half-cell states once, and the face between a */
+/* lane and the lane one cell further along D is formed with           */
+/* __shfl_down by kTileI.  Loads are unit-stride along i in rows of    */
+/* kTileI cells; tiles along D overlap by one cell, so a cell's metric */
+/* is evaluated kTileD / (kTileD - 1) times per direction.             */
+/* ------------------------------------------------------------------ */
+template <int D, class T, int SEL = FACES_ALL>
+__device__ __forceinline__ void faces_tile_block(
+    const BoxDev *__restrict__ boxes, const int2 bt, const Options &o,
+    const Background &bg, const Pools &p)
+{
+    using O = typename FluxOut<T, SEL>::type;
+    const BoxDev &b = boxes[bt.x];
+    const long long wave = ((long long)bt.y + threadIdx.x) >> 6;
+    /* the block's last waves may lie past the box's tiles (uniform per
+     * wavefront, so the shuffles below see whole wavefronts) */
+    if (wave >= face_tiles(b, D))
+        return;
+    constexpr int B = (D == 2) ? 1 : 2; /* the other slow axis */
+    const int lane = threadIdx.x & 63;
+    const int li = lane % kTileI, ld = lane / kTileI;
+    const int nti = (b.in[0] + kTileI - 1) / kTileI;
+    const int ntd = (b.in[D] + 1 + kTileD - 2) / (kTileD - 1);
+    const int ti = (int)(wave % nti);
+    const int td = (int)((wave / nti) % ntd);
+    const int ib = (int)(wave / ((long long)nti * ntd));
+    /* face f along D lies between cells lo - 1 + f and lo + f; this lane
+     * holds cell lo - 1 + f and forms face f with the next row's lane */
+    const int f = td * (kTileD - 1) + ld;
+    const int iu = ti * kTileI + li; /* interior offset along i */
+    const bool makes_face =
+        iu < b.in[0] && ld < kTileD - 1 && f <= b.in[D];
+    int c[3];
+    c[0] = b.lo[0] + min(iu, b.in[0] - 1);
+    c[B] = b.lo[B] + ib;
+    c[D] = b.lo[D] - 1 + min(f, b.in[D] + 1); /* clamped to cell hi + 1 */
+    const long long n = b.gnpts;
+    const double *W = p.W + 6 * b.off_g;
+    const long long stride =
+        (D == 1) ? (long long)b.gn[0] : (long long)b.gn[0] * b.gn[1];
+    const long long oc = goff(b, c[0], c[1], c[2]);
+    const long long fo =
+        (D == 1) ? ((long long)ib * (b.in[1] + 1) + f) * b.in[0] + iu
+                 : ((long long)f * b.in[1] + ib) * b.in[0] + iu;
+    bool mine = makes_face;
+    if (SEL == FACES_FP32)
+    {
+        /* as in faces_x_block: every lane shuffles, a wavefront with no
+         * FP32 face of its own leaves at once */
+        const int fc = p.wbad[b.off_g + oc];
+        const int fn = __shfl_down(fc, kTileI, 64);
+        const bool to64 = makes_face && fp64_face(fc, fn);
+        list_append_lane(p, to64, list_entry(bt.x, D, fo));
+        mine = makes_face && !to64;
+        if (!__any(mine))
+            return;
+    }
+    const PrimT<T> wc = load_prim_as<T>(W, n, oc);
+    PrimT<T> mh = wc, ph = wc;
+    if (o.use_reconstruction)
+    {
+        const PrimT<T> wl = load_prim_as<T>(W, n, oc - stride);
+        const PrimT<T> wr = load_prim_as<T>(W, n, oc + stride);
+        slope_step(wl, wc, wr, T(o.theta), mh, ph);
+    }
+    T xc[3];
+    coords_as(c[0], c[1], c[2], b, xc);
+    GeoT<T> gc;
+    cell_geometry(bg, xc, gc);
+
+    PrimT<T> rp;
+    rp.rho = __shfl_down(mh.rho, kTileI, 64);
+    rp.eps = __shfl_down(mh.eps, kTileI, 64);
+    rp.pressure = __shfl_down(mh.pressure, kTileI, 64);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        rp.v[m] = __shfl_down(mh.v[m], kTileI, 64);
+    GeoT<T> gn;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+        {
+            gn.g[a][e] = __shfl_down(gc.g[a][e], kTileI, 64);
+            gn.gu[a][e] = __shfl_down(gc.gu[a][e], kTileI, 64);
+        }
+    gn.lapse = __shfl_down(gc.lapse, kTileI, 64);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        gn.shift[m] = __shfl_down(gc.shift[m], kTileI, 64);
+
+    if (!mine)
+        return;
+    GeoT<T> face;
+    average_geo(gc, gn, face);
+    ConsT<T> fl;
+    hlle_face(ph, rp, o, face, D, fl);
+    store_flux(face_pool<O>(p, b, D), b.nface[D], fo, fl);
+}
+
+template <int D, class T, int SEL = FACES_ALL>
+__global__ void __launch_bounds__(kBlock)
+    k_faces_t(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
+              Background bg, Pools p)
+{
+    faces_tile_block<D, T, SEL>(boxes, items[blockIdx.x], o, bg, p);
+}
+
+/* The three directions in one launch with the tile kernels for y and z:
+ * x blocks, then y tile blocks, then z tile blocks. */
+template <class T, int SEL = FACES_ALL>
+__global__ void __launch_bounds__(kBlock)
+    k_faces_all_t(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, int nbx,
+                  int nby, Options o, Background bg, Pools p)
+{
+    const int2 bt = items[blockIdx.x];
+    if ((int)blockIdx.x < nbx)
+        faces_x_block<T, SEL>(boxes, bt, o, bg, p);
+    else if ((int)blockIdx.x < nbx + nby)
+        faces_tile_block<1, T, SEL>(boxes, bt, o, bg, p);
+    else
+        faces_tile_block<2, T, SEL>(boxes, bt, o, bg, p);
 }
 
 /* All three face directions of a call in one launch, for levels too small

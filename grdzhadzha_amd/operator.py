@@ -89,7 +89,8 @@ class Context:
 
     def face_launch(self, mixed=False):
         """{'walk', 'fused', 'tuned', 'ms'} of the current level's face
-        launch for FP64 (mixed=False) or FP32 faces."""
+        launch for FP64 (mixed=False) or FP32 faces; walk 0 means the y/z
+        faces are formed in 8 x 8 tiles rather than by walkers."""
         walk, fused, tuned = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_double()
         rc = self.lib.grhd_ctx_get_face_launch(

@@ -556,11 +556,12 @@ int grhd_ctx_get_timing(grhd_ctx *ctx, double *total_ms,
 /* ------------------------------------------------------------------ */
 /* Face-launch tuning (no counterpart in the reference: its CPU loops */
 /* have no launch shape to choose).  The y and z face kernels walk a  */
-/* run of 1, 2, 4 or 8 faces per thread, and the three face           */
-/* directions run as one fused launch or as three kernels on          */
-/* concurrent streams.  By default size rules choose both.  With      */
-/* tuning on, the first call on a new level (a new box list) or in a  */
-/* new face precision times all eight choices on that call's own      */
+/* run of 1, 2, 4 or 8 faces per thread, or form faces in tiles of    */
+/* 8 x 8 cells (one cell per lane), and the three face directions run */
+/* as one fused launch or as three kernels on concurrent streams.  By */
+/* default size rules choose both.  With tuning on, the first call on */
+/* a new level (a new box list) or in a new face precision times all  */
+/* ten choices on that call's own                                     */
 /* primitives and keeps the fastest for the plan; the fluxes, and so  */
 /* every result, are the same under every choice.  A tuning call      */
 /* waits on the host once per choice.                                 */
@@ -570,7 +571,8 @@ int grhd_ctx_get_timing(grhd_ctx *ctx, double *total_ms,
 int grhd_ctx_set_autotune(grhd_ctx *ctx, int enable);
 
 /* The face launch the current level uses for FP64 (mixed = 0) or FP32
- * (mixed = 1) faces: walk length, fused (1) or not (0), whether it was
+ * (mixed = 1) faces: walk length (0 for the tiles), fused (1) or not (0),
+ * whether it was
  * measured (1) or is the rule's (0), and the measured milliseconds per
  * face stage (0 for the rule).  Any output pointer may be NULL.  Returns
  * GRHD_ERR_INVALID_ARGUMENT before the first level call. */

@@ -532,8 +532,8 @@ def test_level_plan_for_4096_boxes_builds_quickly(hip_context):
 
 @pytest.mark.parametrize("mixed", [False, True])
 def test_tuned_face_launch_equals_rule_launch(hip_context, mixed):
-    """A tuning context times the eight face launches (walk length 1, 2, 4
-    or 8; fused or on streams) on its first call over a 10-box binary-PN
+    """A tuning context times the ten face launches (walk length 1, 2, 4
+    or 8, or tiles; fused or on streams) on its first call over a 10-box binary-PN
     level and keeps the fastest.  Its results equal those of a context on
     the size rules bit for bit (state, mask, counts), on the tuning call and
     on the next call, which reuses the choice without tuning again."""
@@ -560,7 +560,7 @@ def test_tuned_face_launch_equals_rule_launch(hip_context, mixed):
         tuned_ctx.close()
     print("face launch: rule", rule, "tuned", choice)
     assert choice["tuned"] and not rule["tuned"]
-    assert choice["walk"] in (1, 2, 4, 8) and choice["ms"] > 0.0
+    assert choice["walk"] in (0, 1, 2, 4, 8) and choice["ms"] > 0.0
     (p0, m0, a0, b0), (p1, m1, a1, b1) = out
     assert np.array_equal(p0, p1)
     assert np.array_equal(m0, m1)
