@@ -423,6 +423,7 @@ def bench_batched(args):
                 "fp64_faces": "beside cells with |beta| >= %g"
                               % args.batched_shift,
                 "value": guarded, "ms_per_step": tg / steps * 1e3,
+                "speedup_vs_plain_per_box": guarded / plain,
                 "speedup_vs_batched_fp64": guarded / batched,
                 "error_vs_batched_fp64": err_g,
                 "gate_1e-6": gate_1e6(err_g)}}
