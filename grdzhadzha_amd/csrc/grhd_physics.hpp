@@ -287,6 +287,38 @@ __device__ __forceinline__ void ks_geometry(T M, T a, const T x[3],
     ks_geometry_core(M, a, x, g, c);
 }
 
+/* Reduced precision: K_ij of a metric with time derivative kdiag delta_ij
+ * (zero for Kerr-Schild) as the Lie derivative of gamma along the shift,
+ * K_ij = (beta^l d_l gamma_ij + gamma_lj d_i beta^l + gamma_il d_j beta^l
+ *         - kdiag delta_ij) / (2 alpha),
+ * which equals the Christoffel form below (KerrSchild.hpp:146-164) in
+ * exact arithmetic: the Christoffel terms reduce to beta^l d_l gamma_ij
+ * once Gamma^k_ij beta_k is written as beta^l Gamma_lij.  It is symmetric
+ * by construction and takes a ninth of the Christoffel form's operations;
+ * only the float instantiations use it (the FP64 path keeps the
+ * reference's form and rounding). */
+template <class T>
+__device__ __forceinline__ void lie_extrinsic_curvature(const GeoT<T> &g,
+                                                        const T dg[3][3][3],
+                                                        const T dshift[3][3],
+                                                        T kdiag, T K[3][3])
+{
+    const T half_over_lapse = T(0.5) / g.lapse;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j)
+        {
+            T v = (i == j) ? -kdiag : T(0);
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+                v += g.shift[l] * dg[l][i][j] + g.g[l][j] * dshift[l][i] +
+                     g.g[i][l] * dshift[l][j];
+            K[i][j] = v * half_over_lapse;
+            K[j][i] = K[i][j];
+        }
+}
+
 /* The derivative half of the full ADM evaluation, given the geometry half
  * (KerrSchild.hpp:118-242 + FixedBGMetric.hpp:32-55): dgamma, dalpha,
  * dbeta, K_ij and d ln sqrt gamma. */
@@ -342,15 +374,35 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
     const GeoT<T> &g = f.geo;
 
     /* d1_gamma[i][j][k] stored as dg[k][i][j] */
+    if constexpr (sizeof(T) < sizeof(double))
+    {
+        /* reduced precision: the symmetric half, mirrored */
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+            for (int j = i; j < 3; ++j)
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                f.dg[k][i][j] =
-                    T(2.0) * (c.el[i] * c.el[j] * dHdx[k] +
-                           H * c.el[i] * dldx[j][k] + H * c.el[j] * dldx[i][k]);
+                for (int k = 0; k < 3; ++k)
+                {
+                    f.dg[k][i][j] =
+                        T(2.0) * (c.el[i] * c.el[j] * dHdx[k] +
+                                  H * c.el[i] * dldx[j][k] +
+                                  H * c.el[j] * dldx[i][k]);
+                    f.dg[k][j][i] = f.dg[k][i][j];
+                }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    f.dg[k][i][j] =
+                        T(2.0) * (c.el[i] * c.el[j] * dHdx[k] +
+                               H * c.el[i] * dldx[j][k] + H * c.el[j] * dldx[i][k]);
+    }
 
     const T lapse2 = g.lapse * g.lapse;
     const T lapse3 = g.lapse * g.lapse * g.lapse;
@@ -369,6 +421,9 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
 
     /* K_ij with the Christoffel symbols formed per (i, j)
      * (GRChombo compute_christoffel, KerrSchild.hpp:146-164) */
+    if constexpr (sizeof(T) < sizeof(double))
+        lie_extrinsic_curvature(g, f.dg, f.dshift, T(0), f.K);
+    else
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -574,6 +629,9 @@ __device__ __forceinline__ void pn_derivs(const PotentialsT<T> &p,
                            T(16.0) * (dvsq / cf - vsq * dcf / (cf * cf));
         f.dlapse[d] = T(0.5) * dls / g.lapse;
     }
+    if constexpr (sizeof(T) < sizeof(double))
+        lie_extrinsic_curvature(g, f.dg, f.dshift, dcf_dt, f.K);
+    else
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
