@@ -374,11 +374,13 @@ def test_later_evaluation_ghost_fill_error(hip_context, scheme):
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (1, 2, 3), (3, 1, 2),
-                                   (2, 5, 1), (9, 1, 1)])
+                                   (2, 5, 1), (9, 1, 1), (13, 11, 9),
+                                   (8, 15, 22)])
 def test_tiny_and_ragged_boxes_rhs_and_rk4(hip_context, shape):
-    """Interiors thinner than the ghost width along some axis: the x-face
-    wave packing, the y/z marching with fewer faces than a segment and the
-    exterior-cell enumeration against the oracle (rhs and one RK4 step)."""
+    """Interiors thinner than the ghost width along some axis, or covered
+    by partial y/z face tiles (8 cells along i by 8 along y or z, 7 faces
+    per column): the x-face wave packing, the tiles and the exterior-cell
+    enumeration against the oracle (rhs and one RK4 step)."""
     bg = KerrSchildParams(1.0, 0.5)
     box = BoxSpec((2, 3, 4), tuple(l + n - 1 for l, n in
                                    zip((2, 3, 4), shape)), 3, 0.37,
