@@ -61,8 +61,10 @@ struct DevStatus
     unsigned long long max_bits;
     double sums[5];              /* level conserved sums */
     unsigned long long nonfinite; /* interior cells with a non-finite value */
-    /* entries of the guarded mixed path's FP64 face list (Pools::list) */
+    /* entries of the guarded mixed path's FP64 face list (Pools::list)
+     * and of its guarded-cell list (Pools::clist) */
     alignas(128) unsigned long long nlist;
+    unsigned long long ncell;
 };
 
 /* One box of a level as the kernels see it. */

@@ -401,6 +401,10 @@ struct grhd_ctx
     long long cap_f[3] = {0, 0, 0};
     int2 *list = nullptr; /* FP64 face list of the guarded mixed path */
     long long cap_list = 0;
+    int2 *clist = nullptr; /* its guarded-cell list */
+    long long cap_clist = 0;
+    double *src = nullptr; /* FP64 sources of its guarded cells */
+    long long cap_src = 0;
     double *UA = nullptr, *UB = nullptr;
     long long cap_u = 0;
     double *acc = nullptr;
@@ -452,9 +456,10 @@ int ensure_scratch(grhd_ctx *c, bool stages, const grhd_params *params)
     const Plan &p = c->plan;
     if (guarded(make_options(params)))
     {
-        const int rc =
+        int rc =
             grow(&c->list, &c->cap_list, p.tot_f[0] + p.tot_f[1] + p.tot_f[2]);
-        if (rc)
+        if (rc || (rc = grow(&c->clist, &c->cap_clist, p.tot_i)) ||
+            (rc = grow(&c->src, &c->cap_src, 5 * p.tot_i)))
             return rc;
     }
     if (c->cap_g < p.tot_g)
@@ -611,6 +616,8 @@ Pools make_pools(grhd_ctx *c, double *state, int cb)
     for (int d = 0; d < 3; ++d)
         p.F[d] = c->F[d];
     p.list = c->list;
+    p.clist = c->clist;
+    p.src = c->src;
     p.st = c->dstat;
     return p;
 }
@@ -937,6 +944,15 @@ void rhs_sequence(grhd_ctx *c, int mode, const grhd_params *params,
     launch_faces(c, o, bs, p, s);
     CellArgs aa = a;
     aa.vol_h = h;
+    if (guarded(o) && o.use_static_sources)
+    {
+        const long long blocks = std::min(
+            kListBlocksMax, std::max(1ll, (c->plan.tot_i + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL(k_guard_sources, dim3((unsigned)blocks),
+                           dim3(kBlock), 0, s, c->plan.d_boxes, o, bs, p,
+                           a.us);
+        tick(c, GRHD_KERNEL_CELLS, s);
+    }
     launch_cells(c, mode, o, bs, bn, bp, bm, p, aa, s);
 }
 
@@ -1361,8 +1377,8 @@ int scratch_on_current_device(unsigned char **out)
  * offsets, 256-aligned). */
 struct NoAllocLayout
 {
-    size_t status, box, items[NCLASS], W, wbad, F[3], list, UA, UB, acc,
-        total;
+    size_t status, box, items[NCLASS], W, wbad, F[3], list, clist, src, UA,
+        UB, acc, total;
     int nblocks[NCLASS];
 };
 
@@ -1395,6 +1411,8 @@ NoAllocLayout noalloc_layout(const BoxDev &b, bool with_list)
     L.list = take(with_list ? sizeof(int2) * (size_t)(b.nface[0] + b.nface[1] +
                                                       b.nface[2])
                             : 0);
+    L.clist = take(with_list ? sizeof(int2) * (size_t)b.inpts : 0);
+    L.src = take(with_list ? sizeof(double) * 5 * (size_t)b.inpts : 0);
     L.UA = take(sizeof(double) * 5 * G);
     L.UB = take(sizeof(double) * 5 * G);
     L.acc = take(sizeof(double) * 5 * I);
@@ -1512,6 +1530,10 @@ int rk4_box_noalloc(const grhd_box *box, const grhd_params *params,
     {
         c.list = reinterpret_cast<int2 *>(base + L.list);
         c.cap_list = tot_f[0] + tot_f[1] + tot_f[2];
+        c.clist = reinterpret_cast<int2 *>(base + L.clist);
+        c.cap_clist = tot_i;
+        c.src = reinterpret_cast<double *>(base + L.src);
+        c.cap_src = 5 * tot_i;
     }
     c.UA = reinterpret_cast<double *>(base + L.UA);
     c.UB = reinterpret_cast<double *>(base + L.UB);
@@ -1925,6 +1947,8 @@ void grhd_ctx_destroy(grhd_ctx *c)
     for (int d = 0; d < 3; ++d)
         (void)hipFree(c->F[d]);
     (void)hipFree(c->list);
+    (void)hipFree(c->clist);
+    (void)hipFree(c->src);
     (void)hipFree(c->UA);
     (void)hipFree(c->UB);
     (void)hipFree(c->acc);

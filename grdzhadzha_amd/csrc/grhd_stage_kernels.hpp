@@ -22,6 +22,8 @@
  *   (walkers) or prim2con, HLLE -> face fluxes F_d (5 x nfaces), each
  *   k_faces_t<1,2> face once
  *   (tiles)
+ *   k_guard_sources  (guarded mixed path only) the FP64 static sources
+ *                of the guarded cells the FP32 face pass listed
  *   k_cells<M>   flux divergence in the reference's summation order,
  *                static-metric sources (:255-298), volume-time sources
  *                (:616-657), then per mode: write rhs, or an RK4 / SSPRK2
@@ -70,6 +72,13 @@ struct Pools
     /* guarded mixed path: the faces left to FP64, one entry each
      * ((direction << 30) | box, face index), as many as DevStatus::nlist */
     int2 *list;
+    /* guarded mixed path: the interior cells that carry kWGuard, one
+     * entry each (box, interior index), as many as DevStatus::ncell, and
+     * the FP64 static sources of those outside the metric-source
+     * atmosphere (k_guard_sources), 5 components over each box's interior,
+     * like the rhs pool */
+    int2 *clist;
+    double *src;
 };
 
 /* Per storage cell flags beside W (Pools::wbad): kWFailed when the cell's
@@ -552,11 +561,12 @@ __device__ __forceinline__ int2 list_entry(int box, int d, long long fo)
     return make_int2(box | (d << 30), (int)fo);
 }
 
-/* Append one entry per lane that takes one: a single atomic per
- * wavefront, entries in lane order from there.  Every lane of the
- * wavefront must reach it. */
-__device__ __forceinline__ void list_append_lane(const Pools &p, bool take,
-                                                 int2 entry)
+/* Append one entry per lane that takes one to `list` (length *count): a
+ * single atomic per wavefront, entries in lane order from there.  Every
+ * lane of the wavefront must reach it. */
+__device__ __forceinline__ void list_append(unsigned long long *count,
+                                            int2 *list, bool take,
+                                            int2 entry)
 {
     const unsigned long long bal = __ballot(take);
     if (bal == 0ull)
@@ -565,10 +575,17 @@ __device__ __forceinline__ void list_append_lane(const Pools &p, bool take,
     const int leader = __ffsll((long long)bal) - 1;
     unsigned long long base = 0ull;
     if (lane == leader)
-        base = atomicAdd(&p.st->nlist, (unsigned long long)__popcll(bal));
+        base = atomicAdd(count, (unsigned long long)__popcll(bal));
     base = __shfl(base, leader, 64);
     if (take)
-        p.list[base + __popcll(bal & ((1ull << lane) - 1ull))] = entry;
+        list[base + __popcll(bal & ((1ull << lane) - 1ull))] = entry;
+}
+
+/* The FP64 face list. */
+__device__ __forceinline__ void list_append_lane(const Pools &p, bool take,
+                                                 int2 entry)
+{
+    list_append(&p.st->nlist, p.list, take, entry);
 }
 
 /* Append the faces of a walk whose bits are set (bit q: face fo0 +
@@ -660,6 +677,17 @@ __device__ __forceinline__ void faces_x_block(const BoxDev *__restrict__ boxes,
         const int fn = shfl_next(fc);
         const bool to64 = makes_face && fp64_face(fc, fn);
         list_append_lane(p, to64, list_entry(bt.x, 0, fo));
+        /* the guarded interior cells, for k_guard_sources: each interior
+         * cell once, from the lane that holds it first */
+        if (o.use_static_sources)
+        {
+            const bool owns = live && lane != 63 && x >= 1 && x <= E - 2;
+            const long long ci =
+                ((long long)(k - b.lo[2]) * b.in[1] + (j - b.lo[1])) *
+                    b.in[0] + (i - b.lo[0]);
+            list_append(&p.st->ncell, p.clist, owns && (fc & kWGuard) != 0,
+                        make_int2(bt.x, (int)ci));
+        }
         mine = makes_face && !to64;
         if (!__any(mine))
             return;
@@ -1056,11 +1084,15 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-/* Empty the FP64 face list before a stage's FP32 pass. */
+/* Empty the FP64 face list and the guarded-cell list before a stage's
+ * FP32 pass. */
 __global__ void k_list_reset(DevStatus *st)
 {
     if (threadIdx.x == 0)
+    {
         st->nlist = 0ull;
+        st->ncell = 0ull;
+    }
 }
 
 /* Mixed precision: the static-metric sources of a cell in FP32 (the
@@ -1187,6 +1219,48 @@ struct CellArgs
     unsigned long long next_eval;
 };
 
+/* The guarded mixed path's FP64 static sources: for each interior cell
+ * that the FP32 face pass listed (its recovery set kWGuard) and that lies
+ * outside the metric-source atmosphere, the same geometry, floors, metric
+ * derivatives and sources as the FP64 path's k_cells (:255-298), into
+ * Pools::src, which k_cells then adds where the FP64 path adds its own.
+ * Grid-stride over the DevStatus::ncell entries. */
+__global__ void __launch_bounds__(kBlock)
+    k_guard_sources(const BoxDev *__restrict__ boxes, Options o,
+                    Background bg_s, Pools p, int us_pool)
+{
+    const unsigned long long cnt = p.st->ncell;
+    for (unsigned long long q = (unsigned long long)blockIdx.x * kBlock +
+                                threadIdx.x;
+         q < cnt; q += (unsigned long long)gridDim.x * kBlock)
+    {
+        const int2 e = p.clist[q];
+        const BoxDev &b = boxes[e.x];
+        const long long it = e.y;
+        const int i = b.lo[0] + (int)(it % b.in[0]);
+        const int j = b.lo[1] + (int)((it / b.in[0]) % b.in[1]);
+        const int k = b.lo[2] + (int)(it / ((long long)b.in[0] * b.in[1]));
+        const long long so = goff(b, i, j, k);
+        const long long n = b.gnpts;
+        const double *W = p.W + 6 * b.off_g;
+        double x[3];
+        coords(i, j, k, b.dx, b.center, x);
+        Geo geo_s;
+        GeoCore core;
+        cell_geometry_core(bg_s, x, geo_s, core);
+        Prim pw = load_prim(W, n, so);
+        floors(pw, o, geo_s.g);
+        if (is_atmosphere(pw, o, o.source_atmosphere_factor))
+            continue;
+        const Cons us = load_cons(pool_cons(p, us_pool, b), n, so);
+        FullGeo fg;
+        cell_derivs(bg_s, x, core, geo_s, fg);
+        Cons src;
+        static_sources(pw, us, o.gamma, fg, src);
+        store_cons(p.src + 5 * b.off_i, b.inpts, it, src);
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* k_cells<MODE>: one thread per interior cell.                        */
 /* ------------------------------------------------------------------ */
@@ -1268,12 +1342,25 @@ __global__ void __launch_bounds__(kBlock)
                 rhs.S[1] += (double)src.S[1];
                 rhs.S[2] += (double)src.S[2];
             }
-            else
+            else if constexpr (!S32)
             {
                 FullGeo fg;
                 cell_derivs(bg_s, x, core, geo_s, fg);
                 Cons src;
                 static_sources(pw, us, o.gamma, fg, src);
+                rhs.D += src.D;
+                rhs.tau += src.tau;
+                rhs.S[0] += src.S[0];
+                rhs.S[1] += src.S[1];
+                rhs.S[2] += src.S[2];
+            }
+            else
+            {
+                /* a guarded cell of the mixed path: its FP64 sources,
+                 * computed the FP64 path's way by k_guard_sources, which
+                 * keeps the derivative half of the metric out of this
+                 * kernel's registers */
+                const Cons src = load_cons(p.src + 5 * b.off_i, b.inpts, it);
                 rhs.D += src.D;
                 rhs.tau += src.tau;
                 rhs.S[0] += src.S[0];

@@ -317,7 +317,8 @@ int grhd_advance_rk4_box(grhd_ctx *ctx, const grhd_box *box,
 
 /* Bytes of device scratch grhd_advance_rk4_box_noalloc may use per GPU:
  * enough for a box of up to 72^3 interior cells with 3 ghost layers
- * (64^3 needs 83 MiB). */
+ * (64^3 needs 83 MiB), or 68^3 on the guarded mixed path, whose FP64 face
+ * list and guarded-cell sources take room too. */
 #define GRHD_NOALLOC_SCRATCH_BYTES (128ull << 20)
 
 /* grhd_advance_rk4_box without a context and without allocating: no
