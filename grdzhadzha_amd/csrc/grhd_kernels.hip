@@ -859,6 +859,21 @@ void launch_faces(grhd_ctx *c, const Options &o, const Background &bg,
         launch_faces_t<double>(c, o, bg, p, s);
 }
 
+/* One stage mode's cell kernel: the FP64 path's, or the mixed paths'. */
+template <int M, class FT, bool S32>
+void launch_cells_mode(grhd_ctx *c, const Options &o, const Background &bs,
+                       const Background &bn, const Background &bp,
+                       const Background &bm, const Pools &p,
+                       const CellArgs &a, hipStream_t s)
+{
+    if constexpr (S32)
+        GRHD_LAUNCH(c, C_INT, (k_cells_mixed<M, FT>), s, o, bs, bn, bp, bm, p,
+                    a);
+    else
+        GRHD_LAUNCH(c, C_INT, (k_cells<M, FT, false>), s, o, bs, bn, bp, bm,
+                    p, a);
+}
+
 template <class FT, bool S32 = (sizeof(FT) < sizeof(double))>
 void launch_cells_t(grhd_ctx *c, int mode, const Options &o,
                     const Background &bs, const Background &bn,
@@ -868,32 +883,25 @@ void launch_cells_t(grhd_ctx *c, int mode, const Options &o,
     switch (mode)
     {
     case MODE_RHS:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RHS, FT, S32>), s, o, bs, bn, bp, bm,
-                    p, a);
+        launch_cells_mode<MODE_RHS, FT, S32>(c, o, bs, bn, bp, bm, p, a, s);
         break;
     case MODE_RK4_1:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_1, FT, S32>), s, o, bs, bn, bp,
-                    bm, p, a);
+        launch_cells_mode<MODE_RK4_1, FT, S32>(c, o, bs, bn, bp, bm, p, a, s);
         break;
     case MODE_RK4_2:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_2, FT, S32>), s, o, bs, bn, bp,
-                    bm, p, a);
+        launch_cells_mode<MODE_RK4_2, FT, S32>(c, o, bs, bn, bp, bm, p, a, s);
         break;
     case MODE_RK4_3:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_3, FT, S32>), s, o, bs, bn, bp,
-                    bm, p, a);
+        launch_cells_mode<MODE_RK4_3, FT, S32>(c, o, bs, bn, bp, bm, p, a, s);
         break;
     case MODE_RK4_4:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_RK4_4, FT, S32>), s, o, bs, bn, bp,
-                    bm, p, a);
+        launch_cells_mode<MODE_RK4_4, FT, S32>(c, o, bs, bn, bp, bm, p, a, s);
         break;
     case MODE_SSP_1:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_1, FT, S32>), s, o, bs, bn, bp,
-                    bm, p, a);
+        launch_cells_mode<MODE_SSP_1, FT, S32>(c, o, bs, bn, bp, bm, p, a, s);
         break;
     default:
-        GRHD_LAUNCH(c, C_INT, (k_cells<MODE_SSP_2, FT, S32>), s, o, bs, bn, bp,
-                    bm, p, a);
+        launch_cells_mode<MODE_SSP_2, FT, S32>(c, o, bs, bn, bp, bm, p, a, s);
         break;
     }
     tick(c, GRHD_KERNEL_CELLS, s);

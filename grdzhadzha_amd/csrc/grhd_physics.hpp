@@ -101,6 +101,29 @@ template <class T> __device__ __forceinline__ T std_min(T a, T b)
 {
     return (b < a) ? b : a;
 }
+/* Division and square root in the templated numerics: the IEEE operations
+ * for double (and the adjoint's dual numbers), so the FP64 path keeps the
+ * reference's rounding; for float (the mixed-precision faces and sources)
+ * the hardware's reciprocal and square root, about one ulp, without the
+ * range-scaling steps of the general FP32 sequences (their values lie far
+ * from the denormal and overflow ranges those steps guard). */
+template <class T> __device__ __forceinline__ T qdiv(T a, T b)
+{
+    return a / b;
+}
+__device__ __forceinline__ float qdiv(float a, float b)
+{
+    return a * __builtin_amdgcn_rcpf(b);
+}
+template <class T> __device__ __forceinline__ T qsqrt(T a)
+{
+    return sqrt(a);
+}
+__device__ __forceinline__ float qsqrt(float a)
+{
+    return __builtin_amdgcn_sqrtf(a);
+}
+
 template <class T = double> __device__ __forceinline__ T kdelta(int i, int j)
 {
     return (i == j) ? T(1) : T(0);
@@ -136,7 +159,7 @@ template <class T>
 __device__ __forceinline__ void inverse_sym(const T m[3][3], T out[3][3])
 {
     const T deth = det_sym(m);
-    const T inv = T(1) / deth;
+    const T inv = qdiv(T(1), deth);
     out[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[1][2]) * inv;
     out[0][1] = (m[0][2] * m[1][2] - m[0][1] * m[2][2]) * inv;
     out[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv;
@@ -199,7 +222,7 @@ __device__ __forceinline__ void coords(int i, int j, int k, double dx,
 
 template <class T> __device__ __forceinline__ T get_radius(const T x[3])
 {
-    const T r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const T r = qsqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
     return std_max(r, T(1e-6));
 }
 
@@ -220,13 +243,13 @@ __device__ __forceinline__ void ks_core(T M, T a, const T x[3], KSCoreT<T> &c)
     c.rho = get_radius(x);
     c.rho2 = c.rho * c.rho;
     c.r2 = T(0.5) * (c.rho2 - a2) +
-           sqrt(T(0.25) * (c.rho2 - a2) * (c.rho2 - a2) + a2 * z * z);
-    c.r = sqrt(c.r2);
-    c.cos_theta = z / c.r;
-    c.H = M * c.r / (c.r2 + a2 * c.cos_theta * c.cos_theta);
-    c.el[0] = (c.r * x[0] + a * x[1]) / (c.r2 + a2);
-    c.el[1] = (c.r * x[1] - a * x[0]) / (c.r2 + a2);
-    c.el[2] = z / c.r;
+           qsqrt(T(0.25) * (c.rho2 - a2) * (c.rho2 - a2) + a2 * z * z);
+    c.r = qsqrt(c.r2);
+    c.cos_theta = qdiv(z, c.r);
+    c.H = qdiv(M * c.r, c.r2 + a2 * c.cos_theta * c.cos_theta);
+    c.el[0] = qdiv(c.r * x[0] + a * x[1], c.r2 + a2);
+    c.el[1] = qdiv(c.r * x[1] - a * x[0], c.r2 + a2);
+    c.el[2] = qdiv(z, c.r);
 }
 
 /* gamma, gamma^-1, lapse, shift (KerrSchild.hpp:79-116 with
@@ -237,7 +260,7 @@ __device__ __forceinline__ void ks_geometry_core(T M, T a, const T x[3],
 {
     ks_core(M, a, x, c);
     const T el_t = T(1);
-    g.lapse = T(1) / sqrt(T(1) + T(2) * c.H * el_t * el_t);
+    g.lapse = qdiv(T(1), qsqrt(T(1) + T(2) * c.H * el_t * el_t));
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -251,7 +274,7 @@ __device__ __forceinline__ void ks_geometry_core(T M, T a, const T x[3],
          * gamma^ij = delta^ij - q l^i l^j, beta^i = q l^i, with
          * q = 2H / (1 + 2H l.l). */
         const T ll = c.el[0] * c.el[0] + c.el[1] * c.el[1] + c.el[2] * c.el[2];
-        const T q = T(2) * c.H / (T(1) + T(2) * c.H * ll);
+        const T q = qdiv(T(2) * c.H, T(1) + T(2) * c.H * ll);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
@@ -303,7 +326,7 @@ __device__ __forceinline__ void lie_extrinsic_curvature(const GeoT<T> &g,
                                                         const T dshift[3][3],
                                                         T kdiag, T K[3][3])
 {
-    const T half_over_lapse = T(0.5) / g.lapse;
+    const T half_over_lapse = qdiv(T(0.5), g.lapse);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -340,33 +363,33 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        const T drhodx = xc[i] / rho;
-        drdx[i] = T(0.5) / r *
+        const T drhodx = qdiv(xc[i], rho);
+        drdx[i] = qdiv(T(0.5), r) *
                   (rho * drhodx +
-                   T(0.5) / sqrt(T(0.25) * (rho2 - a2) * (rho2 - a2) + a2 * z * z) *
+                   qdiv(T(0.5), qsqrt(T(0.25) * (rho2 - a2) * (rho2 - a2) + a2 * z * z)) *
                        (drhodx * rho * (rho2 - a2) +
                         kdelta<T>(i, 2) * T(2.0) * a2 * z));
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        const T dcosthetadx = -z / r2 * drdx[i] + kdelta<T>(i, 2) / r;
-        dHdx[i] = H * (drdx[i] / r -
-                       T(2.0) / (r2 + a2 * cos_theta2) *
+        const T dcosthetadx = qdiv(-z, r2) * drdx[i] + qdiv(kdelta<T>(i, 2), r);
+        dHdx[i] = H * (qdiv(drdx[i], r) -
+                       qdiv(T(2.0), r2 + a2 * cos_theta2) *
                            (r * drdx[i] + a2 * c.cos_theta * dcosthetadx));
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        dldx[0][i] =
-            (xc[0] * drdx[i] + r * kdelta<T>(i, 0) + a * kdelta<T>(i, 1) -
-             T(2.0) * r * drdx[i] * (r * xc[0] + a * xc[1]) / (r2 + a2)) /
-            (r2 + a2);
-        dldx[1][i] =
-            (xc[1] * drdx[i] + r * kdelta<T>(i, 1) - a * kdelta<T>(i, 0) -
-             T(2.0) * r * drdx[i] * (r * xc[1] - a * xc[0]) / (r2 + a2)) /
-            (r2 + a2);
-        dldx[2][i] = -xc[2] * drdx[i] / r2 + kdelta<T>(i, 2) / r;
+        dldx[0][i] = qdiv(
+            xc[0] * drdx[i] + r * kdelta<T>(i, 0) + a * kdelta<T>(i, 1) -
+                qdiv(T(2.0) * r * drdx[i] * (r * xc[0] + a * xc[1]), r2 + a2),
+            r2 + a2);
+        dldx[1][i] = qdiv(
+            xc[1] * drdx[i] + r * kdelta<T>(i, 1) - a * kdelta<T>(i, 0) -
+                qdiv(T(2.0) * r * drdx[i] * (r * xc[1] - a * xc[0]), r2 + a2),
+            r2 + a2);
+        dldx[2][i] = qdiv(-xc[2] * drdx[i], r2) + qdiv(kdelta<T>(i, 2), r);
     }
     const T dltdx = T(0.0);
 
@@ -496,7 +519,7 @@ __device__ __forceinline__ void pn_add_point(PotentialsT<T> &p, T mass, T soft,
         disp[i] = pos[i] - src[i];
         r2 += disp[i] * disp[i];
     }
-    const T ir = T(1) / sqrt(r2);
+    const T ir = qdiv(T(1), qsqrt(r2));
     const T ir3 = ir * ir * ir;
     p.scalar += mass * ir;
     T ddv = T(0);
@@ -555,18 +578,18 @@ __device__ __forceinline__ void pn_geometry_pot(const Background &bg,
     for (int i = 0; i < 3; ++i)
         vsq += p.vec[i] * p.vec[i];
     const T lapse_squared =
-        T(1) - T(2) * U + T(2) * U * U + T(16) * vsq / cf;
+        T(1) - T(2) * U + T(2) * U * U + qdiv(T(16) * vsq, cf);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             g.g[i][j] = T(0);
-    g.lapse = sqrt(lapse_squared);
+    g.lapse = qsqrt(lapse_squared);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
         g.g[i][i] = cf;
-        g.shift[i] = T(-4) * p.vec[i] / cf;
+        g.shift[i] = qdiv(T(-4) * p.vec[i], cf);
     }
     inverse_sym(g.g, g.gu);
 }
@@ -594,7 +617,7 @@ __device__ __forceinline__ void pn_derivs(const PotentialsT<T> &p,
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         vsq += p.vec[i] * p.vec[i];
-    const T lapse_squared = T(1.0) - T(2.0) * U + T(2.0) * U * U + T(16.0) * vsq / cf;
+    const T lapse_squared = T(1.0) - T(2.0) * U + T(2.0) * U * U + qdiv(T(16.0) * vsq, cf);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
@@ -623,11 +646,11 @@ __device__ __forceinline__ void pn_derivs(const PotentialsT<T> &p,
         {
             dvsq += T(2.0) * p.vec[c] * p.dvec[c][d];
             f.dshift[c][d] =
-                -T(4.0) * (p.dvec[c][d] / cf - p.vec[c] * dcf / (cf * cf));
+                -T(4.0) * (qdiv(p.dvec[c][d], cf) - qdiv(p.vec[c] * dcf, cf * cf));
         }
         const T dls = (-T(2.0) + T(4.0) * U) * dU +
-                           T(16.0) * (dvsq / cf - vsq * dcf / (cf * cf));
-        f.dlapse[d] = T(0.5) * dls / g.lapse;
+                           T(16.0) * (qdiv(dvsq, cf) - qdiv(vsq * dcf, cf * cf));
+        f.dlapse[d] = qdiv(T(0.5) * dls, g.lapse);
     }
     if constexpr (sizeof(T) < sizeof(double))
         lie_extrinsic_curvature(g, f.dg, f.dshift, dcf_dt, f.K);
@@ -777,12 +800,12 @@ __device__ __forceinline__ void prim2con(const PrimT<T> &p, T gamma,
          * same quantity without cancellation, from x = W^2 - 1 =
          * v^2 / (1 - v^2):  tau = rho W x/(W + 1) + rho eps W^2 + P x. */
         const T v2 = vsq(p.v, g);
-        const T x = v2 / (T(1) - v2);
-        const T W = T(1) / sqrt(T(1) - v2);
-        const T h = T(1) + p.eps + p.pressure / p.rho;
+        const T x = qdiv(v2, T(1) - v2);
+        const T W = qdiv(T(1), qsqrt(T(1) - v2));
+        const T h = T(1) + p.eps + qdiv(p.pressure, p.rho);
         const T rhW2 = p.rho * h * W * W;
         c.D = p.rho * W;
-        c.tau = p.rho * W * (x / (W + T(1))) + p.rho * p.eps * W * W +
+        c.tau = p.rho * W * qdiv(x, W + T(1)) + p.rho * p.eps * W * W +
                 p.pressure * x;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -790,8 +813,8 @@ __device__ __forceinline__ void prim2con(const PrimT<T> &p, T gamma,
     }
     else
     {
-        const T W = T(1) / sqrt(T(1) - vsq(p.v, g));
-        const T h = T(1) + p.eps + p.pressure / p.rho;
+        const T W = qdiv(T(1), qsqrt(T(1) - vsq(p.v, g)));
+        const T h = T(1) + p.eps + qdiv(p.pressure, p.rho);
         const T rhW2 = p.rho * h * W * W;
         c.D = p.rho * W;
         c.tau = rhW2 - p.pressure - c.D;
@@ -950,12 +973,12 @@ __device__ __forceinline__ void floors_g(PrimT<T> &p, const Options &o,
     const T v2 = vsq(p.v, g);
     if (v2 >= T(o.max_v2))
     {
-        const T scale = sqrt(T(o.max_v2) / v2);
+        const T scale = qsqrt(qdiv(T(o.max_v2), v2));
 #pragma unroll
         for (int i = 0; i < 3; ++i)
             p.v[i] *= scale;
     }
-    p.eps = p.pressure / ((gamma - T(1)) * p.rho);
+    p.eps = qdiv(p.pressure, (gamma - T(1)) * p.rho);
 }
 
 template <class T>
@@ -1038,16 +1061,16 @@ __device__ __forceinline__ void char_speeds(const PrimT<T> &p, T gamma,
                                             T &upper)
 {
     const T v2 = vsq(p.v, g.g);
-    const T h = T(1) + p.eps + p.pressure / p.rho;
-    const T cs2 = gamma * p.pressure / (p.rho * h);
-    const T cs = sqrt(cs2);
+    const T h = T(1) + p.eps + qdiv(p.pressure, p.rho);
+    const T cs2 = qdiv(gamma * p.pressure, p.rho * h);
+    const T cs = qsqrt(cs2);
     const T vel = p.v[d];
     const T den = T(1) - v2 * cs2;
     const T rad = (T(1) - v2) *
                   (g.gu[d][d] * (T(1) - v2 * cs2) - vel * vel * (T(1) - cs2));
-    const T root = cs * sqrt(std_max(T(0), rad));
-    const T lo = (vel * (T(1) - cs2) - root) / den;
-    const T hi = (vel * (T(1) - cs2) + root) / den;
+    const T root = cs * qsqrt(std_max(T(0), rad));
+    const T lo = qdiv(vel * (T(1) - cs2) - root, den);
+    const T hi = qdiv(vel * (T(1) - cs2) + root, den);
     lower = g.lapse * lo - g.shift[d];
     upper = g.lapse * hi - g.shift[d];
 }
@@ -1082,7 +1105,7 @@ __device__ __forceinline__ void hlle_face_g(PrimT<T> lp, PrimT<T> rp,
      * value, possibly 0 * inf, is discarded. */
     const bool left = sl >= T(0);
     const bool right = su <= T(0);
-    const T inv = T(1) / (su - sl);
+    const T inv = qdiv(T(1), su - sl);
     const T hD = (su * fl.D - sl * fr.D + sl * su * (rc.D - lc.D)) * inv;
     const T hT =
         (su * fl.tau - sl * fr.tau + sl * su * (rc.tau - lc.tau)) * inv;
@@ -1136,8 +1159,8 @@ __device__ __forceinline__ void static_sources(const PrimT<T> &p,
 {
     (void)gamma; /* the EOS argument of the reference's signature */
     const GeoT<G> &g = f.geo;
-    const T W = T(1.0) / sqrt(T(1.0) - vsq(p.v, g.g));
-    const T h = T(1.0) + p.eps + p.pressure / p.rho;
+    const T W = qdiv(T(1.0), qsqrt(T(1.0) - vsq(p.v, g.g)));
+    const T h = T(1.0) + p.eps + qdiv(p.pressure, p.rho);
     const T rhW2 = p.rho * h * W * W;
     T Tuu[3][3];
 #pragma unroll

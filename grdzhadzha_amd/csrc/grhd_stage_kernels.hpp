@@ -1267,11 +1267,11 @@ __global__ void __launch_bounds__(kBlock)
 /* FT: the type of the face-flux pool.  S32: the mixed paths' FP32
  * sources may run (sources_in_fp32); false on the FP64 path, whose
  * instantiations carry no FP32 code. */
-template <int MODE, class FT, bool S32 = (sizeof(FT) < sizeof(double))>
-__global__ void __launch_bounds__(kBlock)
-    k_cells(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
-            Background bg_s, Background bg_next, Background bg_plus,
-            Background bg_minus, Pools p, CellArgs a)
+template <int MODE, class FT, bool S32>
+__device__ __forceinline__ void
+cells_block(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items,
+            Options o, Background bg_s, Background bg_next,
+            Background bg_plus, Background bg_minus, Pools p, CellArgs a)
 {
     const BoxDev *bp;
     const long long it = block_item(items, boxes, C_INT, bp);
@@ -1538,6 +1538,32 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (MODE != MODE_RHS && a.count)
         block_add_counts(p.st, cnt[0], cnt[1], cnt[2], cnt[3]);
+}
+
+/* The FP64 path's cell kernel. */
+template <int MODE, class FT, bool S32 = (sizeof(FT) < sizeof(double))>
+__global__ void __launch_bounds__(kBlock)
+    k_cells(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items, Options o,
+            Background bg_s, Background bg_next, Background bg_plus,
+            Background bg_minus, Pools p, CellArgs a)
+{
+    cells_block<MODE, FT, S32>(boxes, items, o, bg_s, bg_next, bg_plus,
+                               bg_minus, p, a);
+}
+
+/* The mixed paths' cell kernel (FP32 sources; plain mixed with a float
+ * flux pool, guarded mixed with a double one), held to 168 VGPRs, three
+ * waves per SIMD: without its FP64 metric derivatives it needs about that
+ * many, and the compiler's own choice of a few more would halve the
+ * waves. */
+template <int MODE, class FT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+    k_cells_mixed(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items,
+                  Options o, Background bg_s, Background bg_next,
+                  Background bg_plus, Background bg_minus, Pools p, CellArgs a)
+{
+    cells_block<MODE, FT, true>(boxes, items, o, bg_s, bg_next, bg_plus,
+                                bg_minus, p, a);
 }
 
 /* Post-update recovery of the interior of the state pool. */
