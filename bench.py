@@ -38,6 +38,10 @@ METRIC = ("GRHD cell-updates/s per RK4 step, plain FP64 vs mixed/batched "
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 RIDGE_FLOP_PER_BYTE = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+# VALU issue model: a wave64 instruction holds a SIMD 4 cycles, an FP64
+# transcendental (rcp, rsq) 16; 1024 SIMDs at the 2.4 GHz peak clock
+SIMDS = 1024
+PEAK_CLOCK_HZ = 2.4e9
 
 # Which parts of the de-duplicated per-cell count (profiles/
 # algorithmic_counts.json, flop_per_cell_stage_by_part) each kernel class
@@ -111,6 +115,26 @@ def measured_step_bytes(n):
         return None
     with open(path) as f:
         return json.load(f).get("bytes_per_rk4_step")
+
+
+def valu_issue(cls, n, avg_launch_s):
+    """The kernel class's issue time at peak, from the committed counts of
+    VALU (and FP64 transcendental) wavefront instructions per launch on the
+    128^3 box, as a fraction of its measured launch time; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "fp64_flops.json")
+    if n != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    insts = data.get("valu_insts_per_launch", {}).get(cls)
+    trans = data.get("trans_f64_per_launch", {}).get(cls)
+    if insts is None or trans is None:
+        return None
+    issue_s = (4.0 * insts + 12.0 * trans) / (SIMDS * PEAK_CLOCK_HZ)
+    return {"valu_insts_per_launch": insts,
+            "trans_f64_per_launch": trans,
+            "issue_ms_at_peak": issue_s * 1e3,
+            "frac": issue_s / avg_launch_s}
 
 
 def measured_flops(cls, n):
@@ -801,6 +825,9 @@ def main():
     issued = measured_flops(dom, args.n)
     if issued:
         common["issued_fp64_flop_per_unit"] = issued / units
+    vi = valu_issue(dom, args.n, avg_s)
+    if vi:
+        common["valu_issue"] = vi
     if counts and dom in KERNEL_PARTS:
         parts = counts["flop_per_cell_stage_by_part"]
         # faces: one face per direction per cell, so 3 faces per cell
