@@ -134,6 +134,7 @@ SIGNATURES = {
                                           _vp]),
     "grhd_cell_geometry_box": (_i, [_vp, _P(GrhdBox), _P(GrhdBackground), _d,
                                     _vp, _vp]),
+    "grhd_fp64_div_sqrt": (_i, [_vp, _vp, _vp, _vp, ctypes.c_longlong, _vp]),
     "grhd_level_compute_rhs": (_i, [_vp, _P(GrhdLevelBox), _i,
                                     _P(GrhdParams), _P(GrhdBackground), _d,
                                     _vp, _i, _i, _vp, _vp, _P(GrhdError),

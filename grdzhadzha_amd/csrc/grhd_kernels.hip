@@ -2132,6 +2132,40 @@ int grhd_cell_geometry_box(grhd_ctx *ctx, const grhd_box *box,
     return hipGetLastError() == hipSuccess ? GRHD_OK : GRHD_ERR_DEVICE;
 }
 
+/* The kernels' FP64 division and square root (qdiv, qsqrt), elementwise,
+ * grid-stride so a grid of at most kMaxDivSqrtBlocks covers any n. */
+constexpr unsigned kMaxDivSqrtBlocks = 4096;
+
+__global__ void __launch_bounds__(kBlock)
+    k_div_sqrt(const double *__restrict__ a, const double *__restrict__ b,
+               double *__restrict__ q, double *__restrict__ r, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < n; t += stride)
+    {
+        q[t] = qdiv(a[t], b[t]);
+        r[t] = qsqrt(a[t]);
+    }
+}
+
+int grhd_fp64_div_sqrt(const double *a, const double *b, double *quotient,
+                       double *root, long long n, void *stream)
+{
+    if (n < 0 || (n > 0 && (!a || !b || !quotient || !root)))
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (n == 0)
+        return GRHD_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned grid = std::min<long long>(grid_of(n), kMaxDivSqrtBlocks);
+    hipLaunchKernelGGL(k_div_sqrt, dim3(grid), dim3(kBlock), 0, s, a, b,
+                       quotient, root, n);
+    if (hipGetLastError() != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return GRHD_ERR_DEVICE;
+    return GRHD_OK;
+}
+
 /* ---------------- a level ---------------- */
 int grhd_level_compute_rhs(grhd_ctx *ctx, const grhd_level_box *boxes,
                            int nbox, const grhd_params *params,

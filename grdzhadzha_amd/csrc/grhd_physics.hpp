@@ -8,8 +8,9 @@
  * translation unit is compiled with -ffp-contract=off: no multiply-add is
  * fused, as on the CPU.  pow(x, -0.5), pow(x, 2), pow(x, 3) of the
  * Kerr-Schild lapse are written 1/sqrt(x), x*x, x*x*x, and the natural log
- * is grhd_log below, so only IEEE-exact operations (+ - * / sqrt) and
- * frexp appear.
+ * is grhd_log below, so only correctly rounded operations (+ - *, and the
+ * division and square root of qdiv and qsqrt, IEEE over the ranges stated
+ * there) and frexp appear.
  *
  * Citations are to the reference repository (Source/...).
  */
@@ -101,12 +102,34 @@ template <class T> __device__ __forceinline__ T std_min(T a, T b)
 {
     return (b < a) ? b : a;
 }
-/* Division and square root in the templated numerics: the IEEE operations
- * for double (and the adjoint's dual numbers), so the FP64 path keeps the
- * reference's rounding; for float (the mixed-precision faces and sources)
- * the hardware's reciprocal and square root, about one ulp, without the
- * range-scaling steps of the general FP32 sequences (their values lie far
- * from the denormal and overflow ranges those steps guard). */
+/* Division and square root of the device numerics.
+ *
+ * float (the mixed-precision faces and sources): the hardware reciprocal
+ * and square root, about one ulp, without the range-scaling steps of the
+ * general FP32 sequences (their values lie far from the denormal and
+ * overflow ranges those steps guard).
+ *
+ * double: correctly rounded, as IEEE a / b and sqrt(a).  The compiler's
+ * FP64 division is a Newton-Raphson refinement of the hardware reciprocal
+ * (v_rcp_f64, two steps), the quotient corrected once by its residual,
+ * wrapped in v_div_scale (both operands scaled by 2^+-128 when their
+ * exponents would make an intermediate overflow or go subnormal) and
+ * v_div_fixup (zeros, infinities, NaN, overflow, underflow).  qdiv keeps
+ * the refinement and v_div_fixup and drops the scaling: 9 instructions
+ * instead of 11, none of which writes VCC, and the refined reciprocal of
+ * a denominator is shared by every division by it.  Where the scaling
+ * would do nothing -- a = 0 or |a| >= 2^-969, b normal with |b| < 2^1022,
+ * exponent(a) - exponent(b) < 768, a normal quotient -- the instructions
+ * are the compiler's, so the result is the IEEE quotient bit for bit;
+ * zero, infinite and NaN operands get the IEEE result from v_div_fixup.
+ * The FP64 square root likewise drops the 2^256 scaling of arguments
+ * below 2^-767: above that bound (and for +-0, +inf, negative and NaN
+ * arguments) it is the IEEE square root bit for bit.  Outside these
+ * ranges (operands below about 1e-290 or above 1e300) a result may differ
+ * from IEEE's by amounts of that size.  The kernels' values (densities
+ * and pressures at or above their floors, metric components, velocities
+ * that are zero or physical) lie inside them.  The adjoint's dual numbers
+ * use their own operators. */
 template <class T> __device__ __forceinline__ T qdiv(T a, T b)
 {
     return a / b;
@@ -115,6 +138,17 @@ __device__ __forceinline__ float qdiv(float a, float b)
 {
     return a * __builtin_amdgcn_rcpf(b);
 }
+__device__ __forceinline__ double qdiv(double a, double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    const double q = a * r;
+    const double rem = fma(-b, q, a);
+    return __builtin_amdgcn_div_fixup(fma(rem, r, q), b, a);
+}
 template <class T> __device__ __forceinline__ T qsqrt(T a)
 {
     return sqrt(a);
@@ -122,6 +156,23 @@ template <class T> __device__ __forceinline__ T qsqrt(T a)
 __device__ __forceinline__ float qsqrt(float a)
 {
     return __builtin_amdgcn_sqrtf(a);
+}
+/* Goldschmidt-style refinement of the hardware reciprocal square root
+ * y ~ 1/sqrt(x): s ~ sqrt(x) and h ~ y/2 improved together, then two
+ * residual corrections of s; +-0 and +inf (class mask 0x260) return x. */
+__device__ __forceinline__ double qsqrt(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, s, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    double d = fma(-s, s, x);
+    s = fma(d, h, s);
+    d = fma(-s, s, x);
+    s = fma(d, h, s);
+    return __builtin_amdgcn_class(x, 0x260) ? x : s;
 }
 
 template <class T = double> __device__ __forceinline__ T kdelta(int i, int j)
@@ -189,7 +240,7 @@ __device__ __forceinline__ double grhd_log(double x)
         m = m * 2.0;
         k = k - 1;
     }
-    const double s = (m - 1.0) / (m + 1.0);
+    const double s = qdiv(m - 1.0, m + 1.0);
     const double s2 = s * s;
     double p = 1.0 / 25.0;
     p = 1.0 / 23.0 + s2 * p;
@@ -470,7 +521,7 @@ __device__ __forceinline__ void ks_derivs(T M, T a, const T xc[3],
                 for (int m = 0; m < 3; ++m)
                     Kij += -T(2.0) * ULL * g.g[k][m] * g.shift[m];
             }
-            Kij *= T(0.5) / g.lapse;
+            Kij *= qdiv(T(0.5), g.lapse);
             f.K[i][j] = Kij;
         }
 
@@ -680,7 +731,7 @@ __device__ __forceinline__ void pn_derivs(const PotentialsT<T> &p,
             }
             if (i == j)
                 Kij -= dcf_dt;
-            Kij *= T(0.5) / g.lapse;
+            Kij *= qdiv(T(0.5), g.lapse);
             f.K[i][j] = Kij;
         }
 #pragma unroll
@@ -838,10 +889,10 @@ __device__ __forceinline__ double presid_at(double p, double D, double tau,
                                             double max_v2, PresidAt &s)
 {
     s.epp = tau + D + p;
-    s.v2raw = S2 / (s.epp * s.epp);
+    s.v2raw = qdiv(S2, s.epp * s.epp);
     s.v2 = s.v2raw > max_v2 ? max_v2 : s.v2raw;
-    s.W = 1.0 / sqrt(1.0 - s.v2);
-    return D * s.W + gamma * p * s.W * s.W / (gamma - 1.0) - s.epp;
+    s.W = qdiv(1.0, qsqrt(1.0 - s.v2));
+    return D * s.W + qdiv(gamma * p * s.W * s.W, gamma - 1.0) - s.epp;
 }
 
 __device__ __forceinline__ double presid(double p, double D, double tau,
@@ -858,9 +909,9 @@ __device__ __forceinline__ double presid_deriv_at(double p, double D,
                                                   const PresidAt &s)
 {
     const double W = s.W;
-    const double dW_dp = -W * W * W * s.v2 / s.epp;
+    const double dW_dp = qdiv(-W * W * W * s.v2, s.epp);
     const double dW2_dp = 2.0 * W * dW_dp;
-    return D * dW_dp + gamma * (W * W + p * dW2_dp) / (gamma - 1.0) - 1.0;
+    return D * dW_dp + qdiv(gamma * (W * W + p * dW2_dp), gamma - 1.0) - 1.0;
 }
 
 /* recover_primitive (Valencia.hpp:116-228).  Returns success; `residual`
@@ -916,7 +967,7 @@ __device__ __forceinline__ bool con2prim(const Cons &c, const Options &o,
                 lower = p;
         }
         const double der = presid_deriv_at(p, c.D, gamma, at);
-        double np = p - res / der;
+        double np = p - qdiv(res, der);
         if (!isfinite(np) || np <= lower || np >= upper)
             np = 0.5 * (lower + upper);
         p = np;
@@ -930,16 +981,16 @@ __device__ __forceinline__ bool con2prim(const Cons &c, const Options &o,
     if (at.v2raw >= mv2)
         return false;
     const double W = at.W;
-    out.rho = c.D / W;
+    out.rho = qdiv(c.D, W);
     out.pressure = p;
-    out.eps = p / ((gamma - 1.0) * out.rho);
+    out.eps = qdiv(p, (gamma - 1.0) * out.rho);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
         double vi = 0.0;
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            vi += gu[i][j] * c.S[j] / epp;
+            vi += qdiv(gu[i][j] * c.S[j], epp);
         out.v[i] = vi;
     }
     return true;
@@ -992,7 +1043,7 @@ __device__ __forceinline__ void atmosphere_prim(const Options &o, Prim &p)
 {
     p.rho = o.rho_floor;
     p.pressure = o.pressure_floor;
-    p.eps = p.pressure / ((o.gamma - 1.0) * p.rho);
+    p.eps = qdiv(p.pressure, (o.gamma - 1.0) * p.rho);
     p.v[0] = p.v[1] = p.v[2] = 0.0;
 }
 

@@ -1374,7 +1374,7 @@ cells_block(const BoxDev *__restrict__ boxes, const int2 *__restrict__ items,
             cell_geometry(bg_plus, x, gp);
             cell_geometry(bg_minus, x, gm);
             const double lp = log_sqrt_det(gp), lm = log_sqrt_det(gm);
-            const double dlog = (lp - lm) / (2.0 * a.vol_h);
+            const double dlog = qdiv(lp - lm, 2.0 * a.vol_h);
             rhs.D += 0.0 - us.D * dlog;
             rhs.tau += 0.0 - us.tau * dlog;
 #pragma unroll

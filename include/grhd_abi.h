@@ -354,6 +354,18 @@ int grhd_cell_geometry_box(grhd_ctx *ctx, const grhd_box *box,
                            const grhd_background *background, double time,
                            double *out, void *stream);
 
+/* The FP64 division and square root every kernel uses, applied
+ * elementwise for checking them against IEEE arithmetic (no counterpart
+ * in the reference, whose CPU build uses the IEEE operations directly):
+ * quotient[i] = a[i] / b[i] and root[i] = sqrt(a[i]) for n doubles in
+ * device memory.  They equal the IEEE results bit for bit where the
+ * hardware sequences need no range scaling: a = 0 or |a| >= 2^-969, b
+ * normal below 2^1022, exponent(a) - exponent(b) < 768, a normal
+ * quotient; square roots of arguments of 2^-767 and above, +-0, +inf,
+ * negative and NaN arguments.  Synchronises the stream. */
+int grhd_fp64_div_sqrt(const double *a, const double *b, double *quotient,
+                       double *root, long long n, void *stream);
+
 
 
 /* ------------------------------------------------------------------ */
