@@ -181,6 +181,7 @@ SIGNATURES = {
     "grhd_ctx_set_timing": (_i, [_vp, _i]),
     "grhd_ctx_get_timing": (_i, [_vp, _P(_d), _P(ctypes.c_longlong)]),
     "grhd_ctx_set_autotune": (_i, [_vp, _i]),
+    "grhd_ctx_set_face_launch": (_i, [_vp, _i, _i, _i]),
     "grhd_ctx_get_face_launch": (_i, [_vp, _i, _P(_i), _P(_i), _P(_i),
                                       _P(_d)]),
 }

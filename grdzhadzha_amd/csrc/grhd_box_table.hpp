@@ -22,28 +22,26 @@ constexpr int kKeyBits = 43; /* 2 x (cells of a level) < 2^43 */
 constexpr int kTileI = 8, kTileD = 8;
 constexpr int kSegTile = 0;
 
-/* The y/z face launch of a level of `cells` interior cells: the number of
- * faces per walker of the marching kernels (k_faces), or kSegTile for the
- * tile kernels (k_faces_t).  A walk shares each cell's metric and slopes
- * between the two faces it touches, but it is sequential: walks of 8
- * faces on a 32^3 box leave 80 waves for 1024 SIMDs.  Walks get shorter
- * until the level yields at least 2^16 walkers (one wave per SIMD); where
- * that would leave walks of one face (two metric evaluations per face),
- * below 2^17 cells, the tiles take over (8 / 7 metric evaluations per
- * face, every cell its own lane).  Measured per RK4 step (ms, walk length
- * 1 / 2 / 4 / 8): 32^3 0.19 / 0.21 / 0.26 / 0.36, 64^3 0.48 / 0.45 / 0.43
- * / 0.52, 96^3 1.19 / 1.13 / 1.12 / 1.09; on the 32^3 Michel box the
- * tiles took 0.136 ms against 0.144 for walks of one face.  On larger
- * levels the tiles were no faster in FP64 and 20% slower in FP32.  The
- * fluxes do not depend on the choice. */
+/* Interior cells of a level below which the y/z faces are formed in tiles
+ * (and, in grhd_kernels.hip, the three face directions run as one fused
+ * launch). */
+constexpr long long kTileCells = 3ll << 20;
+
+/* The y/z face launch of a level of `cells` interior cells: kSegTile for
+ * the tile kernels (k_faces_t: every cell its own lane, 8 / 7 metric
+ * evaluations per face) below kTileCells, else walks of 8 faces per
+ * thread of the marching kernels (k_faces: each cell's metric and slopes
+ * shared by the two faces it touches, 9 / 8 evaluations per face, but
+ * sequential along the walk).  Measured in RK4 steps of the Kerr torus
+ * (HIP events, face launch per stage, FP64): 64^3 0.038 ms tiles against
+ * 0.045 for walks of 4, 96^3 0.099 against 0.115 for walks of 8, 128^3
+ * 0.223 against 0.234; FP32 tiles equal to walks of 8 at 128^3 and 4%
+ * faster over the step at 96^3.  On the 192^3 and 256^3 boxes and the
+ * 6M-cell box-set levels the walks of 8 were faster in the steps (the
+ * tiles lost 3-4% there).  The fluxes do not depend on the choice. */
 __host__ inline int march_segment(long long cells)
 {
-    if (cells < (1ll << 17))
-        return kSegTile;
-    int seg = 8;
-    while (seg > 1 && cells / seg < (1ll << 16))
-        seg /= 2;
-    return seg;
+    return cells < kTileCells ? kSegTile : 8;
 }
 
 /* Device-side status block of one call.  The failure key, which every

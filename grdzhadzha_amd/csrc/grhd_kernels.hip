@@ -424,6 +424,10 @@ struct grhd_ctx
      * plan is rebuilt whenever a call names another box list) */
     bool autotune = false;
     hipEvent_t tune_ev[2] = {nullptr, nullptr};
+    /* a face launch pinned per precision (grhd_ctx_set_face_launch): the
+     * kSegs index, -1 for none, and fused or not */
+    int pin_seg[2] = {-1, -1};
+    bool pin_fused[2] = {false, false};
     std::vector<TuneRecord> tuned_levels;
     /* instrumentation */
     bool timing = false;
@@ -664,11 +668,13 @@ void launch_prims(grhd_ctx *c, const Options &o, const Background &bg,
 
 /* Interior cells of a call from which the three face kernels run as
  * separate launches on concurrent streams; below it one fused launch
- * (k_faces_all) does all three directions.  Measured per RK4 step, fused
- * against streams: 32^3 0.148 / 0.262 ms, 64^3 0.394 / 0.473, 128^3
- * 2.260 / 2.293, 160^3 4.232 / 4.145, 256^3 16.86 / 16.54, the box-set
- * levels (6M cells each) 22.4 / 21.2 ms for all three levels. */
-constexpr long long kFusedFacesCells = 3ll << 20;
+ * (k_faces_all, or k_faces_all_t with the tiles, which the size rule
+ * takes below the same count) does all three directions.  Measured per
+ * RK4 step with walks, fused against streams: 32^3 0.148 / 0.262 ms,
+ * 64^3 0.394 / 0.473, 128^3 2.260 / 2.293, 160^3 4.232 / 4.145, 256^3
+ * 16.86 / 16.54, the box-set levels (6M cells each) 22.4 / 21.2 ms for
+ * all three levels. */
+constexpr long long kFusedFacesCells = kTileCells;
 
 /* The three face directions are independent (W in, each its own flux
  * pool out), so they can run as one fused launch or as three kernels.
@@ -801,11 +807,12 @@ void tune_faces(grhd_ctx *c, const Options &o, const Background &bg,
     remember_tuning(c);
 }
 
-/* The faces of one stage.  By default the size rules choose: the walk
- * length from march_segment, and one fused launch below kFusedFacesCells
- * interior cells, where the fused kernel's register count costs the x
- * faces less than the two launch boundaries it removes.  A tuning context
- * uses what tune_faces measured instead.  The faces timing class covers
+/* The faces of one stage.  By default the size rules choose: tiles or
+ * the walk length from march_segment, and one fused launch below
+ * kFusedFacesCells interior cells, where the fused kernel's register
+ * count costs the x faces less than the two launch boundaries it
+ * removes.  A launch pinned on the context comes first, then what a
+ * tuning context measured (tune_faces).  The faces timing class covers
  * all three directions. */
 template <class T, int SEL = FACES_ALL>
 void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
@@ -814,13 +821,19 @@ void launch_faces_t(grhd_ctx *c, const Options &o, const Background &bg,
     Plan &pl = c->plan;
     const int prec = sizeof(T) < sizeof(double) ? 1 : 0;
     /* the two passes of the guarded path keep the size rules */
-    const bool tunes = c->autotune && SEL == FACES_ALL;
+    const bool pinned = c->pin_seg[prec] >= 0 && SEL == FACES_ALL;
+    const bool tunes = c->autotune && SEL == FACES_ALL && !pinned;
     if (tunes && !pl.tune_tried[prec])
         tune_faces<T>(c, o, bg, p, s);
     bool fused = pl.tot_i < kFusedFacesCells;
     int k = pl.seg_rule;
-    if (tunes && pl.tuned[prec] &&
-        build_seg_tables(pl, pl.seg_pick[prec]) == GRHD_OK)
+    if (pinned && build_seg_tables(pl, c->pin_seg[prec]) == GRHD_OK)
+    {
+        k = c->pin_seg[prec];
+        fused = c->pin_fused[prec];
+    }
+    else if (tunes && pl.tuned[prec] &&
+             build_seg_tables(pl, pl.seg_pick[prec]) == GRHD_OK)
     {
         k = pl.seg_pick[prec];
         fused = pl.fused_pick[prec];
@@ -2347,17 +2360,39 @@ int grhd_ctx_set_autotune(grhd_ctx *ctx, int enable)
     return GRHD_OK;
 }
 
+int grhd_ctx_set_face_launch(grhd_ctx *ctx, int mixed, int walk, int fused)
+{
+    if (!ctx || (mixed != 0 && mixed != 1) || (fused != 0 && fused != 1))
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (walk == -1)
+    {
+        ctx->pin_seg[mixed] = -1;
+        return GRHD_OK;
+    }
+    for (int k = 0; k < kNumSegs; ++k)
+        if (kSegs[k] == walk)
+        {
+            ctx->pin_seg[mixed] = k;
+            ctx->pin_fused[mixed] = fused != 0;
+            return GRHD_OK;
+        }
+    return GRHD_ERR_INVALID_ARGUMENT;
+}
+
 int grhd_ctx_get_face_launch(grhd_ctx *ctx, int mixed, int *walk, int *fused,
                              int *tuned, double *ms)
 {
     if (!ctx || ctx->plan.ncomp < 0 || (mixed != 0 && mixed != 1))
         return GRHD_ERR_INVALID_ARGUMENT;
     const Plan &pl = ctx->plan;
-    const bool t = ctx->autotune && pl.tuned[mixed];
+    const int pin = ctx->pin_seg[mixed];
+    const bool t = pin < 0 && ctx->autotune && pl.tuned[mixed];
     if (walk)
-        *walk = kSegs[t ? pl.seg_pick[mixed] : pl.seg_rule];
+        *walk = kSegs[pin >= 0 ? pin : t ? pl.seg_pick[mixed] : pl.seg_rule];
     if (fused)
-        *fused = t ? pl.fused_pick[mixed] : pl.tot_i < kFusedFacesCells;
+        *fused = pin >= 0 ? ctx->pin_fused[mixed]
+                 : t      ? pl.fused_pick[mixed]
+                          : pl.tot_i < kFusedFacesCells;
     if (tuned)
         *tuned = t;
     if (ms)

@@ -87,6 +87,18 @@ class Context:
         if rc != abi.GRHD_OK:
             raise RuntimeError("grhd_ctx_set_autotune failed (%d)" % rc)
 
+    def set_face_launch(self, walk, fused=False, mixed=False):
+        """Pin the face launch of later calls for FP64 (mixed=False) or
+        FP32 faces: walk 1, 2, 4 or 8 faces per thread, or 0 for the 8 x 8
+        tiles, fused into one launch or on streams; walk=None unpins.  A
+        pin comes before tuning; results do not depend on it."""
+        w = -1 if walk is None else int(walk)
+        rc = self.lib.grhd_ctx_set_face_launch(self.handle, int(bool(mixed)),
+                                               w, int(bool(fused)))
+        if rc != abi.GRHD_OK:
+            raise ValueError("grhd_ctx_set_face_launch: walk %r fused %r "
+                             "(%d)" % (walk, fused, rc))
+
     def face_launch(self, mixed=False):
         """{'walk', 'fused', 'tuned', 'ms'} of the current level's face
         launch for FP64 (mixed=False) or FP32 faces; walk 0 means the y/z

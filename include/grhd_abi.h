@@ -572,23 +572,34 @@ int grhd_ctx_get_timing(grhd_ctx *ctx, double *total_ms,
 /* run of 1, 2, 4 or 8 faces per thread, or form faces in tiles of    */
 /* 8 x 8 cells (one cell per lane), and the three face directions run */
 /* as one fused launch or as three kernels on concurrent streams.  By */
-/* default size rules choose both.  With tuning on, the first call on */
-/* a new level (a new box list) or in a new face precision times all  */
-/* ten choices on that call's own                                     */
-/* primitives and keeps the fastest for the plan; the fluxes, and so  */
-/* every result, are the same under every choice.  A tuning call      */
-/* waits on the host once per choice.                                 */
+/* default size rules choose both: tiles and one fused launch below   */
+/* 3 x 2^20 interior cells, walks of 8 on streams above.  With tuning */
+/* on, the first call on a new level (a new box list) or in a new     */
+/* face precision times all ten choices on that call's own primitives */
+/* and keeps the fastest for the plan.  A launch can also be pinned.  */
+/* The fluxes, and so every result, are the same under every choice.  */
+/* A tuning call waits on the host once per choice.  The guarded      */
+/* mixed path's two face passes keep the size rules.                 */
 /* ------------------------------------------------------------------ */
 
 /* Enable (1) or disable (0) tuning; disabling returns to the rules. */
 int grhd_ctx_set_autotune(grhd_ctx *ctx, int enable);
 
+/* Pin the face launch of this context's later calls in one face
+ * precision (mixed = 0: FP64 faces, 1: FP32 faces): walk 1, 2, 4 or 8
+ * faces per thread, or 0 for the tiles, and fused (1) or three launches
+ * on streams (0).  A pinned launch comes before tuning; walk = -1 unpins
+ * (back to tuning or the rules).  For a launch measured once offline, or
+ * for checking that every launch gives the same results. */
+int grhd_ctx_set_face_launch(grhd_ctx *ctx, int mixed, int walk,
+                             int fused);
+
 /* The face launch the current level uses for FP64 (mixed = 0) or FP32
  * (mixed = 1) faces: walk length (0 for the tiles), fused (1) or not (0),
- * whether it was
- * measured (1) or is the rule's (0), and the measured milliseconds per
- * face stage (0 for the rule).  Any output pointer may be NULL.  Returns
- * GRHD_ERR_INVALID_ARGUMENT before the first level call. */
+ * whether it was measured (1) or is the rule's or a pinned one (0), and
+ * the measured milliseconds per face stage (0 otherwise).  Any output
+ * pointer may be NULL.  Returns GRHD_ERR_INVALID_ARGUMENT before the
+ * first level call. */
 int grhd_ctx_get_face_launch(grhd_ctx *ctx, int mixed, int *walk,
                              int *fused, int *tuned, double *ms);
 

@@ -476,8 +476,9 @@ def test_level_64_box_tiling_equals_single_box(hip_context):
 
 def test_face_launch_forms_agree(hip_context):
     """Calls below 3 * 2^20 interior cells launch the three face directions
-    as one fused kernel with walks of one face (on a 32^3 box); larger calls
-    launch them as three kernels on concurrent streams with walks of 8.
+    as one fused kernel with the y/z faces in tiles (on a 32^3 box); larger
+    calls launch them as three kernels on concurrent streams with walks of
+    8.
     The 32^3 Michel box advanced alone equals the same box advanced in a
     level whose 160^3 second box (its own domain, at shifted indices) puts
     the call above the threshold, bit for bit."""
@@ -565,6 +566,46 @@ def test_tuned_face_launch_equals_rule_launch(hip_context, mixed):
     assert np.array_equal(p0, p1)
     assert np.array_equal(m0, m1)
     assert (a0, b0) == (a1, b1)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_every_pinned_face_launch_equals_rule_launch(hip_context, mixed):
+    """Each of the ten face launches pinned on a context (tiles or walks of
+    1, 2, 4 or 8 faces; fused or on streams) advances a 10-box binary-PN
+    level exactly like the size rules do (state, mask, counts), in FP64
+    and with FP32 faces; unpinning returns to the rule."""
+    from grdzhadzha_amd.operator import Context
+    bg, opts, levels, _ = workloads.binary_pn_levels(nbox=10, nlevels=1)
+    boxes, fabs, dt = levels[0]
+    opts = dataclasses.replace(opts, mixed_precision=mixed)
+
+    def run(ctx):
+        lev = LevelOperator(bg, boxes, opts, time_dependent=True,
+                            context=ctx)
+        pool = lev.pack(fabs)
+        mask = torch.zeros(lev.num_interior, dtype=torch.uint8,
+                           device="cuda")
+        d = lev.advance_rk4(pool, dt, nsteps=1, time=0.0, mask=mask)
+        return pool.cpu().numpy(), mask.cpu().numpy(), d
+
+    p0, m0, d0 = run(hip_context)
+    rule = hip_context.face_launch(mixed=mixed)
+    ctx = Context()
+    try:
+        for walk in (0, 1, 2, 4, 8):
+            for fused in (True, False):
+                ctx.set_face_launch(walk, fused, mixed=mixed)
+                p1, m1, d1 = run(ctx)
+                assert ctx.face_launch(mixed=mixed) == {
+                    "walk": walk, "fused": fused, "tuned": False, "ms": 0.0}
+                assert np.array_equal(p1, p0), (walk, fused)
+                assert np.array_equal(m1, m0), (walk, fused)
+                assert d1 == d0
+        ctx.set_face_launch(None, mixed=mixed)
+        run(ctx)
+        assert ctx.face_launch(mixed=mixed) == rule
+    finally:
+        ctx.close()
 
 
 def test_tuning_is_remembered_per_level(hip_context):

@@ -111,13 +111,45 @@ def test_rhs_torus_kerr(hip_context):
     _rhs_case(small_torus(), hip_context)
 
 
-@pytest.mark.parametrize("n", [64, 96])
-def test_rhs_torus_longer_face_walks(hip_context, n):
-    """Boxes below 48^3 give the marching y/z face kernels walks of one
-    face; 64^3 gives walks of 4 faces, 96^3 walks of 8, where each walker
-    carries a cell's metric and slopes to its next face.  The rhs still
-    equals the oracle's."""
-    _rhs_case(workloads.torus_box(n=n, nsteps=1), hip_context)
+def test_rhs_torus_96(hip_context):
+    _rhs_case(workloads.torus_box(n=96, nsteps=1), hip_context)
+
+
+FACE_LAUNCHES = [(walk, fused) for walk in (0, 1, 2, 4, 8)
+                 for fused in (True, False)]
+
+
+def test_rhs_torus_every_face_launch(hip_context):
+    """The ten face launches pinned in turn on a 64^3 torus box: the y/z
+    faces formed in 8 x 8 tiles (walk 0) or by walkers of 1, 2, 4 or 8
+    faces, each carrying a cell's metric and slopes to its next face; the
+    three directions in one fused launch or three on streams.  Every one
+    gives the oracle's rhs, mask and ghost-filled state bit for bit."""
+    w = workloads.torus_box(n=64, nsteps=1)
+    O = oracle.Oracle(w.background, w.box, w.options)
+    st_ref = np.ascontiguousarray(w.state).copy()
+    rhs_ref, mask_ref = O.compute_rhs(st_ref, time=0.0, with_mask=True)
+    op = FixedBGLevelDataFiniteVolumeOperator(w.background, w.box, w.options,
+                                              context=hip_context)
+    try:
+        for walk, fused in FACE_LAUNCHES:
+            hip_context.set_face_launch(walk, fused)
+            st = to_dev(w.state)
+            rhs = torch.zeros((st.shape[0],) + w.box.interior_shape,
+                              dtype=torch.float64, device="cuda")
+            mask = torch.zeros(w.box.interior_shape, dtype=torch.uint8,
+                               device="cuda")
+            op.compute_rhs(rhs, st, source_mask=mask)
+            torch.cuda.synchronize()
+            assert hip_context.face_launch() == {
+                "walk": walk, "fused": fused, "tuned": False, "ms": 0.0}
+            what = "walk %d, fused %s" % (walk, fused)
+            assert_bitwise(rhs.cpu().numpy(), rhs_ref, "rhs, " + what)
+            assert np.array_equal(mask.cpu().numpy(), mask_ref), what
+            assert_bitwise(st.cpu().numpy(), st_ref,
+                           "state after the ghost fill, " + what)
+    finally:
+        hip_context.set_face_launch(None)
 
 
 def test_rhs_no_reconstruction_no_sources(hip_context):
@@ -249,7 +281,7 @@ def test_rk4_torus_two_steps(hip_context):
 def test_rk4_torus_128_full_size_one_step(hip_context):
     """The benchmark workload itself at full size: one RK4 step of the
     128^3 Kerr a = 0.9 torus (2.1M interior cells, minmod, dt = 0.25 dx),
-    the size at which the face kernels walk 8 faces per thread and run
+    the size at which the face kernels form the y/z faces in tiles and run
     fused.  State, masks and recovery counts equal the oracle's bit for
     bit.  The oracle needs a few seconds on
     the host's cores."""
