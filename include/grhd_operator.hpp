@@ -301,6 +301,15 @@ class Context
     /* time the face launches on each new level and keep the fastest
      * (results are the same either way) */
     void set_autotune(bool on) { (void)grhd_ctx_set_autotune(m_ctx, on); }
+    /* pin the face launch of one precision: walk 1, 2, 4 or 8, 0 for the
+     * tiles, -1 to unpin (results are the same either way) */
+    void set_face_launch(bool mixed, int walk, bool fused)
+    {
+        if (grhd_ctx_set_face_launch(m_ctx, mixed ? 1 : 0, walk,
+                                     fused ? 1 : 0) != GRHD_OK)
+            throw std::invalid_argument("GRHD set_face_launch: walk must "
+                                        "be -1, 0, 1, 2, 4 or 8");
+    }
 
   private:
     grhd_ctx *m_ctx = nullptr;

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <vector>
@@ -37,9 +38,11 @@ template <class T> static T read_struct(const std::string &path)
 
 int main(int argc, char **argv)
 {
-    if (argc != 5)
+    if (argc != 5 && argc != 7)
     {
-        std::fprintf(stderr, "usage: %s dir cfl max_dt t_end\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s dir cfl max_dt t_end [walk fused]\n",
+                     argv[0]);
         return 2;
     }
     const std::string dir = argv[1];
@@ -97,6 +100,9 @@ int main(int argc, char **argv)
 
         const ProblemDomain domain(dom);
         auto ctx = std::make_shared<Context>();
+        if (argc == 7) /* a pinned FP64 face launch */
+            ctx->set_face_launch(false, std::atoi(argv[5]),
+                                 std::atoi(argv[6]) != 0);
         FixedBGLevelDataFiniteVolumeOperator op(
             IdealGasEOS(p.adiabatic_index), bg, dx, c, atm, p.limiter_theta,
             p.use_reconstruction != 0, rec, p.use_static_metric_sources != 0,

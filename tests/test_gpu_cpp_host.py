@@ -60,6 +60,13 @@ def test_cpp_operator_michel_leveldata(hip_context, tmp_path):
     assert np.array_equal(got, pool.cpu().numpy())
     assert float(t) == tt
 
+    # the same run with the face launch pinned through the C++ context
+    # (walks of 2 faces, three launches on streams): the same state
+    res = subprocess.run([DRIVER, d, "0.2", "0.02", "0.1", "2", "0"],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert np.array_equal(np.fromfile(os.path.join(d, "state_out.bin")), got)
+
 
 HOOKS = os.path.join(ROOT, "tests", "cpp", "_build", "hooks_driver")
 
