@@ -1062,12 +1062,34 @@ int finish(grhd_ctx *c, hipStream_t s, grhd_error *err,
 }
 
 /* ---------------- level operations ---------------- */
+/* A level call with no boxes: a process of a distributed level may own
+ * none of its boxes, and the reference's loops over the level's boxes
+ * then do nothing.  Such a call checks its parameters, launches nothing
+ * and succeeds; its outputs are an empty level's (zero counts, max 1/dt
+ * and sums, a finite state).  `diag` (nullable) is zeroed. */
+int empty_level(const grhd_ctx *c, const grhd_params *params,
+                const grhd_background *bg, int ncomp, int cb,
+                grhd_diagnostics *diag = nullptr)
+{
+    if (!c)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    const int rc = params ? check_common(params, bg, ncomp, cb)
+                   : (cb < 0 || cb + GRHD_NUM_COMPS > ncomp)
+                       ? GRHD_ERR_INVALID_ARGUMENT
+                       : GRHD_OK;
+    if (!rc && diag)
+        std::memset(diag, 0, sizeof(*diag));
+    return rc;
+}
+
 int level_rhs(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
               const grhd_params *params, const grhd_background *bg,
               double time, double *state, int ncomp, int cb, double *rhs,
               unsigned char *mask, bool exchange, grhd_error *err, void *st,
               bool accumulate = false)
 {
+    if (nbox == 0)
+        return empty_level(c, params, bg, ncomp, cb);
     if (!state || !rhs)
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
@@ -1102,6 +1124,8 @@ int level_recover(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
                   double time, double *state, int ncomp, int cb,
                   unsigned char *mask, grhd_diagnostics *diag, void *st)
 {
+    if (nbox == 0)
+        return empty_level(c, params, bg, ncomp, cb, diag);
     if (!state)
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
@@ -1122,6 +1146,14 @@ int level_max_inv_dt(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
                      double time, const double *state, int ncomp, int cb,
                      double *out, grhd_error *err, void *st)
 {
+    if (nbox == 0)
+    {
+        const int rc = out ? empty_level(c, params, bg, ncomp, cb)
+                           : GRHD_ERR_INVALID_ARGUMENT;
+        if (!rc)
+            *out = 0.0;
+        return rc;
+    }
     if (!state || !out)
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
@@ -1197,8 +1229,29 @@ int level_checkpoint(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
                      double *state, int ncomp, const int *dlo, const int *dhi,
                      int ghosts, double *global, bool pack, void *st)
 {
-    if (!state || !global || ncomp < 1)
+    if (!global || ncomp < 1 || (!state && nbox != 0))
         return GRHD_ERR_INVALID_ARGUMENT;
+    if (nbox == 0)
+    {
+        /* no boxes: packing gives the zero-filled checkpoint grid,
+         * unpacking writes nothing */
+        if (!c)
+            return GRHD_ERR_INVALID_ARGUMENT;
+        Plan none;
+        CkptGrid g;
+        long long cells = 0;
+        const int rc = ckpt_grid(none, dlo, dhi, ghosts, g, cells);
+        if (rc || !pack)
+            return rc;
+        hipStream_t s = (hipStream_t)st;
+        const long long n = cells * ncomp;
+        hipLaunchKernelGGL(k_fill, dim3(grid_of(n)), dim3(kBlock), 0, s,
+                           global, n, 0.0);
+        if (hipGetLastError() != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return GRHD_ERR_DEVICE;
+        return GRHD_OK;
+    }
     int rc = prepare(c, boxes, nbox, ncomp);
     if (rc)
         return rc;
@@ -1233,6 +1286,9 @@ int level_update(grhd_ctx *ctx, const grhd_level_box *boxes, int nbox,
                  unsigned char *mask, grhd_diagnostics *diagnostics,
                  void *stream)
 {
+    if (nbox == 0)
+        return empty_level(ctx, params, background, ncomp, comp_base,
+                           diagnostics);
     if (!state || !rhs)
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, background, ncomp, comp_base);
@@ -1271,6 +1327,8 @@ int level_ssprk2(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
                  unsigned char *mask, bool exchange, grhd_diagnostics *diag,
                  grhd_error *err, void *st)
 {
+    if (nbox == 0)
+        return empty_level(c, params, bg, ncomp, cb, diag);
     if (!state)
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
@@ -1318,7 +1376,11 @@ int level_rk4(grhd_ctx *c, const grhd_level_box *boxes, int nbox,
               int cb, unsigned char *mask, bool exchange,
               grhd_diagnostics *diag, grhd_error *err, void *st)
 {
-    if (!state || nsteps < 0)
+    if (nsteps < 0)
+        return GRHD_ERR_INVALID_ARGUMENT;
+    if (nbox == 0)
+        return empty_level(c, params, bg, ncomp, cb, diag);
+    if (!state)
         return GRHD_ERR_INVALID_ARGUMENT;
     int rc = check_common(params, bg, ncomp, cb);
     if (rc || (rc = prepare(c, boxes, nbox, ncomp)) ||
@@ -2236,6 +2298,14 @@ int grhd_level_conserved_sum(grhd_ctx *ctx, const grhd_level_box *boxes,
 {
     if (!sum)
         return GRHD_ERR_INVALID_ARGUMENT;
+    if (nbox == 0)
+    {
+        const int rc = empty_level(ctx, nullptr, nullptr, ncomp, comp_base);
+        if (!rc)
+            for (int m = 0; m < 5; ++m)
+                sum[m] = 0.0;
+        return rc;
+    }
     const int rc = level_diagnostics(ctx, boxes, nbox, state_pool, ncomp,
                                      comp_base, stream);
     if (rc)
@@ -2253,6 +2323,13 @@ int grhd_level_conserved_state_is_finite(grhd_ctx *ctx,
 {
     if (!is_finite)
         return GRHD_ERR_INVALID_ARGUMENT;
+    if (nbox == 0)
+    {
+        const int rc = empty_level(ctx, nullptr, nullptr, ncomp, comp_base);
+        if (!rc)
+            *is_finite = 1;
+        return rc;
+    }
     const int rc = level_diagnostics(ctx, boxes, nbox, state_pool, ncomp,
                                      comp_base, stream);
     if (rc)

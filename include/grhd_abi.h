@@ -374,7 +374,16 @@ int grhd_fp64_div_sqrt(const double *a, const double *b, double *quotient,
 
 /* One box of a level and where its FArrayBox starts in the level's state
  * pool (in doubles).  The pool holds each box's FAB (ncomp components
- * over its storage box) at its offset. */
+ * over its storage box) at its offset.
+ *
+ * Every grhd_level_* call takes the boxes this process owns, `nbox` of
+ * them, and nbox may be 0 (a rank of a distributed level can own none;
+ * the reference's loops over its DataIterator then do nothing).  Such a
+ * call checks its parameters, launches nothing and returns GRHD_OK with
+ * an empty level's outputs: zero diagnostics, a maximum 1/dt of 0 (the
+ * reference's local maximum before MPI_Allreduce), zero sums, a finite
+ * state, and a packed checkpoint of zeros; `boxes` and the pools may be
+ * NULL. */
 typedef struct grhd_level_box
 {
     grhd_box box;

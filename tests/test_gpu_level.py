@@ -656,3 +656,85 @@ def test_failed_plan_is_not_reused(hip_context):
             ctypes.c_void_p(dummy.data_ptr()), 11, 0, None,
             ctypes.byref(diag), None)
         assert rc == abi.GRHD_ERR_INVALID_ARGUMENT
+
+
+def test_level_calls_with_no_boxes(hip_context):
+    """A process of a distributed level may own none of its boxes (the
+    reference's loops over the DataIterator then do nothing).  Every level
+    entry point with nbox = 0 and NULL boxes and pools succeeds with an
+    empty level's outputs (zero counts, max 1/dt 0, zero sums, a finite
+    state, a zero checkpoint grid, an untouched one on unpack), and still
+    rejects bad parameters."""
+    import ctypes
+    from grdzhadzha_amd import abi
+    from grdzhadzha_amd.options import KerrSchildParams, OperatorOptions
+    lib, h = hip_context.lib, hip_context.handle
+    params = OperatorOptions().to_c()
+    bg = KerrSchildParams(1.0, 0.5).to_c()
+    P = ctypes.byref
+    OK = abi.GRHD_OK
+
+    def fresh_diag():
+        d = abi.GrhdDiagnostics()
+        d.num_cells = d.num_failed_recoveries = 7
+        return d
+
+    err = abi.GrhdError()
+    assert lib.grhd_level_compute_rhs(h, None, 0, P(params), P(bg), 0.0,
+                                      None, 11, 0, None, None, P(err),
+                                      None) == OK
+    assert lib.grhd_level_add_conserved_rhs(h, None, 0, P(params), P(bg),
+                                            0.0, None, 11, 0, None, P(err),
+                                            None) == OK
+    for call in ("recover", "update", "ssprk2", "rk4"):
+        d = fresh_diag()
+        if call == "recover":
+            rc = lib.grhd_level_recover_primitives(
+                h, None, 0, P(params), P(bg), 0.0, None, 11, 0, None, P(d),
+                None)
+        elif call == "update":
+            rc = lib.grhd_level_update_conserved(
+                h, None, 0, P(params), P(bg), None, 11, 0, None, 0.1, 1, 0.1,
+                None, P(d), None)
+        elif call == "ssprk2":
+            rc = lib.grhd_level_advance_ssprk2(
+                h, None, 0, P(params), P(bg), 0.0, 0.1, None, 11, 0, None,
+                P(d), P(err), None)
+        else:
+            rc = lib.grhd_level_advance_rk4(
+                h, None, 0, P(params), P(bg), 0.0, 0.1, 3, None, 11, 0, None,
+                P(d), P(err), None)
+        assert rc == OK, call
+        assert (d.num_cells, d.num_failed_recoveries) == (0, 0), call
+    m = ctypes.c_double(5.0)
+    assert lib.grhd_level_compute_max_inverse_dt(
+        h, None, 0, P(params), P(bg), 0.0, None, 11, 0, P(m), P(err),
+        None) == OK
+    assert m.value == 0.0
+    sums = (ctypes.c_double * 5)(*([1.0] * 5))
+    assert lib.grhd_level_conserved_sum(h, None, 0, None, 11, 0, sums,
+                                        None) == OK
+    assert list(sums) == [0.0] * 5
+    fin = ctypes.c_int(0)
+    assert lib.grhd_level_conserved_state_is_finite(h, None, 0, None, 11, 0,
+                                                    P(fin), None) == OK
+    assert fin.value == 1
+    lo, hi = (ctypes.c_int * 3)(0, 0, 0), (ctypes.c_int * 3)(3, 2, 1)
+    ck = torch.full((2 * 10 * 9 * 8,), 3.0, dtype=torch.float64,
+                    device="cuda")
+    assert lib.grhd_level_checkpoint_unpack(h, None, 0, ck.data_ptr(), 2, lo,
+                                            hi, 3, None, None) == OK
+    assert bool((ck == 3.0).all())
+    assert lib.grhd_level_checkpoint_pack(h, None, 0, None, 2, lo, hi, 3,
+                                          ck.data_ptr(), None) == OK
+    assert bool((ck == 0.0).all())
+    # parameters are still checked
+    assert lib.grhd_level_compute_rhs(h, None, 0, P(params), P(bg), 0.0,
+                                      None, 10, 0, None, None, P(err),
+                                      None) == abi.GRHD_ERR_INVALID_ARGUMENT
+    assert lib.grhd_level_advance_rk4(
+        h, None, 0, P(params), P(bg), 0.0, 0.1, -1, None, 11, 0, None,
+        None, P(err), None) == abi.GRHD_ERR_INVALID_ARGUMENT
+    assert lib.grhd_level_advance_rk4(
+        h, None, -1, P(params), P(bg), 0.0, 0.1, 1, None, 11, 0, None,
+        None, P(err), None) == abi.GRHD_ERR_INVALID_ARGUMENT
