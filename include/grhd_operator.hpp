@@ -204,7 +204,9 @@ inline IntBox bounding_box(const std::vector<IntBox> &grids)
 /* LevelData<FArrayBox> in HBM.  `pool` is a device pointer to
  * pool_size() doubles (the caller owns it).  The layout's ProblemDomain
  * (DisjointBoxLayout::physDomain) is given, or the grids' bounding box,
- * non-periodic. */
+ * non-periodic.  With the domain given, the grids may be empty: a rank
+ * that owns none of the level's boxes, on which every operation does
+ * nothing. */
 class DeviceLevelData
 {
   public:
@@ -213,6 +215,9 @@ class DeviceLevelData
         : DeviceLevelData(grids, ProblemDomain(bounding_box(grids)), ghosts,
                           ncomp, pool)
     {
+        if (m_grids.empty())
+            throw std::invalid_argument(
+                "DeviceLevelData: no grids and no ProblemDomain");
     }
 
     DeviceLevelData(std::vector<IntBox> grids, const ProblemDomain &domain,
@@ -220,7 +225,7 @@ class DeviceLevelData
         : m_grids(std::move(grids)), m_domain(domain), m_ghosts(ghosts),
           m_ncomp(ncomp), m_pool(pool)
     {
-        if (ghosts < 0 || ncomp <= 0 || m_grids.empty())
+        if (ghosts < 0 || ncomp <= 0)
             throw std::invalid_argument("DeviceLevelData: bad layout");
         long long off = 0;
         for (const IntBox &b : m_grids)

@@ -132,10 +132,21 @@ int main(int argc, char **argv)
         (void)hipFree(pool);
         std::ofstream out(dir + "/state_out.bin", std::ios::binary);
         out.write(reinterpret_cast<const char *>(host.data()), bytes);
+        /* a rank that owns none of the level's boxes: the same calls on
+         * an empty layout of the same domain do nothing, and its stable
+         * dt is the reference's cfl / 1e-14 */
+        DeviceLevelData none(std::vector<IntBox>{}, domain, ghosts, ncomp,
+                             nullptr);
+        const double dt_none = op.compute_stable_dt(none, domain, cfl);
+        op.advance_ssprk2(none, domain, max_dt);
+        const ConservedSum s_none =
+            compute_leveldata_conserved_sum(none, domain, dx, c, 0,
+                                            ctx->get());
         std::ofstream res(dir + "/result.txt");
         res.precision(17);
         res << steps << ' ' << t << ' ' << s0.D << ' ' << s1.D << ' '
-            << (finite ? 1 : 0) << '\n';
+            << (finite ? 1 : 0) << ' ' << dt_none << ' ' << s_none.D
+            << '\n';
         std::printf("level_driver: %d steps to t = %.17g\n", steps, t);
     }
     catch (const std::exception &e)

@@ -43,9 +43,11 @@ def test_cpp_operator_michel_leveldata(hip_context, tmp_path):
     res = subprocess.run([DRIVER, d, "0.2", "0.02", "0.1"],
                          capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
-    steps, t, m0, m1, finite = open(os.path.join(d, "result.txt")).read(
-    ).split()
+    steps, t, m0, m1, finite, dt_none, m_none = open(
+        os.path.join(d, "result.txt")).read().split()
     assert int(steps) == 5 and int(finite) == 1
+    # the same operator on an empty layout (a rank with no boxes)
+    assert float(dt_none) == 0.2 / 1.0e-14 and float(m_none) == 0.0
     assert float("%.6g" % float(m0)) == 0.00172117
     assert float("%.6g" % float(m1)) == 0.00172335
     got = np.fromfile(os.path.join(d, "state_out.bin"))
