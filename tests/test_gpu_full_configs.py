@@ -193,8 +193,8 @@ def test_c4_guarded_mixed_200_steps_within_1e6(hip_context):
 
 @pytest.mark.xfail(strict=False, reason=(
     "measured on one MI355X, 200 steps, every face and source FP32: "
-    "relative L1 D 5.3e-7, S1 1.4e-6, S2 1.6e-6, S3 6.1e-6, tau 3.6e-6, "
-    "accretion rate 4.6e-5; nine tenths of the error sits in the inflow at "
+    "relative L1 D 5.7e-7, S1 1.5e-6, S2 1.7e-6, S3 6.6e-6, tau 3.7e-6, "
+    "accretion rate 5.1e-5; nine tenths of the error sits in the inflow at "
     "r < 12 M, where alpha v - beta is a small difference of large terms"))
 def test_c4_mixed_200_steps_within_1e6(hip_context):
     w = c4_workload()
