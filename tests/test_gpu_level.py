@@ -738,3 +738,23 @@ def test_level_calls_with_no_boxes(hip_context):
     assert lib.grhd_level_advance_rk4(
         h, None, -1, P(params), P(bg), 0.0, 0.1, 1, None, 11, 0, None,
         None, P(err), None) == abi.GRHD_ERR_INVALID_ARGUMENT
+
+
+def test_python_level_with_no_boxes(hip_context):
+    """The Python level mirror over an empty box list (a rank with no
+    boxes): every operation runs through the C-ABI's nbox = 0 path."""
+    from grdzhadzha_amd.options import KerrSchildParams, OperatorOptions
+    lev = LevelOperator(KerrSchildParams(1.0, 0.5), [], OperatorOptions(),
+                        context=hip_context)
+    pool = lev.new_pool()
+    assert pool.numel() == 0 and lev.num_interior == 0
+    rhs = torch.zeros(0, dtype=torch.float64, device="cuda")
+    lev.compute_rhs(rhs, pool)
+    lev.recover_primitives(pool)
+    assert lev.compute_max_inverse_dt(pool) == 0.0
+    assert lev.compute_stable_dt(pool, 0.5) == 0.5 / 1e-14
+    d = lev.advance_rk4(pool, 0.1, nsteps=2)
+    d2 = lev.advance_ssprk2(pool, 0.1)
+    assert d == d2
+    assert list(lev.conserved_sum(pool)) == [0.0] * 5
+    assert lev.conserved_state_is_finite(pool)
