@@ -152,6 +152,80 @@ def test_rhs_torus_every_face_launch(hip_context):
         hip_context.set_face_launch(None)
 
 
+def random_box_case(seed):
+    """A seeded random box away from the hole: sides 2..19 cells, 3
+    ghosts, dx 0.2..0.5, a Kerr-Schild spin in [0, 0.95), a random
+    adiabatic index and limiter, and a perturbed linear primitive field
+    with 5% of its cells at the density and pressure floors."""
+    rng = np.random.default_rng(1000 + seed)
+    n = rng.integers(2, 20, 3)
+    lo = rng.integers(25, 40, 3)
+    box = BoxSpec(tuple(lo), tuple(lo + n - 1), 3,
+                  float(rng.uniform(0.2, 0.5)),
+                  tuple(float(v) for v in rng.uniform(-2.0, 2.0, 3)))
+    bg = KerrSchildParams(1.0, float(rng.uniform(0.0, 0.95)))
+    gamma = float(rng.choice([4.0 / 3.0, 1.4, 5.0 / 3.0]))
+    opts = OperatorOptions(adiabatic_index=gamma,
+                           limiter_theta=float(rng.choice([1.0, 1.5, 2.0])))
+    st = reference_test_state(box, bg, gamma)
+    geo = oracle.Oracle(bg, box).cell_geometry()
+    shape = box.storage_shape
+    xi = rng.uniform(-1.0, 1.0, (5,) + shape)
+    prim = st[5:11].copy()
+    prim[0] *= 1.0 + 0.3 * xi[0]
+    prim[1] *= 1.0 + 0.3 * xi[1]
+    for m in range(3):
+        prim[3 + m] += 0.05 * xi[2 + m]
+    prim[2] = (gamma - 1.0) * prim[0] * prim[1]
+    # a few cells at the floors: density around rho_floor (some below it),
+    # pressure between the floor and twice it
+    low = rng.uniform(0.0, 1.0, shape) < 0.05
+    nlow = int(low.sum())
+    prim[0][low] = 1.0e-12 * rng.uniform(0.5, 2.0, nlow)
+    prim[2][low] = 1.0e-12 * rng.uniform(1.0, 2.0, nlow)
+    prim[1][low] = prim[2][low] / ((gamma - 1.0) * prim[0][low])
+    for idx in np.ndindex(shape):
+        g6 = geo[0:6][(slice(None),) + idx]
+        g = np.array([[g6[0], g6[1], g6[2]], [g6[1], g6[3], g6[4]],
+                      [g6[2], g6[4], g6[5]]])
+        pv = prim[(slice(None),) + idx]
+        st[(slice(0, 5),) + idx] = oracle.prim2con(pv, gamma, g)
+        st[(slice(5, 11),) + idx] = pv
+    return box, bg, opts, st
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_rhs_random_boxes(hip_context, seed):
+    """Seeded random boxes (sizes, positions, dx, spin, Gamma, limiter,
+    perturbed states with cells at and below the floors; six of the eight
+    take the metric-source atmosphere test somewhere), each with one of
+    the ten face launches pinned: the rhs, mask and ghost-filled state
+    equal the oracle's bit for bit."""
+    box, bg, opts, st0 = random_box_case(seed)
+    walk, fused = FACE_LAUNCHES[seed % len(FACE_LAUNCHES)]
+    O = oracle.Oracle(bg, box, opts)
+    st_ref = st0.copy()
+    rhs_ref, mask_ref = O.compute_rhs(st_ref, time=0.0, with_mask=True)
+    op = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
+                                              context=hip_context)
+    try:
+        hip_context.set_face_launch(walk, fused)
+        st = to_dev(st0)
+        rhs = torch.zeros((st.shape[0],) + box.interior_shape,
+                          dtype=torch.float64, device="cuda")
+        mask = torch.zeros(box.interior_shape, dtype=torch.uint8,
+                           device="cuda")
+        op.compute_rhs(rhs, st, source_mask=mask)
+        torch.cuda.synchronize()
+    finally:
+        hip_context.set_face_launch(None)
+    what = "seed %d, box %s..%s, walk %d fused %s" % (seed, box.lo, box.hi,
+                                                     walk, fused)
+    assert_bitwise(rhs.cpu().numpy(), rhs_ref, "rhs, " + what)
+    assert np.array_equal(mask.cpu().numpy(), mask_ref), what
+    assert_bitwise(st.cpu().numpy(), st_ref, "ghost-filled state, " + what)
+
+
 def test_rhs_no_reconstruction_no_sources(hip_context):
     w = small_torus()
     w.options.use_reconstruction = False
@@ -268,6 +342,18 @@ def _step_case(w, ctx, method, nsteps=1, time=0.0):
     assert_bitwise(got[sl], st_ref[sl], method + " interior state")
     assert diag == diag_ref
     assert np.array_equal(mask.cpu().numpy(), mask_ref)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_rk4_random_boxes(hip_context, seed):
+    """Two RK4 steps (dt = 0.2 dx) of the seeded random boxes of
+    test_rhs_random_boxes under the size rule's face launch: state,
+    recovery counts and masks equal the oracle's bit for bit."""
+    import types
+    box, bg, opts, st0 = random_box_case(seed)
+    w = types.SimpleNamespace(background=bg, box=box, options=opts,
+                              state=st0, dt=0.2 * box.dx)
+    _step_case(w, hip_context, "rk4", 2)
 
 
 def test_rk4_michel_one_step(hip_context):
