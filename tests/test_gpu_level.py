@@ -758,3 +758,61 @@ def test_python_level_with_no_boxes(hip_context):
     assert d == d2
     assert list(lev.conserved_sum(pool)) == [0.0] * 5
     assert lev.conserved_state_is_finite(pool)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_split_level_equals_single_box(hip_context, seed):
+    """A seeded random box (random_box_case of the parity tests) made its
+    own ProblemDomain with random periodic directions, split at random
+    planes into 2 to 4 boxes: two RK4 steps of the level -- internal
+    ghosts exchanged on the device, wrapped where periodic -- equal the
+    single box's bit for bit.
+
+    Along a non-periodic direction every plane lies at least `ghosts`
+    cells from the domain faces.  The reference fills a box's physical
+    ghosts from the box's own edge cells
+    (FixedBGFArrayBoxFiniteVolume.hpp:130-146), so a box that ends 1 or 2
+    cells short of a domain face fills the ghosts past that face from
+    itself, not from the cells the single box uses, and the two differ
+    there (as they do in the reference)."""
+    import dataclasses as dc
+    from test_gpu_parity import random_box_case
+    rng = np.random.default_rng(2000 + seed)
+    box0, bg, opts, state = random_box_case(seed)
+    per = tuple(int(v) for v in rng.integers(0, 2, 3))
+    box = dc.replace(box0, periodic=per)
+    parts = [box]
+    g = box.ghosts
+
+    def planes(b, d):
+        """Where box b may be cut along d: cells < at and >= at."""
+        lo, hi = b.lo[d] + 1, b.hi[d]
+        if not per[d]:
+            lo, hi = max(lo, box.lo[d] + g), min(hi, box.hi[d] + 1 - g)
+        return list(range(lo, hi + 1))
+
+    for _ in range(int(rng.integers(1, 4))):
+        cand = [(n, d) for n, b in enumerate(parts) for d in range(3)
+                if planes(b, d)]
+        if not cand:
+            break
+        n, d = cand[int(rng.integers(0, len(cand)))]
+        b = parts.pop(n)
+        opts_at = planes(b, d)
+        at = opts_at[int(rng.integers(0, len(opts_at)))]
+        parts.extend(split_box(b, d, at))
+    dt = 0.2 * box.dx
+    single = LevelOperator(bg, [box], opts, context=hip_context)
+    pool = single.pack([state])
+    single.advance_rk4(pool, dt, nsteps=2)
+    whole = single.unpack(pool)[0]
+    lev = LevelOperator(bg, parts, opts, context=hip_context)
+    p2 = lev.pack([sub_fab(state, box, h) for h in parts])
+    lev.advance_rk4(p2, dt, nsteps=2)
+    assert len(parts) >= 2
+    for h, f in zip(parts, lev.unpack(p2)):
+        sl = (slice(None),) + tuple(
+            slice(h.lo[d] - box.glo[d], h.hi[d] - box.glo[d] + 1)
+            for d in (2, 1, 0))
+        hs = (slice(None),) + h.interior_slices()
+        assert np.array_equal(f[hs], whole[sl]), (seed, per, h.lo, h.hi)

@@ -344,6 +344,24 @@ def _step_case(w, ctx, method, nsteps=1, time=0.0):
     assert np.array_equal(mask.cpu().numpy(), mask_ref)
 
 
+@pytest.mark.parametrize("inside", [1, 2])
+def test_rhs_box_short_of_its_domain_face(hip_context, inside):
+    """A box whose low x face lies 1 or 2 cells inside its ProblemDomain:
+    its ghost layers inside the domain are internal ghosts (the caller's
+    values), and the ones past the domain face are outflow ghosts filled
+    from the box's own edge cells, as the reference does
+    (FixedBGFArrayBoxFiniteVolume.hpp:130-146), not from the domain's
+    edge.  The rhs, mask and ghost-filled state equal the oracle's bit for
+    bit."""
+    import types
+    box0, bg, opts, st0 = random_box_case(3)
+    dlo = (box0.lo[0] - inside, box0.lo[1], box0.lo[2])
+    box = dataclasses.replace(box0, domain_lo=dlo)
+    w = types.SimpleNamespace(background=bg, box=box, options=opts,
+                              state=st0)
+    _rhs_case(w, hip_context)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_rk4_random_boxes(hip_context, seed):
     """Two RK4 steps (dt = 0.2 dx) of the seeded random boxes of
