@@ -778,7 +778,7 @@ def test_random_split_level_equals_single_box(hip_context, seed):
     import dataclasses as dc
     from test_gpu_parity import random_box_case
     rng = np.random.default_rng(2000 + seed)
-    box0, bg, opts, state = random_box_case(seed)
+    box0, bg, opts, state, _ = random_box_case(seed)
     per = tuple(int(v) for v in rng.integers(0, 2, 3))
     box = dc.replace(box0, periodic=per)
     parts = [box]

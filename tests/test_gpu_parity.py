@@ -152,23 +152,40 @@ def test_rhs_torus_every_face_launch(hip_context):
         hip_context.set_face_launch(None)
 
 
-def random_box_case(seed):
+def random_box_case(seed, binary=False):
     """A seeded random box away from the hole: sides 2..19 cells, 3
-    ghosts, dx 0.2..0.5, a Kerr-Schild spin in [0, 0.95), a random
+    ghosts, dx 0.2..0.5, a Kerr-Schild spin in [0, 0.95) (or, with
+    `binary`, a circular binary of random masses, separation and phase
+    at a random time, with the volume-time sources on), a random
     adiabatic index and limiter, and a perturbed linear primitive field
-    with 5% of its cells at the density and pressure floors."""
+    with 5% of its cells at the density and pressure floors.  Returns
+    (box, background, options, state, time)."""
     rng = np.random.default_rng(1000 + seed)
     n = rng.integers(2, 20, 3)
     lo = rng.integers(25, 40, 3)
     box = BoxSpec(tuple(lo), tuple(lo + n - 1), 3,
                   float(rng.uniform(0.2, 0.5)),
                   tuple(float(v) for v in rng.uniform(-2.0, 2.0, 3)))
-    bg = KerrSchildParams(1.0, float(rng.uniform(0.0, 0.95)))
+    spin = float(rng.uniform(0.0, 0.95))
     gamma = float(rng.choice([4.0 / 3.0, 1.4, 5.0 / 3.0]))
-    opts = OperatorOptions(adiabatic_index=gamma,
-                           limiter_theta=float(rng.choice([1.0, 1.5, 2.0])))
+    theta = float(rng.choice([1.0, 1.5, 2.0]))
+    time = 0.0
+    if binary:
+        m1 = float(rng.uniform(0.3, 0.7))
+        bg = CircularBinaryPNParams(
+            mass_1=m1, mass_2=1.0 - m1,
+            separation=float(rng.uniform(4.0, 10.0)),
+            phase=float(rng.uniform(0.0, 6.0)),
+            softening_radius=float(rng.uniform(0.3, 1.0)))
+        time = float(rng.uniform(0.0, 5.0))
+        opts = OperatorOptions(adiabatic_index=gamma, limiter_theta=theta,
+                               use_metric_volume_time_sources=True,
+                               metric_time_derivative_step=1.0e-3)
+    else:
+        bg = KerrSchildParams(1.0, spin)
+        opts = OperatorOptions(adiabatic_index=gamma, limiter_theta=theta)
     st = reference_test_state(box, bg, gamma)
-    geo = oracle.Oracle(bg, box).cell_geometry()
+    geo = oracle.Oracle(bg, box).cell_geometry(time=time)
     shape = box.storage_shape
     xi = rng.uniform(-1.0, 1.0, (5,) + shape)
     prim = st[5:11].copy()
@@ -191,23 +208,26 @@ def random_box_case(seed):
         pv = prim[(slice(None),) + idx]
         st[(slice(0, 5),) + idx] = oracle.prim2con(pv, gamma, g)
         st[(slice(5, 11),) + idx] = pv
-    return box, bg, opts, st
+    return box, bg, opts, st, time
 
 
 @pytest.mark.parametrize("seed", range(8))
 def test_rhs_random_boxes(hip_context, seed):
-    """Seeded random boxes (sizes, positions, dx, spin, Gamma, limiter,
-    perturbed states with cells at and below the floors; six of the eight
-    take the metric-source atmosphere test somewhere), each with one of
-    the ten face launches pinned: the rhs, mask and ghost-filled state
-    equal the oracle's bit for bit."""
-    box, bg, opts, st0 = random_box_case(seed)
+    """Seeded random boxes (sizes, positions, dx, Gamma, limiter,
+    perturbed states with cells at and below the floors), on Kerr-Schild
+    (seeds 0-3) or on a random circular binary at a random time with the
+    volume-time sources (seeds 4-7, the time-dependent operator), each
+    with one of the ten face launches pinned: the rhs, mask and
+    ghost-filled state equal the oracle's bit for bit."""
+    box, bg, opts, st0, t = random_box_case(seed, binary=seed >= 4)
     walk, fused = FACE_LAUNCHES[seed % len(FACE_LAUNCHES)]
-    O = oracle.Oracle(bg, box, opts)
+    td = seed >= 4
+    O = oracle.Oracle(bg, box, opts, time_dependent=td)
     st_ref = st0.copy()
-    rhs_ref, mask_ref = O.compute_rhs(st_ref, time=0.0, with_mask=True)
-    op = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
-                                              context=hip_context)
+    rhs_ref, mask_ref = O.compute_rhs(st_ref, time=t, with_mask=True)
+    cls = (TimeDependentFixedBGLevelDataFiniteVolumeOperator if td
+           else FixedBGLevelDataFiniteVolumeOperator)
+    op = cls(bg, box, opts, context=hip_context)
     try:
         hip_context.set_face_launch(walk, fused)
         st = to_dev(st0)
@@ -215,7 +235,7 @@ def test_rhs_random_boxes(hip_context, seed):
                           dtype=torch.float64, device="cuda")
         mask = torch.zeros(box.interior_shape, dtype=torch.uint8,
                            device="cuda")
-        op.compute_rhs(rhs, st, source_mask=mask)
+        op.compute_rhs(rhs, st, time=t, source_mask=mask)
         torch.cuda.synchronize()
     finally:
         hip_context.set_face_launch(None)
@@ -354,7 +374,7 @@ def test_rhs_box_short_of_its_domain_face(hip_context, inside):
     edge.  The rhs, mask and ghost-filled state equal the oracle's bit for
     bit."""
     import types
-    box0, bg, opts, st0 = random_box_case(3)
+    box0, bg, opts, st0, _ = random_box_case(3)
     dlo = (box0.lo[0] - inside, box0.lo[1], box0.lo[2])
     box = dataclasses.replace(box0, domain_lo=dlo)
     w = types.SimpleNamespace(background=bg, box=box, options=opts,
@@ -368,7 +388,7 @@ def test_rk4_random_boxes(hip_context, seed):
     test_rhs_random_boxes under the size rule's face launch: state,
     recovery counts and masks equal the oracle's bit for bit."""
     import types
-    box, bg, opts, st0 = random_box_case(seed)
+    box, bg, opts, st0, _ = random_box_case(seed)
     w = types.SimpleNamespace(background=bg, box=box, options=opts,
                               state=st0, dt=0.2 * box.dx)
     _step_case(w, hip_context, "rk4", 2)
