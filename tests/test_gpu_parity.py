@@ -340,6 +340,38 @@ def test_max_inverse_dt(hip_context):
         op.compute_stable_dt(to_dev(st), 0.0)
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_max_inverse_dt_and_ssprk2_random_boxes(hip_context, seed):
+    """The random boxes of test_rhs_random_boxes: max 1/dt equals the
+    oracle's exactly, and one SSPRK2 step at the CFL dt (0.3) gives the
+    oracle's state, counts and masks bit for bit."""
+    import types
+    box, bg, opts, st0, t = random_box_case(seed, binary=seed >= 4)
+    td = seed >= 4
+    cls = (TimeDependentFixedBGLevelDataFiniteVolumeOperator if td
+           else FixedBGLevelDataFiniteVolumeOperator)
+    op = cls(bg, box, opts, context=hip_context)
+    O = oracle.Oracle(bg, box, opts, time_dependent=td)
+    ref = O.compute_max_inverse_dt(st0.copy(), time=t)
+    assert op.compute_max_inverse_dt(to_dev(st0), time=t) == ref
+    dt = O.compute_stable_dt(st0.copy(), 0.3, time=t)
+    w = types.SimpleNamespace(background=bg, box=box, options=opts,
+                              state=st0, dt=dt)
+    if td:
+        st_ref = st0.copy()
+        st = to_dev(st_ref)
+        mask = torch.zeros(box.interior_shape, dtype=torch.uint8,
+                           device="cuda")
+        diag_ref, mask_ref = O.advance_ssprk2(st_ref, dt, time=t)
+        diag = op.advance_ssprk2(st, dt, time=t, mask=mask)
+        sl = (slice(None),) + box.interior_slices()
+        assert_bitwise(st.cpu().numpy()[sl], st_ref[sl], "ssprk2 state")
+        assert diag == diag_ref
+        assert np.array_equal(mask.cpu().numpy(), mask_ref)
+    else:
+        _step_case(w, hip_context, "ssprk2")
+
+
 # ---------------------------------------------------------------- steps
 
 
