@@ -71,6 +71,32 @@ def test_noalloc_rk4_equals_context_rk4(hip_context, first, second,
         assert torch.equal(ma, mb)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_noalloc_random_boxes(hip_context, seed):
+    """The seeded random boxes of the parity tests (2..19 cells a side,
+    random spin, Gamma, limiter, near-floor cells), one after another on
+    the same static array: two RK4 steps through the allocation-free entry
+    equal the context path's state, mask and counts bit for bit."""
+    import types
+    from test_gpu_parity import random_box_case
+    box, bg, opts, st0, _ = random_box_case(seed)
+    w = types.SimpleNamespace(box=box, background=bg, options=opts,
+                              dt=0.2 * box.dx)
+    print("noalloc random box", seed, box.lo, box.hi, flush=True)
+    op = FixedBGLevelDataFiniteVolumeOperator(bg, box, opts,
+                                              context=hip_context)
+    a = torch.from_numpy(st0.copy()).cuda()
+    ma = torch.zeros(box.interior_shape, dtype=torch.uint8, device="cuda")
+    da = op.advance_rk4(a, w.dt, 2, mask=ma)
+    b = torch.from_numpy(st0.copy()).cuda()
+    mb = torch.zeros(box.interior_shape, dtype=torch.uint8, device="cuda")
+    rc, db, _ = noalloc_rk4(w, b, 2, mb)
+    assert rc == abi.GRHD_OK
+    assert db == da
+    assert torch.equal(a, b)
+    assert torch.equal(ma, mb)
+
+
 def test_noalloc_rejects_boxes_beyond_its_scratch(hip_context):
     w = workloads.torus_box(n=16, nsteps=1)
     big = BoxSpec((0, 0, 0), (95, 95, 95), 3, 0.5, (24.0, 24.0, 24.0))
