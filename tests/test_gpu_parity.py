@@ -3,7 +3,7 @@
 Bar: bit for bit.  The rhs, the state (interior after a step; every storage
 cell after an rhs, whose ghost fill mutates it), recovery masks, reset
 counts and dt equal the oracle's exactly: both evaluate the reference's
-formulas in the same order without contraction.  (The north star's bar is
+formulas in the same order without contraction.  (The FP64 path's bar is
 1e-12 relative per conserved variable; bit-equality is stronger.)  On a
 mismatch the message reports the ulp distance and the max-normalised
 error.
